@@ -1,0 +1,175 @@
+"""Throughput of the fictitious-wave-domain FDTD forward step on one MI355X.
+
+Workload (default): 256^3 interior + 16-cell CPML (288^3 = 23,887,872 cells),
+FP64, seeded layered + box conductivity, x-dipole Gaussian-derivative source,
+256 receivers on a 16x16 surface grid, full-volume running DFT of E at 8
+log-spaced frequencies in 0.1-4 Hz, Nt = 8000 samples.  A "step" is one time
+step of that shot: E + CPML-E + source, H + CPML-H, receiver gather, and its
+share of the volume-DFT flush.  Inputs (model, fields, accumulators) are
+resident in HBM before the timed region.
+
+Prints one JSON line:
+  value = Gcell-updates/s (cells incl. CPML x steps / wall time of the K steps),
+  roofline = dominant kernel's algorithmic bytes / its mean launch time (HIP
+             events on the library's stream) against 8 TB/s,
+  cpu_baseline = the CPU oracle (reference loop structure) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s + % HBM peak (256³ fp64); served shots/s, p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def slab_cells(g):
+    nc2 = 2 * g.ncpml
+    return g.iz * g.iy * nc2, g.iz * nc2 * g.ix, nc2 * g.iy * g.ix
+
+
+def kernel_bytes(g, nf, ring):
+    """Algorithmic HBM bytes per launch of each kernel (FP64 = 8 B)."""
+    N = g.ncells
+    sx, sy, sz = slab_cells(g)
+    shell = sx + sy + sz
+    return {
+        # read H(3) + E(3) + eps', write E(3); psi-E: 2 arrays read+write per slab cell
+        "efield": N * (3 * 8 + 3 * 8 + 8 + 3 * 8) + shell * 2 * 16,
+        # read E(3), read+write H(3); eps' in the shells; psi-H 2 arrays read+write
+        "hfield": N * (3 * 8 + 3 * 16) + shell * (8 + 2 * 16),
+        # ring slots of E(3) read once + complex accumulators read+write
+        "flush": N * (ring * 3 * 8 + 3 * nf * 32),
+    }
+
+
+def survey_step_bytes(g, nf, nrec):
+    """B_step = N(6*16+8) + sum_axes 2*ncpml*n^2*64 + N*3*Nf*32 + Nrec*3*Nf*32."""
+    N = g.ncells
+    n2 = g.ix * g.iy
+    return N * (6 * 16 + 8) + 3 * 2 * g.ncpml * n2 * 64 + N * 3 * nf * 32 + nrec * 3 * nf * 32
+
+
+def cpu_baseline(wl, steps):
+    import oracle
+    g = wl.grid
+    S = oracle.OracleSim(g.ix, g.iy, g.iz, g.ncpml, g.dx, g.dy, g.dz, g.dt, g.omega0, g.cmax)
+    S.set_sigma(wl.sigma)
+    S.init_cpml()
+    S.media_coeff_3d()
+    w = oracle.dft_weights(wl.it, g.dt, g.omega0, wl.omegas)
+    acc = np.zeros((3, len(wl.omegas), g.ncells), dtype=np.complex128)
+    traces = np.zeros((3, len(wl.receivers), wl.it))
+    src1 = tuple(c + 1 for c in wl.src)
+    S.forward(0, wl.signal, src1, 1, rec1=wl.receivers + 1, it=wl.it, weights=w, acc=acc,
+              traces=traces)                      # warm-up step
+    t0 = time.perf_counter()
+    S.forward(0, wl.signal, src1, steps, rec1=wl.receivers + 1, it=wl.it, step0=1, weights=w,
+              acc=acc, traces=traces)
+    dt = time.perf_counter() - t0
+    return {"value": g.ncells * steps / dt / 1e9, "unit": "Gcell-updates/s",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{steps} steps of the {wl.name} workload ({g.ix}^3 cells incl. CPML, "
+                      f"{len(wl.omegas)}-frequency running volume DFT, {len(wl.receivers)} "
+                      f"receivers) in the reference's loop structure, after 1 warm-up step; "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--workload", default="roofline_256", choices=["roofline_256", "plain_64"])
+    ap.add_argument("--ring", type=int, default=32)
+    ap.add_argument("--profile-steps", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(shots shard across GPUs by construction; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from mittet_main_amd import Model, configs
+
+    wl = configs.roofline_256() if args.workload == "roofline_256" else configs.plain_64()
+    g = wl.grid
+    nf = len(wl.omegas)
+    need = args.warmup + args.steps + args.profile_steps
+    if need > wl.steps:
+        sys.exit(f"--warmup + --steps + --profile-steps = {need} exceeds the {wl.steps} steps "
+                 f"of the workload")
+    log(f"[bench] workload {wl.name}: {g.ix}x{g.iy}x{g.iz} cells, it={wl.it}, nf={nf}, "
+        f"nrec={len(wl.receivers)}, ring={args.ring}")
+    model = Model(g)
+    model.media_coeff_3d(wl.sigma)
+    shot = model.shot(wl.it, wl.signal, wl.src, wl.src_comp, wl.receivers, wl.omegas,
+                      ring=args.ring)
+    shot.step_n(args.warmup)
+    shot.sync()
+    log(f"[bench] warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    shot.step_n(args.steps)
+    shot.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    log(f"[bench] timed {args.steps} steps: {wall:.3f} s")
+    prof = shot.profile_steps(args.profile_steps)
+    log(f"[bench] per-kernel ms: {prof}")
+
+    kb = kernel_bytes(g, nf, args.ring)
+    per_step = {"efield": prof["efield"], "hfield": prof["hfield"],
+                "flush": prof["flush"] / args.ring}
+    dom = max(per_step, key=per_step.get)
+    dom_ms = prof[dom]
+    achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
+    ms_step = wall / args.steps * 1e3
+    value = g.ncells * args.steps / wall / 1e9
+    sb = survey_step_bytes(g, nf, len(wl.receivers))
+    result = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "interior": g.ix - 2 * g.ncpml, "ncpml": g.ncpml,
+                   "cells": g.ncells, "nt": wl.it, "nf_vol": nf, "nrec": len(wl.receivers),
+                   "ring": args.ring, "sigma_sha256_16": configs.sigma_digest(wl.sigma)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": kb[dom], "ms_per_launch": round(dom_ms, 5)},
+        "kernel_ms": {k: round(v, 5) for k, v in prof.items()},
+        "step_vs_survey_bytes": {"survey_bytes_per_step": sb,
+                                 "equivalent_GBps": round(sb / (ms_step * 1e-3) / 1e9, 1)},
+        "interior_gcells": round((g.ix - 2 * g.ncpml) ** 3 * args.steps / wall / 1e9, 4),
+    }
+    shot.close()
+    model.close()
+    if not args.no_cpu_baseline:
+        log(f"[bench] cpu baseline: {args.cpu_steps} steps of the oracle")
+        result["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+        result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,193 @@
+/*
+ * mittet_fdtd.h -- C ABI of the MI355X fictitious-wave-domain FDTD library
+ * (libmittet_fdtd.so, hand-written HIP for gfx950).
+ *
+ * The reference (satan_i C v1.4) exposes its hot path as void functions over
+ * process globals, e.g. `void e_field_cpml42(double *EX, ..., double *HZ)`
+ * (satan_i/fwi3d_cpu_function_v1.4.h:1-72, fwi3d_cpu_cpmlabc_v1.4.h:1-17,
+ * fwi3d_cpu_lattice_v1.4.h:1).  This ABI keeps the same operator names and
+ * argument meaning, made re-entrant: the reference's globals live in an opaque
+ * model (grid, conductivity, CPML profiles) and shot (fields, psi shells,
+ * receivers, transforms) owned by the library in HBM.
+ *
+ * Conventions
+ *   - every entry point returns an int status (FWI3D_OK == 0, negative on error;
+ *     fwi3d_last_error() gives the message).  Nothing calls exit().
+ *   - cell coordinates are 0-based (i, j, k), x fastest: c = (k*iy + j)*ix + i.
+ *     The reference's 1-based config coordinates map as i = jx-1 (function.c:524).
+ *   - host buffers are caller-owned; device buffers are library-owned.
+ *   - all arithmetic is FP64.  Complex arrays are interleaved (re, im) doubles.
+ *   - work is queued on the stream given at creation (NULL: the library's own
+ *     stream); results are valid after fwi3d_*_sync or a get_* call.
+ *   - no file I/O.
+ */
+#ifndef MITTET_FDTD_H
+#define MITTET_FDTD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FWI3D_OK 0
+#define FWI3D_EINVAL (-1)   /* bad argument or configuration */
+#define FWI3D_ENOMEM (-2)   /* device or host allocation failed */
+#define FWI3D_EDEVICE (-3)  /* HIP runtime error */
+#define FWI3D_ESTATE (-4)   /* call not valid in the object's current state */
+#define FWI3D_EBUDGET (-5)  /* request exceeds the admission budget */
+
+/* Field / trace component ids */
+enum { FWI3D_EX = 0, FWI3D_EY, FWI3D_EZ, FWI3D_HX, FWI3D_HY, FWI3D_HZ };
+
+/* CPML profile ids for fwi3d_model_get_profile (satan_i/fwi3d_cpu_cpmlabc_v1.4.c:40-53) */
+enum {
+    FWI3D_P_ESIG = 0, FWI3D_P_MSIG, FWI3D_P_EKAP, FWI3D_P_MKAP, FWI3D_P_AE, FWI3D_P_AM,
+    FWI3D_P_BE, FWI3D_P_BH, FWI3D_P_CE, FWI3D_P_CH, FWI3D_P_KED, FWI3D_P_KHD,
+    FWI3D_NPROFILES
+};
+
+/* Every scalar the reference hard-codes or reads from data/model_env.dat. */
+typedef struct fwi3d_grid_params {
+    int ix, iy, iz;        /* cells per axis, CPML included */
+    int ncpml;             /* CPML thickness in cells */
+    double dx, dy, dz;     /* m */
+    double dt;             /* s (see fwi3d_lattice_time_3d) */
+    double omega0;         /* fictitious-domain reference angular frequency, rad/s */
+    double mu0;            /* 12.5663706e-7 in the C reference (alloex1.h:3) */
+    double c1, c2;         /* 4th-order staggered coefficients (1.125, (double)-0.04167f) */
+    double cmax;           /* sqrt(2*omega0/mu0/sigma_min): sets the CPML sigma_max */
+    double rcoef;          /* CPML reflection coefficient ((double)0.01f) */
+    double nn, order, ma;  /* grading exponents (3, 0, 1) */
+    double kappa_max;      /* 1 */
+    double alpha_max;      /* (double)0.2f */
+    double opt_to_max[3];  /* per-axis sigma_max factor ({10, 1, 1} in the reference) */
+} fwi3d_grid_params;
+
+/* One forward-modelling shot. */
+typedef struct fwi3d_shot_params {
+    int it;                  /* time samples: the loop runs it-1 steps (rec2rec.c:48) */
+    int src_comp;            /* 0/1/2: electric dipole along x/y/z */
+    int src_i, src_j, src_k; /* 0-based source cell */
+    const double *signal;    /* host, length it: source waveform s[n] */
+    int nrec;                /* receivers */
+    const int *rec_ijk;      /* host, 3*nrec 0-based cells */
+    int nf_vol;              /* volume E running-DFT frequencies (0: off) */
+    const double *omega_vol; /* host, nf_vol angular frequencies (rad/s) */
+    int ring;                /* E history slots between DFT flushes (0: library default) */
+} fwi3d_shot_params;
+
+typedef struct fwi3d_model fwi3d_model;
+typedef struct fwi3d_shot fwi3d_shot;
+typedef struct fwi3d_batch fwi3d_batch;
+
+const char *fwi3d_last_error(void);
+const char *fwi3d_version(void);
+
+/* ---- setup (L1) ------------------------------------------------------- */
+
+/* satan_i/fwi3d_cpu_lattice_v1.4.c:6-54: cmax = sqrt(2*omega0/mu0/minval),
+ * dt = courant*6/7*factor (factor (double)0.999f in the reference). */
+double fwi3d_lattice_time_3d(double minval, double omega0, double mu0,
+                             double dx, double dy, double dz, double factor,
+                             double *cmax_out);
+
+/* Creates the grid, computes the CPML profiles (replaces init_cpml,
+ * satan_i/fwi3d_cpu_cpmlabc_v1.4.c:7-175).  stream: hipStream_t or NULL. */
+int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **out);
+int fwi3d_model_destroy(fwi3d_model *m);
+/* host copy of one 1-D profile (length ix/iy/iz) */
+int fwi3d_model_get_profile(const fwi3d_model *m, int axis, int which, double *out);
+
+/* Uploads sigma (ix*iy*iz, S/m, > 0) and derives eps' = sigma/2/omega0 on the
+ * device.  Replaces media_coeff_3d (satan_i/fwi3d_cpu_function_v1.4.c:325-435):
+ * the 12 coefficient volumes are recomputed in-kernel from eps' and the 1-D
+ * profiles instead of being stored. */
+int fwi3d_media_coeff_3d(fwi3d_model *m, const double *sigma);
+
+/* ---- shots --------------------------------------------------------------- */
+
+int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **out);
+int fwi3d_shot_destroy(fwi3d_shot *s);
+/* device bytes this shot would allocate (admission control) */
+long long fwi3d_shot_bytes(const fwi3d_grid_params *p, const fwi3d_shot_params *sp);
+
+/* satan_i/fwi3d_cpu_function_v1.4.c:964-986 (+ traces, DFT accumulators, step counter) */
+int fwi3d_set_zero_eh(fwi3d_shot *s);
+
+/* ---- per-step operators (L2), same names and order as the reference loop
+ * satan_i/fwi3d_cpu_rec2rec_v1.4.c:49-54.  sign = +1 forward, -1 for the
+ * sign-flipped *_bp variants (function.c:1292-1372). */
+int fwi3d_e_field_cpml42(fwi3d_shot *s, int sign);  /* function.c:1184-1236 */
+int fwi3d_epml_abcs4(fwi3d_shot *s);                /* cpmlabc.c:191-196 */
+int fwi3d_read_source_3d(fwi3d_shot *s, int step);  /* function.c:514-529 */
+int fwi3d_h_field_cpml42(fwi3d_shot *s, int sign);  /* function.c:1238-1290 */
+int fwi3d_hpml_abcs4(fwi3d_shot *s);                /* cpmlabc.c:198-203 */
+int fwi3d_output(fwi3d_shot *s, int step);          /* function.c:765-785 (traces) */
+
+/* Fused steps: E+CPML-E+source, H+CPML-H, receivers, volume DFT; advances the
+ * shot's own step counter by nsteps (bounded by it-1). */
+int fwi3d_step_n(fwi3d_shot *s, int nsteps);
+/* the whole it-1 step loop of one shot (rec2rec.c:48-57), graph-captured */
+int fwi3d_run(fwi3d_shot *s);
+int fwi3d_shot_sync(fwi3d_shot *s);
+/* Runs nsteps fused steps with HIP events around every launch on the shot's
+ * stream and returns the mean duration in ms of each kernel kind:
+ * ms[0] E+CPML-E+source, ms[1] H+CPML-H, ms[2] receiver gather,
+ * ms[3] volume-DFT flush (per flush launch), ms[4] whole step (per step).
+ * Advances the shot like fwi3d_step_n.  No counterpart in the reference
+ * (its timing is a clock() around main, rec2rec.c:18,65). */
+int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms);
+int fwi3d_shot_step(const fwi3d_shot *s); /* steps done so far */
+
+/* ---- readback -------------------------------------------------------------- */
+
+/* which: FWI3D_EX..FWI3D_HZ; out: ix*iy*iz doubles */
+int fwi3d_get_field(fwi3d_shot *s, int which, double *out);
+int fwi3d_set_field(fwi3d_shot *s, int which, const double *in);
+/* traces: [6][nrec][it] doubles, E then H, sample n = after step n */
+int fwi3d_get_traces(fwi3d_shot *s, double *out);
+/* volume E spectra: complex [3][nf_vol][ix*iy*iz] = sum_n E[n]*w_n(omega) */
+int fwi3d_get_volume_dft(fwi3d_shot *s, double *out);
+
+/* ---- fictitious -> diffusive transform (L3) ------------------------------- */
+
+/* Receiver spectra at arbitrary angular frequencies, one wavefront reduction
+ * per (component, receiver, frequency):
+ *   X(omega) = sum_{n<it} trace[n] * dt*exp(-a n dt)*exp(i a n dt), a = sqrt(omega0*omega).
+ * comps: bit mask of FWI3D_EX..FWI3D_HZ; out complex [6][nrec][nfreq] (unused comps 0).
+ * Magnetic components get the f_to_d_h factor sqrt(-2*omega0/(i*omega))
+ * (ver0040/cpu_f_to_d_h.f90:146-148) when h_factor != 0. */
+int fwi3d_receiver_dft(fwi3d_shot *s, int nfreq, const double *omegas, int h_factor,
+                       double *out);
+/* f_to_d_e / f_to_d_h on already-accumulated spectra (host arrays, complex):
+ * E passes through; H is multiplied by sqrt(-2*omega0/(i*omega)). */
+int fwi3d_f_to_d_h(double omega0, int nfreq, const double *omegas, long long count,
+                   double *spectra);
+/* Source spectrum of the waveform at the same frequencies with the electric-source
+ * factor: J(omega) = csqrt(-2*omega0/(i*omega)) * sum J[n] w_n, J(0) := 2*omega0
+ * (satan_i/fwi3d_cpu_function_v1.4.c:2044-2050).  out complex [nfreq]. */
+int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal,
+                          int nfreq, const double *omegas, double *out);
+/* satan_i/fwi3d_cpu_function_v1.4.c:1976-2089 (laplaceToFreq) for every receiver
+ * component of the shot at bins omega_n = 2*pi*n/(it*dt), n < nbins:
+ * E_n, G_n = E_n/J_n and OBS_n = G_n * FFTW_BACKWARD(strue)[n].
+ * e_w, g_w, obs: complex [3][nrec][nbins]; j_w: complex [nbins]. */
+int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue,
+                          double *e_w, double *j_w, double *g_w, double *obs);
+
+/* ---- multi-shot serving (persistent multi-shot launches) ------------------ */
+
+/* A batch advances all member shots (any models/grid sizes) with one launch per
+ * half-step.  Shots join and leave between chunks; each keeps its own step. */
+int fwi3d_batch_create(int max_shots, int chunk, void *stream, fwi3d_batch **out);
+int fwi3d_batch_destroy(fwi3d_batch *b);
+int fwi3d_batch_add(fwi3d_batch *b, fwi3d_shot *s);
+int fwi3d_batch_remove(fwi3d_batch *b, fwi3d_shot *s);
+/* advances every member by up to nsteps (finished shots idle) */
+int fwi3d_batch_step(fwi3d_batch *b, int nsteps);
+int fwi3d_batch_sync(fwi3d_batch *b);
+int fwi3d_batch_size(const fwi3d_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1114 @@
+// fdtd_api.hip -- host side of libmittet_fdtd.so: the C ABI declared in
+// include/mittet_fdtd.h.  Owns the model (grid, eps', CPML profiles), shots
+// (fields, psi shells, receivers, DFT accumulators) and batches in HBM, and
+// drives the kernels of fdtd_kernels.hip on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mittet_fdtd.h"
+#include "fdtd_dev.h"
+
+using mt::ShotDev;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(FWI3D_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                            \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        return hipSuccess;
+    return hipMalloc((void **)p, count * sizeof(T));
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p)
+        (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int axis_len(const fwi3d_grid_params &p, int a) { return a == 0 ? p.ix : (a == 1 ? p.iy : p.iz); }
+double axis_step(const fwi3d_grid_params &p, int a) { return a == 0 ? p.dx : (a == 1 ? p.dy : p.dz); }
+
+// CPML profiles of satan_i/fwi3d_cpu_cpmlabc_v1.4.c:7-175 for one axis:
+// sigma graded with ((offset)/(nc-1))^(nn+order) from sigma_max
+// = (nn+order+1)*cmax*ln(1/R)/(2*nc*h)*factor, kappa graded with exponent nn,
+// alpha linear from 0 at the outer edge to alpha_max, and the recursive
+// convolution constants b = exp(-(sigma/kappa+alpha)dt),
+// c = sigma*(b-1)/(sigma+kappa*alpha)/kappa.  E quantities sit on integer
+// offsets, H quantities half a cell inward; offsets are formed in single
+// precision as the reference does, so the outermost cell exceeds sigma_max.
+void cpml_axis(const fwi3d_grid_params &p, int a, std::vector<double> &out)
+{
+    const int n = axis_len(p, a), nc = p.ncpml;
+    const double h = axis_step(p, a);
+    out.assign((size_t)FWI3D_NPROFILES * n, 0.0);
+    auto at = [&](int row, int i) -> double & { return out[(size_t)row * n + i]; };
+    const double sig_max = (p.nn + p.order + 1.0) * p.cmax * std::log(1.0 / p.rcoef) /
+                           (2.0 * (nc * h)) * p.opt_to_max[a];
+    const double den = (double)((float)nc - 1.0f);
+    for (int i = 0; i < n; ++i) {
+        bool shell = true;
+        double oe = 0, om = 0, ae = 0, am = 0;  // E/H offsets for sigma/kappa and alpha
+        if (i < nc) {
+            oe = (double)(nc - i);
+            om = (double)((float)(nc - i) - 0.5f);
+            ae = (double)i;
+            am = (double)((float)i + 0.5f);
+        } else if (i >= n - nc) {
+            oe = (double)((float)(i - n) + 1.0f + (float)nc);
+            om = (double)((float)(i - n) + 0.5f + (float)nc);
+            ae = (double)((float)(n - i) - 1.0f);
+            am = (double)((float)(n - i) - 0.5f);
+        } else {
+            shell = false;
+        }
+        if (shell) {
+            const double es = sig_max * std::pow(oe / den, p.nn + p.order);
+            const double ms = sig_max * std::pow(om / den, p.nn + p.order);
+            const double ek = 1.0 + (p.kappa_max - 1.0) * std::pow(oe / den, p.nn);
+            const double mk = 1.0 + (p.kappa_max - 1.0) * std::pow(om / den, p.nn);
+            const double ea = p.alpha_max * std::pow(ae / den, p.ma);
+            const double ma = p.alpha_max * std::pow(am / den, p.ma);
+            const double be = std::exp(-(es / ek + ea) * p.dt);
+            const double bh = std::exp(-(ms / mk + ma) * p.dt);
+            at(FWI3D_P_ESIG, i) = es;
+            at(FWI3D_P_MSIG, i) = ms;
+            at(FWI3D_P_EKAP, i) = ek;
+            at(FWI3D_P_MKAP, i) = mk;
+            at(FWI3D_P_AE, i) = ea;
+            at(FWI3D_P_AM, i) = ma;
+            at(FWI3D_P_BE, i) = be;
+            at(FWI3D_P_BH, i) = bh;
+            at(FWI3D_P_CE, i) = es * (be - 1.0) / (es + ek * ea) / ek;
+            at(FWI3D_P_CH, i) = ms * (bh - 1.0) / (ms + mk * ma) / mk;
+        } else {
+            at(FWI3D_P_EKAP, i) = 1.0;
+            at(FWI3D_P_MKAP, i) = 1.0;
+        }
+        at(FWI3D_P_KED, i) = at(FWI3D_P_EKAP, i) * h;
+        at(FWI3D_P_KHD, i) = at(FWI3D_P_MKAP, i) * h;
+    }
+}
+
+// host rows (FWI3D_P_*) -> device rows (mt::PR_*)
+void pack_device_profile(const std::vector<double> &hp, int n, std::vector<double> &dp)
+{
+    static const int rows[mt::PR_N] = {FWI3D_P_ESIG, FWI3D_P_MSIG, FWI3D_P_BE, FWI3D_P_BH,
+                                       FWI3D_P_CE,   FWI3D_P_CH,   FWI3D_P_KED, FWI3D_P_KHD};
+    dp.assign((size_t)mt::PR_N * n, 0.0);
+    for (int r = 0; r < mt::PR_N; ++r)
+        std::memcpy(&dp[(size_t)r * n], &hp[(size_t)rows[r] * n], sizeof(double) * n);
+}
+
+int tiles(int n, int t) { return (n + t - 1) / t; }
+
+constexpr int kDefaultChunk = 32;
+
+}  // namespace
+
+struct fwi3d_model {
+    fwi3d_grid_params p;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<double> prof[3];
+    double *d_prof[3] = {nullptr, nullptr, nullptr};
+    double *d_sig = nullptr;
+    double *d_eps = nullptr;
+    std::vector<double> sig;  // host copy (source scaling)
+    int version = 0;          // bumps on every media_coeff_3d
+    int nshots = 0;
+};
+
+struct fwi3d_shot {
+    fwi3d_model *m = nullptr;
+    hipStream_t st = nullptr;
+    int it = 0, nrec = 0, nf = 0, ring = 1;
+    int src_comp = 0, src_cell = -1;
+    std::vector<double> signal, omega;
+    std::vector<int> rec;
+    double *d_e = nullptr, *d_h = nullptr, *d_psi[3] = {nullptr, nullptr, nullptr};
+    double *d_amp = nullptr, *d_traces = nullptr, *d_w = nullptr, *d_acc = nullptr;
+    int *d_rec = nullptr, *d_step = nullptr, *d_pre = nullptr;
+    ShotDev desc{};
+    ShotDev *d_desc = nullptr;
+    bool desc_dirty = true;
+    int nblocks = 0, ncellblocks = 0;
+    int step = 0;          // steps done (host mirror of *d_step)
+    int amp_version = -1;
+    hipGraphExec_t graph = nullptr;
+    fwi3d_batch *batch = nullptr;
+};
+
+struct fwi3d_batch {
+    int max_shots = 0, chunk = kDefaultChunk;
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    std::vector<fwi3d_shot *> shots;
+    ShotDev *d_desc = nullptr;
+    int *d_pre = nullptr, *d_pre_cells = nullptr;
+    int nblocks = 0, ncellblocks = 0;
+    bool dirty = true;
+    hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+int upload_desc(fwi3d_shot *s)
+{
+    HIP_TRY(hipMemcpyAsync(s->d_desc, &s->desc, sizeof(ShotDev), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->desc_dirty = false;
+    return FWI3D_OK;
+}
+
+// source decrement per step, satan_i/fwi3d_cpu_function_v1.4.c:524-527
+int ensure_amp(fwi3d_shot *s)
+{
+    fwi3d_model *m = s->m;
+    if (m->version == 0)
+        return fail(FWI3D_ESTATE, "fwi3d_media_coeff_3d has not been called on the model");
+    if (s->amp_version == m->version || s->src_cell < 0)
+        return FWI3D_OK;
+    const fwi3d_grid_params &p = m->p;
+    std::vector<double> amp((size_t)std::max(s->it - 1, 1), 0.0);
+    const double sg = m->sig[(size_t)s->src_cell];
+    for (int n = 0; n + 1 < s->it; ++n)
+        amp[n] = (s->signal[n] + s->signal[n + 1]) / 2.0 * p.dt * 2.0 * p.omega0 / sg / p.dx /
+                 p.dy / p.dz;
+    HIP_TRY(hipMemcpyAsync(s->d_amp, amp.data(), sizeof(double) * amp.size(),
+                           hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->amp_version = m->version;
+    return FWI3D_OK;
+}
+
+int check_shot_ready(fwi3d_shot *s)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    int rc = ensure_amp(s);
+    if (rc)
+        return rc;
+    if (s->desc_dirty && (rc = upload_desc(s)))
+        return rc;
+    return FWI3D_OK;
+}
+
+// launches for local steps j in [0, L) followed by the DFT flush and step advance
+int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const int *pre_cells,
+                 int ncellblocks, int L, bool any_dft, bool any_rec, hipStream_t st)
+{
+    for (int j = 0; j < L; ++j) {
+        HIP_TRY(mt::launch_efield(mt::M_UPD | mt::M_PSI | mt::M_SRC, d, ns, pre, nblocks, j, -1,
+                                  1, st));
+        HIP_TRY(mt::launch_hfield(mt::M_UPD | mt::M_PSI, d, ns, pre, nblocks, j, 1, st));
+        if (any_rec)
+            HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
+    }
+    if (any_dft)
+        HIP_TRY(mt::launch_flush(d, ns, pre_cells, ncellblocks, L, st));
+    HIP_TRY(mt::launch_advance(d, ns, L, st));
+    return FWI3D_OK;
+}
+
+int capture_chunk(hipGraphExec_t *out, const ShotDev *d, int ns, const int *pre, int nblocks,
+                  const int *pre_cells, int ncellblocks, int L, bool any_dft, bool any_rec,
+                  hipStream_t st)
+{
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = launch_chunk(d, ns, pre, nblocks, pre_cells, ncellblocks, L, any_dft, any_rec, st);
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc)
+        return rc;
+    if (e != hipSuccess)
+        return fail(FWI3D_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+        return fail(FWI3D_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return FWI3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fwi3d_last_error(void) { return g_err.c_str(); }
+const char *fwi3d_version(void) { return "mittet_fdtd 0.1 gfx950"; }
+
+double fwi3d_lattice_time_3d(double minval, double omega0, double mu0, double dx, double dy,
+                             double dz, double factor, double *cmax_out)
+{
+    const double cmax = std::sqrt(2.0 * omega0 / mu0 / minval);
+    const double courant =
+        1.0 / cmax / std::sqrt(1.0 / std::pow(dx, 2) + 1.0 / std::pow(dy, 2) + 1.0 / std::pow(dz, 2));
+    if (cmax_out)
+        *cmax_out = cmax;
+    return courant * 6.0 / 7.0 * factor;
+}
+
+int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **out)
+{
+    if (!p || !out)
+        return fail(FWI3D_EINVAL, "null argument");
+    *out = nullptr;
+    if (p->ncpml < 2)
+        return fail(FWI3D_EINVAL, "ncpml must be >= 2 (got %d)", p->ncpml);
+    for (int a = 0; a < 3; ++a) {
+        const int n = axis_len(*p, a);
+        if (n < 2 * p->ncpml + 4)
+            return fail(FWI3D_EINVAL, "axis %d: %d cells cannot hold two %d-cell CPML shells", a,
+                        n, p->ncpml);
+        if (!(axis_step(*p, a) > 0))
+            return fail(FWI3D_EINVAL, "grid step of axis %d must be > 0", a);
+    }
+    if ((long long)p->ix * p->iy * p->iz >= (1LL << 31))
+        return fail(FWI3D_EINVAL, "grid of %lld cells exceeds 2^31", (long long)p->ix * p->iy * p->iz);
+    if (!(p->dt > 0) || !(p->omega0 > 0) || !(p->mu0 > 0) || !(p->cmax > 0))
+        return fail(FWI3D_EINVAL, "dt, omega0, mu0 and cmax must be > 0");
+    fwi3d_model *m = new fwi3d_model();
+    m->p = *p;
+    if (stream) {
+        m->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete m;
+            return fail(FWI3D_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+        }
+        m->own_stream = true;
+    }
+    for (int a = 0; a < 3; ++a) {
+        cpml_axis(*p, a, m->prof[a]);
+        std::vector<double> dp;
+        const int n = axis_len(*p, a);
+        pack_device_profile(m->prof[a], n, dp);
+        if (dalloc(&m->d_prof[a], dp.size()) != hipSuccess ||
+            hipMemcpy(m->d_prof[a], dp.data(), sizeof(double) * dp.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            fwi3d_model_destroy(m);
+            return fail(FWI3D_ENOMEM, "profile upload failed");
+        }
+    }
+    const size_t N = (size_t)p->ix * p->iy * p->iz;
+    if (dalloc(&m->d_sig, N) != hipSuccess || dalloc(&m->d_eps, N) != hipSuccess) {
+        fwi3d_model_destroy(m);
+        return fail(FWI3D_ENOMEM, "cannot allocate %zu-cell model", N);
+    }
+    *out = m;
+    return FWI3D_OK;
+}
+
+int fwi3d_model_destroy(fwi3d_model *m)
+{
+    if (!m)
+        return FWI3D_OK;
+    if (m->nshots > 0)
+        return fail(FWI3D_ESTATE, "model still has %d shots", m->nshots);
+    for (auto &q : m->d_prof)
+        dfree(q);
+    dfree(m->d_sig);
+    dfree(m->d_eps);
+    if (m->own_stream && m->stream)
+        (void)hipStreamDestroy(m->stream);
+    delete m;
+    return FWI3D_OK;
+}
+
+int fwi3d_model_get_profile(const fwi3d_model *m, int axis, int which, double *out)
+{
+    if (!m || !out || axis < 0 || axis > 2 || which < 0 || which >= FWI3D_NPROFILES)
+        return fail(FWI3D_EINVAL, "bad profile request");
+    const int n = axis_len(m->p, axis);
+    std::memcpy(out, &m->prof[axis][(size_t)which * n], sizeof(double) * n);
+    return FWI3D_OK;
+}
+
+int fwi3d_media_coeff_3d(fwi3d_model *m, const double *sigma)
+{
+    if (!m || !sigma)
+        return fail(FWI3D_EINVAL, "null argument");
+    const size_t N = (size_t)m->p.ix * m->p.iy * m->p.iz;
+    for (size_t c = 0; c < N; ++c)
+        if (!(sigma[c] > 0) || !std::isfinite(sigma[c]))
+            return fail(FWI3D_EINVAL, "sigma[%zu] = %g: conductivity must be finite and > 0", c,
+                        sigma[c]);
+    m->sig.assign(sigma, sigma + N);
+    HIP_TRY(hipMemcpyAsync(m->d_sig, sigma, sizeof(double) * N, hipMemcpyHostToDevice, m->stream));
+    HIP_TRY(mt::launch_eps(m->d_sig, m->d_eps, (int)N, m->p.omega0, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    m->version++;
+    return FWI3D_OK;
+}
+
+long long fwi3d_shot_bytes(const fwi3d_grid_params *p, const fwi3d_shot_params *sp)
+{
+    if (!p || !sp)
+        return -1;
+    const long long N = (long long)p->ix * p->iy * p->iz;
+    const long long nc2 = 2LL * p->ncpml;
+    const int ring = sp->nf_vol > 0 ? (sp->ring > 0 ? sp->ring : kDefaultChunk) : 1;
+    long long b = 8LL * (3LL * N * ring + 3LL * N);                          // E ring, H
+    b += 8LL * 4 * ((long long)p->iz * p->iy * nc2 + (long long)p->iz * nc2 * p->ix +
+                    nc2 * p->iy * p->ix);                                     // psi slabs
+    b += 8LL * sp->it + 8LL * 6 * sp->nrec * (long long)sp->it + 4LL * sp->nrec;
+    b += 16LL * sp->nf_vol * (long long)sp->it + 16LL * 3 * sp->nf_vol * N;  // weights, acc
+    return b;
+}
+
+int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **out)
+{
+    if (!m || !sp || !out)
+        return fail(FWI3D_EINVAL, "null argument");
+    *out = nullptr;
+    const fwi3d_grid_params &p = m->p;
+    if (sp->it < 2)
+        return fail(FWI3D_EINVAL, "it must be >= 2");
+    if (sp->src_comp < 0 || sp->src_comp > 2)
+        return fail(FWI3D_EINVAL, "src_comp must be 0, 1 or 2");
+    const bool has_src = sp->signal != nullptr;
+    if (has_src && (sp->src_i < 0 || sp->src_i >= p.ix || sp->src_j < 0 || sp->src_j >= p.iy ||
+                    sp->src_k < 0 || sp->src_k >= p.iz))
+        return fail(FWI3D_EINVAL, "source cell (%d,%d,%d) outside the %dx%dx%d grid", sp->src_i,
+                    sp->src_j, sp->src_k, p.ix, p.iy, p.iz);
+    if (sp->nrec < 0 || (sp->nrec > 0 && !sp->rec_ijk))
+        return fail(FWI3D_EINVAL, "bad receiver list");
+    for (int r = 0; r < sp->nrec; ++r) {
+        const int *q = sp->rec_ijk + 3 * r;
+        if (q[0] < 0 || q[0] >= p.ix || q[1] < 0 || q[1] >= p.iy || q[2] < 0 || q[2] >= p.iz)
+            return fail(FWI3D_EINVAL, "receiver %d at (%d,%d,%d) outside the grid", r, q[0], q[1],
+                        q[2]);
+    }
+    if (sp->nf_vol < 0 || (sp->nf_vol > 0 && !sp->omega_vol))
+        return fail(FWI3D_EINVAL, "bad volume DFT frequency list");
+    for (int f = 0; f < sp->nf_vol; ++f)
+        if (!(sp->omega_vol[f] >= 0) || !std::isfinite(sp->omega_vol[f]))
+            return fail(FWI3D_EINVAL, "omega_vol[%d] must be finite and >= 0", f);
+    int ring = 1;
+    if (sp->nf_vol > 0) {
+        ring = sp->ring > 0 ? sp->ring : kDefaultChunk;
+        if (ring > mt::RING_MAX)
+            return fail(FWI3D_EINVAL, "ring %d exceeds %d", ring, mt::RING_MAX);
+    }
+
+    fwi3d_shot *s = new fwi3d_shot();
+    s->m = m;
+    s->st = m->stream;
+    s->it = sp->it;
+    s->nrec = sp->nrec;
+    s->nf = sp->nf_vol;
+    s->ring = ring;
+    s->src_comp = sp->src_comp;
+    if (has_src) {
+        s->src_cell = (sp->src_k * p.iy + sp->src_j) * p.ix + sp->src_i;
+        s->signal.assign(sp->signal, sp->signal + sp->it);
+    }
+    for (int r = 0; r < sp->nrec; ++r) {
+        const int *q = sp->rec_ijk + 3 * r;
+        s->rec.push_back((q[2] * p.iy + q[1]) * p.ix + q[0]);
+    }
+    if (s->nf)
+        s->omega.assign(sp->omega_vol, sp->omega_vol + s->nf);
+
+    const size_t N = (size_t)p.ix * p.iy * p.iz;
+    const size_t nc2 = 2 * (size_t)p.ncpml;
+    const size_t slab[3] = {(size_t)p.iz * p.iy * nc2, (size_t)p.iz * nc2 * p.ix,
+                            nc2 * p.iy * p.ix};
+    bool ok = dalloc(&s->d_e, 3 * N * ring) == hipSuccess &&
+              dalloc(&s->d_h, 3 * N) == hipSuccess &&
+              dalloc(&s->d_psi[0], mt::PSI_N * slab[0]) == hipSuccess &&
+              dalloc(&s->d_psi[1], mt::PSI_N * slab[1]) == hipSuccess &&
+              dalloc(&s->d_psi[2], mt::PSI_N * slab[2]) == hipSuccess &&
+              dalloc(&s->d_amp, (size_t)sp->it) == hipSuccess &&
+              dalloc(&s->d_traces, std::max<size_t>(1, 6 * (size_t)sp->nrec * sp->it)) == hipSuccess &&
+              dalloc(&s->d_rec, std::max<size_t>(1, (size_t)sp->nrec)) == hipSuccess &&
+              dalloc(&s->d_step, 1) == hipSuccess && dalloc(&s->d_pre, 2) == hipSuccess &&
+              dalloc(&s->d_desc, 1) == hipSuccess;
+    if (ok && s->nf)
+        ok = dalloc(&s->d_w, 2 * (size_t)s->nf * sp->it) == hipSuccess &&
+             dalloc(&s->d_acc, 2 * 3 * (size_t)s->nf * N) == hipSuccess;
+    if (!ok) {
+        m->nshots++;
+        fwi3d_shot_destroy(s);
+        return fail(FWI3D_ENOMEM, "device allocation failed for a %zu-cell shot (%lld bytes)", N,
+                    fwi3d_shot_bytes(&p, sp));
+    }
+    m->nshots++;
+
+    int rc = FWI3D_OK;
+    const int zero2[2] = {0, 0};
+    if (hipMemcpy(s->d_pre, zero2, sizeof zero2, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(FWI3D_EDEVICE, "prefix upload failed");
+    if (!rc && sp->nrec &&
+        hipMemcpy(s->d_rec, s->rec.data(), sizeof(int) * sp->nrec, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(FWI3D_EDEVICE, "receiver upload failed");
+    if (!rc && s->nf) {
+        // w[n][f] = dt*exp(-a n dt)*(cos, sin)(a n dt), a = sqrt(omega0*omega_f)
+        std::vector<double> w(2 * (size_t)s->nf * sp->it);
+        for (int f = 0; f < s->nf; ++f) {
+            const double a = std::sqrt(p.omega0 * s->omega[f]);
+            for (int n = 0; n < sp->it; ++n) {
+                const double x = a * n * p.dt;
+                const double e = p.dt * std::exp(-x);
+                w[2 * ((size_t)n * s->nf + f)] = e * std::cos(x);
+                w[2 * ((size_t)n * s->nf + f) + 1] = e * std::sin(x);
+            }
+        }
+        if (hipMemcpy(s->d_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(FWI3D_EDEVICE, "weight upload failed");
+    }
+    if (rc) {
+        fwi3d_shot_destroy(s);
+        return rc;
+    }
+
+    ShotDev &d = s->desc;
+    d.ix = p.ix;
+    d.iy = p.iy;
+    d.iz = p.iz;
+    d.nc = p.ncpml;
+    d.n = (int)N;
+    d.nbx = tiles(p.ix, mt::TX);
+    d.nby = tiles(p.iy, mt::TY);
+    d.it = sp->it;
+    d.dt = p.dt;
+    d.omega0 = p.omega0;
+    d.mu0 = p.mu0;
+    d.c1 = p.c1;
+    d.c2 = p.c2;
+    d.dl[0] = p.dx;
+    d.dl[1] = p.dy;
+    d.dl[2] = p.dz;
+    for (int a = 0; a < 3; ++a) {
+        d.prof[a] = m->d_prof[a];
+        d.psi[a] = s->d_psi[a];
+        d.slab[a] = (long long)slab[a];
+    }
+    d.eps = m->d_eps;
+    d.e = s->d_e;
+    d.ring = ring;
+    d.cur = 0;
+    d.h = s->d_h;
+    d.src_comp = s->src_comp;
+    d.src_cell = s->src_cell;
+    d.amp = s->d_amp;
+    d.nrec = s->nrec;
+    d.rec_cell = s->d_rec;
+    d.traces = s->d_traces;
+    d.nf = s->nf;
+    d.w = s->d_w;
+    d.acc = s->d_acc;
+    d.step = s->d_step;
+    s->nblocks = d.nbx * d.nby * p.iz;
+    s->ncellblocks = tiles((int)N, 256);
+    if ((rc = upload_desc(s)) || (rc = fwi3d_set_zero_eh(s))) {
+        fwi3d_shot_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return FWI3D_OK;
+}
+
+int fwi3d_shot_destroy(fwi3d_shot *s)
+{
+    if (!s)
+        return FWI3D_OK;
+    if (s->batch)
+        fwi3d_batch_remove(s->batch, s);
+    if (s->st)
+        (void)hipStreamSynchronize(s->st);
+    if (s->graph)
+        (void)hipGraphExecDestroy(s->graph);
+    dfree(s->d_e);
+    dfree(s->d_h);
+    for (auto &q : s->d_psi)
+        dfree(q);
+    dfree(s->d_amp);
+    dfree(s->d_traces);
+    dfree(s->d_w);
+    dfree(s->d_acc);
+    dfree(s->d_rec);
+    dfree(s->d_step);
+    dfree(s->d_pre);
+    dfree(s->d_desc);
+    if (s->m)
+        s->m->nshots--;
+    delete s;
+    return FWI3D_OK;
+}
+
+int fwi3d_set_zero_eh(fwi3d_shot *s)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot is in a batch");
+    const fwi3d_grid_params &p = s->m->p;
+    const size_t N = (size_t)p.ix * p.iy * p.iz;
+    HIP_TRY(mt::launch_fill(s->d_e, 3 * N * s->ring, 0.0, s->st));
+    HIP_TRY(mt::launch_fill(s->d_h, 3 * N, 0.0, s->st));
+    for (int a = 0; a < 3; ++a)
+        HIP_TRY(mt::launch_fill(s->d_psi[a], mt::PSI_N * (size_t)s->desc.slab[a], 0.0, s->st));
+    HIP_TRY(mt::launch_fill(s->d_traces, std::max<size_t>(1, 6 * (size_t)s->nrec * s->it), 0.0, s->st));
+    if (s->nf)
+        HIP_TRY(mt::launch_fill(s->d_acc, 2 * 3 * (size_t)s->nf * N, 0.0, s->st));
+    static const int kZero = 0;
+    HIP_TRY(hipMemcpyAsync(s->d_step, &kZero, sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->step = 0;
+    if (s->desc.cur != 0) {
+        s->desc.cur = 0;
+        s->desc_dirty = true;
+    }
+    return FWI3D_OK;
+}
+
+static int single_op(fwi3d_shot *s, int which, int sign, int step)
+{
+    int rc = check_shot_ready(s);
+    if (rc)
+        return rc;
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot is in a batch");
+    if (sign != 1 && sign != -1)
+        return fail(FWI3D_EINVAL, "sign must be +1 or -1");
+    switch (which) {
+    case 0:
+        HIP_TRY(mt::launch_efield(mt::M_UPD | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+                                  0, sign, s->st));
+        break;
+    case 1:
+        HIP_TRY(mt::launch_efield(mt::M_PSI | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+                                  0, 1, s->st));
+        break;
+    case 2:
+        HIP_TRY(mt::launch_hfield(mt::M_UPD | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+                                  sign, s->st));
+        break;
+    case 3:
+        HIP_TRY(mt::launch_hfield(mt::M_PSI | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+                                  1, s->st));
+        break;
+    case 4:
+        if (step < 0 || step >= s->it - 1)
+            return fail(FWI3D_EINVAL, "step %d outside [0, %d)", step, s->it - 1);
+        HIP_TRY(mt::launch_source(s->d_desc, step, s->st));
+        break;
+    case 5:
+        if (step < 0 || step >= s->it - 1)
+            return fail(FWI3D_EINVAL, "step %d outside [0, %d)", step, s->it - 1);
+        HIP_TRY(mt::launch_receivers(s->d_desc, 1, 0, step, 1, s->st));
+        break;
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_e_field_cpml42(fwi3d_shot *s, int sign) { return single_op(s, 0, sign, 0); }
+int fwi3d_epml_abcs4(fwi3d_shot *s) { return single_op(s, 1, 1, 0); }
+int fwi3d_h_field_cpml42(fwi3d_shot *s, int sign) { return single_op(s, 2, sign, 0); }
+int fwi3d_hpml_abcs4(fwi3d_shot *s) { return single_op(s, 3, 1, 0); }
+int fwi3d_read_source_3d(fwi3d_shot *s, int step) { return single_op(s, 4, 1, step); }
+int fwi3d_output(fwi3d_shot *s, int step) { return single_op(s, 5, 1, step); }
+
+int fwi3d_step_n(fwi3d_shot *s, int nsteps)
+{
+    int rc = check_shot_ready(s);
+    if (rc)
+        return rc;
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot is in a batch");
+    if (nsteps < 0)
+        return fail(FWI3D_EINVAL, "nsteps < 0");
+    const int chunk = s->nf ? s->ring : kDefaultChunk;
+    const bool dft = s->nf > 0, rec = s->nrec > 0;
+    int left = std::min(nsteps, s->it - 1 - s->step);
+    while (left > 0) {
+        const int L = std::min(chunk, left);
+        if (L == chunk) {
+            if (!s->graph &&
+                (rc = capture_chunk(&s->graph, s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre,
+                                    s->ncellblocks, L, dft, rec, s->st)))
+                return rc;
+            HIP_TRY(hipGraphLaunch(s->graph, s->st));
+        } else {
+            if ((rc = launch_chunk(s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre, s->ncellblocks, L,
+                                   dft, rec, s->st)))
+                return rc;
+        }
+        s->step += L;
+        left -= L;
+        if (s->ring > 1 && L % s->ring) {
+            s->desc.cur = (s->desc.cur + L) % s->ring;
+            if ((rc = upload_desc(s)))
+                return rc;
+        }
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
+{
+    int rc = check_shot_ready(s);
+    if (rc)
+        return rc;
+    if (!ms || nsteps <= 0)
+        return fail(FWI3D_EINVAL, "bad profile request");
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot is in a batch");
+    const int chunk = s->nf ? s->ring : kDefaultChunk;
+    hipEvent_t ev[2];
+    HIP_TRY(hipEventCreate(&ev[0]));
+    HIP_TRY(hipEventCreate(&ev[1]));
+    double acc[5] = {0, 0, 0, 0, 0};
+    int cnt[5] = {0, 0, 0, 0, 0};
+    auto timed = [&](int slot, auto &&launch) -> int {
+        HIP_TRY(hipEventRecord(ev[0], s->st));
+        HIP_TRY(launch());
+        HIP_TRY(hipEventRecord(ev[1], s->st));
+        HIP_TRY(hipEventSynchronize(ev[1]));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+        acc[slot] += t;
+        cnt[slot] += 1;
+        return FWI3D_OK;
+    };
+    int left = std::min(nsteps, s->it - 1 - s->step);
+    const int steps_run = left;
+    while (left > 0 && !rc) {
+        const int L = std::min(chunk, left);
+        for (int j = 0; j < L && !rc; ++j) {
+            rc = timed(0, [&] { return mt::launch_efield(mt::M_UPD | mt::M_PSI | mt::M_SRC, s->d_desc, 1,
+                                                         s->d_pre, s->nblocks, j, -1, 1, s->st); });
+            if (!rc)
+                rc = timed(1, [&] { return mt::launch_hfield(mt::M_UPD | mt::M_PSI, s->d_desc, 1, s->d_pre,
+                                                             s->nblocks, j, 1, s->st); });
+            if (!rc && s->nrec)
+                rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
+        }
+        if (!rc && s->nf)
+            rc = timed(3, [&] { return mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st); });
+        if (!rc)
+            rc = timed(4, [&] { return mt::launch_advance(s->d_desc, 1, L, s->st); });
+        if (rc)
+            break;
+        s->step += L;
+        left -= L;
+        if (s->ring > 1 && L % s->ring) {
+            s->desc.cur = (s->desc.cur + L) % s->ring;
+            rc = upload_desc(s);
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc)
+        return rc;
+    for (int q = 0; q < 4; ++q)
+        ms[q] = cnt[q] ? acc[q] / cnt[q] : 0.0;
+    ms[4] = steps_run ? (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / steps_run : 0.0;
+    return FWI3D_OK;
+}
+
+int fwi3d_run(fwi3d_shot *s)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    return fwi3d_step_n(s, s->it - 1 - s->step);
+}
+
+int fwi3d_shot_sync(fwi3d_shot *s)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_shot_step(const fwi3d_shot *s) { return s ? s->step : -1; }
+
+int fwi3d_get_field(fwi3d_shot *s, int which, double *out)
+{
+    if (!s || !out || which < 0 || which > 5)
+        return fail(FWI3D_EINVAL, "bad field request");
+    const size_t N = (size_t)s->desc.n;
+    const double *src = which < 3 ? s->d_e + ((size_t)s->desc.cur * 3 + which) * N
+                                  : s->d_h + (size_t)(which - 3) * N;
+    HIP_TRY(hipMemcpyAsync(out, src, sizeof(double) * N, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_set_field(fwi3d_shot *s, int which, const double *in)
+{
+    if (!s || !in || which < 0 || which > 5)
+        return fail(FWI3D_EINVAL, "bad field request");
+    const size_t N = (size_t)s->desc.n;
+    double *dst = which < 3 ? s->d_e + ((size_t)s->desc.cur * 3 + which) * N
+                            : s->d_h + (size_t)(which - 3) * N;
+    HIP_TRY(hipMemcpyAsync(dst, in, sizeof(double) * N, hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_get_traces(fwi3d_shot *s, double *out)
+{
+    if (!s || !out)
+        return fail(FWI3D_EINVAL, "null argument");
+    if (s->nrec == 0)
+        return FWI3D_OK;
+    HIP_TRY(hipMemcpyAsync(out, s->d_traces, sizeof(double) * 6 * (size_t)s->nrec * s->it,
+                           hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_get_volume_dft(fwi3d_shot *s, double *out)
+{
+    if (!s || !out)
+        return fail(FWI3D_EINVAL, "null argument");
+    if (!s->nf)
+        return fail(FWI3D_ESTATE, "shot has no volume DFT frequencies");
+    HIP_TRY(hipMemcpyAsync(out, s->d_acc, sizeof(double) * 2 * 3 * (size_t)s->nf * s->desc.n,
+                           hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return FWI3D_OK;
+}
+
+static int device_rows_dft(hipStream_t st, const double *d_rows, int nrows, int it, double dt,
+                           double omega0, int nfreq, const double *omegas, double *out_host)
+{
+    double *d_om = nullptr, *d_out = nullptr;
+    if (dalloc(&d_om, (size_t)nfreq) != hipSuccess ||
+        dalloc(&d_out, 2 * (size_t)nrows * nfreq) != hipSuccess) {
+        dfree(d_om);
+        return fail(FWI3D_ENOMEM, "spectrum buffers");
+    }
+    hipError_t e = hipMemcpyAsync(d_om, omegas, sizeof(double) * nfreq, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = mt::launch_receiver_dft(d_rows, nrows, it, dt, omega0, nfreq, d_om, d_out, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out_host, d_out, sizeof(double) * 2 * nrows * nfreq,
+                           hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    dfree(d_om);
+    dfree(d_out);
+    if (e != hipSuccess)
+        return fail(FWI3D_EDEVICE, "receiver DFT: %s", hipGetErrorString(e));
+    return FWI3D_OK;
+}
+
+static std::complex<double> h_factor(double omega0, double om)
+{
+    // sqrt(-2*omega0/(i*omega)), ver0040/cpu_f_to_d_h.f90:146-148
+    return std::sqrt(std::complex<double>(-2.0 * omega0, 0.0) / std::complex<double>(0.0, 1.0) / om);
+}
+
+int fwi3d_receiver_dft(fwi3d_shot *s, int nfreq, const double *omegas, int hfac, double *out)
+{
+    if (!s || !omegas || !out || nfreq <= 0)
+        return fail(FWI3D_EINVAL, "bad receiver DFT request");
+    if (s->nrec == 0)
+        return FWI3D_OK;
+    int rc = device_rows_dft(s->st, s->d_traces, 6 * s->nrec, s->it, s->m->p.dt, s->m->p.omega0,
+                             nfreq, omegas, out);
+    if (rc || !hfac)
+        return rc;
+    for (int f = 0; f < nfreq; ++f) {
+        const std::complex<double> k = h_factor(s->m->p.omega0, omegas[f]);
+        for (int row = 3 * s->nrec; row < 6 * s->nrec; ++row) {
+            std::complex<double> v(out[2 * ((size_t)row * nfreq + f)],
+                                   out[2 * ((size_t)row * nfreq + f) + 1]);
+            v *= k;
+            out[2 * ((size_t)row * nfreq + f)] = v.real();
+            out[2 * ((size_t)row * nfreq + f) + 1] = v.imag();
+        }
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_f_to_d_h(double omega0, int nfreq, const double *omegas, long long count, double *sp)
+{
+    if (!omegas || !sp || nfreq <= 0 || count < 0)
+        return fail(FWI3D_EINVAL, "bad f_to_d_h request");
+    for (long long q = 0; q < count; ++q)
+        for (int f = 0; f < nfreq; ++f) {
+            const std::complex<double> k = h_factor(omega0, omegas[f]);
+            std::complex<double> v(sp[2 * (q * nfreq + f)], sp[2 * (q * nfreq + f) + 1]);
+            v *= k;
+            sp[2 * (q * nfreq + f)] = v.real();
+            sp[2 * (q * nfreq + f) + 1] = v.imag();
+        }
+    return FWI3D_OK;
+}
+
+int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal, int nfreq,
+                          const double *omegas, double *out)
+{
+    if (!signal || !omegas || !out || it < 1 || nfreq <= 0)
+        return fail(FWI3D_EINVAL, "bad source spectrum request");
+    double *d_sig = nullptr;
+    if (dalloc(&d_sig, (size_t)it) != hipSuccess)
+        return fail(FWI3D_ENOMEM, "signal buffer");
+    int rc = FWI3D_OK;
+    if (hipMemcpy(d_sig, signal, sizeof(double) * it, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(FWI3D_EDEVICE, "signal upload");
+    if (!rc)
+        rc = device_rows_dft(nullptr, d_sig, 1, it, dt, omega0, nfreq, omegas, out);
+    dfree(d_sig);
+    if (rc)
+        return rc;
+    for (int f = 0; f < nfreq; ++f) {
+        std::complex<double> v(out[2 * f], out[2 * f + 1]);
+        v = omegas[f] == 0.0 ? std::complex<double>(2.0 * omega0, 0.0) : v * h_factor(omega0, omegas[f]);
+        out[2 * f] = v.real();
+        out[2 * f + 1] = v.imag();
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue, double *e_w, double *j_w,
+                          double *g_w, double *obs)
+{
+    if (!s || !strue || !e_w || !j_w || !g_w || !obs || nbins <= 0 || nbins > s->it)
+        return fail(FWI3D_EINVAL, "bad laplace_to_freq request");
+    if (s->signal.empty())
+        return fail(FWI3D_ESTATE, "shot has no source waveform");
+    const int it = s->it;
+    const double dt = s->m->p.dt, omega0 = s->m->p.omega0;
+    const double om = 2.0 * M_PI / it / dt;
+    std::vector<double> omg(nbins);
+    for (int n = 0; n < nbins; ++n)
+        omg[n] = om * n;
+    int rc = FWI3D_OK;
+    if (s->nrec)
+        rc = device_rows_dft(s->st, s->d_traces, 3 * s->nrec, it, dt, omega0, nbins, omg.data(), e_w);
+    if (!rc)
+        rc = fwi3d_source_spectrum(dt, omega0, it, s->signal.data(), nbins, omg.data(), j_w);
+    if (rc)
+        return rc;
+    // FFTW_BACKWARD (un-normalised exp(+2 pi i j n / it)) of the true waveform, bins n < nbins
+    std::vector<std::complex<double>> tru(nbins);
+    for (int n = 0; n < nbins; ++n) {
+        double re = 0.0, im = 0.0;
+        for (int q = 0; q < it; ++q) {
+            const long long mm = ((long long)q * n) % it;
+            const double ang = 2.0 * M_PI * (double)mm / (double)it;
+            re += strue[q] * std::cos(ang);
+            im += strue[q] * std::sin(ang);
+        }
+        tru[n] = {re, im};
+    }
+    for (int row = 0; row < 3 * s->nrec; ++row)
+        for (int n = 0; n < nbins; ++n) {
+            const size_t q = (size_t)row * nbins + n;
+            const std::complex<double> e(e_w[2 * q], e_w[2 * q + 1]), jv(j_w[2 * n], j_w[2 * n + 1]);
+            const std::complex<double> g = e / jv, o = g * tru[n];
+            g_w[2 * q] = g.real();
+            g_w[2 * q + 1] = g.imag();
+            obs[2 * q] = o.real();
+            obs[2 * q + 1] = o.imag();
+        }
+    return FWI3D_OK;
+}
+
+// ---------------------------------------------------------------- batches
+
+int fwi3d_batch_create(int max_shots, int chunk, void *stream, fwi3d_batch **out)
+{
+    if (!out || max_shots <= 0)
+        return fail(FWI3D_EINVAL, "bad batch request");
+    if (chunk <= 0)
+        chunk = kDefaultChunk;
+    if (chunk > mt::RING_MAX)
+        return fail(FWI3D_EINVAL, "chunk %d exceeds %d", chunk, mt::RING_MAX);
+    fwi3d_batch *b = new fwi3d_batch();
+    b->max_shots = max_shots;
+    b->chunk = chunk;
+    if (stream) {
+        b->st = (hipStream_t)stream;
+    } else if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(FWI3D_EDEVICE, "hipStreamCreate failed");
+    } else {
+        b->own_stream = true;
+    }
+    if (dalloc(&b->d_desc, (size_t)max_shots) != hipSuccess ||
+        dalloc(&b->d_pre, (size_t)max_shots + 1) != hipSuccess ||
+        dalloc(&b->d_pre_cells, (size_t)max_shots + 1) != hipSuccess) {
+        fwi3d_batch_destroy(b);
+        return fail(FWI3D_ENOMEM, "batch descriptors");
+    }
+    *out = b;
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_destroy(fwi3d_batch *b)
+{
+    if (!b)
+        return FWI3D_OK;
+    if (b->st)
+        (void)hipStreamSynchronize(b->st);
+    for (auto *s : b->shots)
+        s->batch = nullptr;
+    if (b->graph)
+        (void)hipGraphExecDestroy(b->graph);
+    dfree(b->d_desc);
+    dfree(b->d_pre);
+    dfree(b->d_pre_cells);
+    if (b->own_stream && b->st)
+        (void)hipStreamDestroy(b->st);
+    delete b;
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_add(fwi3d_batch *b, fwi3d_shot *s)
+{
+    if (!b || !s)
+        return fail(FWI3D_EINVAL, "null argument");
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot already in a batch");
+    if ((int)b->shots.size() >= b->max_shots)
+        return fail(FWI3D_EBUDGET, "batch holds at most %d shots", b->max_shots);
+    if (s->ring != 1 && s->ring != b->chunk)
+        return fail(FWI3D_EINVAL, "shot ring %d must equal the batch chunk %d", s->ring, b->chunk);
+    int rc = ensure_amp(s);
+    if (rc)
+        return rc;
+    // the shot's own stream must be idle before the batch stream touches it
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->batch = b;
+    b->shots.push_back(s);
+    b->dirty = true;
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_remove(fwi3d_batch *b, fwi3d_shot *s)
+{
+    if (!b || !s || s->batch != b)
+        return fail(FWI3D_EINVAL, "shot is not in this batch");
+    HIP_TRY(hipStreamSynchronize(b->st));
+    b->shots.erase(std::find(b->shots.begin(), b->shots.end(), s));
+    s->batch = nullptr;
+    s->desc_dirty = true;
+    b->dirty = true;
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_size(const fwi3d_batch *b) { return b ? (int)b->shots.size() : -1; }
+
+static int batch_upload(fwi3d_batch *b)
+{
+    const int ns = (int)b->shots.size();
+    std::vector<ShotDev> descs(ns);
+    std::vector<int> pre(ns + 1, 0), prec(ns + 1, 0);
+    for (int q = 0; q < ns; ++q) {
+        descs[q] = b->shots[q]->desc;
+        pre[q + 1] = pre[q] + b->shots[q]->nblocks;
+        prec[q + 1] = prec[q] + b->shots[q]->ncellblocks;
+    }
+    b->nblocks = pre[ns];
+    b->ncellblocks = prec[ns];
+    if (ns) {
+        HIP_TRY(hipMemcpyAsync(b->d_desc, descs.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice,
+                               b->st));
+        HIP_TRY(hipMemcpyAsync(b->d_pre, pre.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice,
+                               b->st));
+        HIP_TRY(hipMemcpyAsync(b->d_pre_cells, prec.data(), sizeof(int) * (ns + 1),
+                               hipMemcpyHostToDevice, b->st));
+        HIP_TRY(hipStreamSynchronize(b->st));
+    }
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->dirty = false;
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
+{
+    if (!b || nsteps < 0)
+        return fail(FWI3D_EINVAL, "bad batch step");
+    int rc;
+    if (b->dirty && (rc = batch_upload(b)))
+        return rc;
+    const int ns = (int)b->shots.size();
+    if (ns == 0)
+        return FWI3D_OK;
+    int left = 0;
+    bool dft = false, rec = false;
+    for (auto *s : b->shots) {
+        left = std::max(left, std::min(nsteps, s->it - 1 - s->step));
+        dft = dft || s->nf > 0;
+        rec = rec || s->nrec > 0;
+    }
+    while (left > 0) {
+        const int L = std::min(b->chunk, left);
+        if (L == b->chunk) {
+            if (!b->graph && (rc = capture_chunk(&b->graph, b->d_desc, ns, b->d_pre, b->nblocks,
+                                                 b->d_pre_cells, b->ncellblocks, L, dft, rec, b->st)))
+                return rc;
+            HIP_TRY(hipGraphLaunch(b->graph, b->st));
+        } else if ((rc = launch_chunk(b->d_desc, ns, b->d_pre, b->nblocks, b->d_pre_cells,
+                                      b->ncellblocks, L, dft, rec, b->st))) {
+            return rc;
+        }
+        left -= L;
+        bool moved = false;
+        for (auto *s : b->shots) {
+            const int done = std::min(L, s->it - 1 - s->step);
+            s->step += done;
+            if (s->ring > 1 && done % s->ring) {
+                s->desc.cur = (s->desc.cur + done) % s->ring;
+                moved = true;
+            }
+            s->desc_dirty = true;
+        }
+        if (moved && (rc = batch_upload(b)))
+            return rc;
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_sync(fwi3d_batch *b)
+{
+    if (!b)
+        return fail(FWI3D_EINVAL, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->st));
+    return FWI3D_OK;
+}
+
+}  // extern "C"

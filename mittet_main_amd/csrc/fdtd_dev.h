@@ -1,0 +1,70 @@
+// fdtd_dev.h -- device-side shot descriptor and kernel launchers shared by the
+// HIP kernels (fdtd_kernels.hip) and the C-ABI host layer (fdtd_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace mt {
+
+// Packed per-axis 1-D CPML profile rows on the device: prof[a][row * n_a + idx].
+enum { PR_ESIG = 0, PR_MSIG, PR_BE, PR_BH, PR_CE, PR_CH, PR_KED, PR_KHD, PR_N };
+
+// psi arrays per axis: slab arrays of [0,nc) U [n-nc,n) along the axis.
+enum { PSI_EP = 0, PSI_EM, PSI_HP, PSI_HM, PSI_N };
+
+constexpr int TX = 64;   // threads along x (one wavefront row, 512 B coalesced)
+constexpr int TY = 4;    // rows per block
+constexpr int RING_MAX = 32;
+
+// Everything a kernel needs about one shot.  Lives in device memory; blocks
+// find their shot through an exclusive prefix of per-shot block counts.
+struct ShotDev {
+    int ix, iy, iz, nc;
+    int n;                    // cells
+    int nbx, nby;             // xy tiles
+    int it;                   // samples; steps 0 .. it-2 exist
+    double dt, omega0, mu0, c1, c2;
+    double dl[3];
+    const double *prof[3];    // [PR_N][n_a]
+    const double *eps;        // eps' = sigma/2/omega0 per cell
+    double *e;                // E ring: slot q, comp d at e + (q*3 + d)*n
+    int ring, cur;            // ring slots; slot that holds E at chunk start
+    double *h;                // H [3][n], updated in place
+    double *psi[3];           // [PSI_N][slab[a]]
+    long long slab[3];
+    int src_comp;
+    int src_cell;             // -1: no source
+    const double *amp;        // [it-1] source decrement per step
+    int nrec;
+    const int *rec_cell;      // [nrec]
+    double *traces;           // [6][nrec][it]
+    int nf;
+    const double *w;          // complex [it][nf]
+    double *acc;              // complex [3][nf][n]
+    int *step;                // absolute step of local index j = 0 of the current chunk
+};
+
+enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
+
+// Kernel launchers (fdtd_kernels.hip).  `pre` is the exclusive prefix of
+// per-shot block counts (device), `nblocks` its total.
+hipError_t launch_efield(int mode, const ShotDev *d, int ns, const int *pre, int nblocks,
+                         int j, int nover, int sign, hipStream_t st);
+hipError_t launch_hfield(int mode, const ShotDev *d, int ns, const int *pre, int nblocks,
+                         int j, int sign, hipStream_t st);
+hipError_t launch_source(const ShotDev *d, int n, hipStream_t st);
+hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
+                            hipStream_t st);
+hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int L,
+                        hipStream_t st);
+hipError_t launch_advance(const ShotDev *d, int ns, int L, hipStream_t st);
+hipError_t launch_receiver_dft(const double *traces, int ncomp_rows, int it, double dt,
+                               double omega0, int nfreq, const double *omegas,
+                               double *out, hipStream_t st);
+hipError_t launch_bin_dft(const double *traces, int nrows, int it, double dt, double omega0,
+                          int nbins, double *out, hipStream_t st);
+hipError_t launch_fill(double *p, size_t count, double v, hipStream_t st);
+hipError_t launch_eps(const double *sig, double *eps, int n, double omega0, hipStream_t st);
+
+}  // namespace mt

@@ -1,0 +1,553 @@
+/*
+ * fwi3d_oracle.c -- TEST INFRASTRUCTURE ONLY (see fwi3d_oracle.h).
+ *
+ * CPU restatement of the reference's C v1.4 forward path, in the reference's
+ * structure: 12 per-cell coefficient volumes, full-volume psi arrays, one
+ * OpenMP loop nest per field component / CPML face, receiver gather per step.
+ * This is also the "port" CPU baseline timed beside the GPU path.
+ *
+ * Each function names the reference lines it restates.  Compile with
+ * -ffp-contract=off: the evaluation order below is the reference's, and
+ * fused multiply-adds would change the last bits.
+ */
+#include "fwi3d_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+struct or_state {
+    or_params p;
+    long n;                        /* ix*iy*iz */
+    double *vol[OR_NARRAYS];
+    double *prof[3][OR_NPROFILES];
+};
+
+static int axis_len(const or_params *p, int a)
+{
+    return a == 0 ? p->ix : (a == 1 ? p->iy : p->iz);
+}
+
+static double axis_step(const or_params *p, int a)
+{
+    return a == 0 ? p->dx : (a == 1 ? p->dy : p->dz);
+}
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* satan_i/fwi3d_cpu_lattice_v1.4.c:35-39: cmax from the minimum conductivity,
+ * Courant number of the 3-D grid, dt = courant*6/7*factor (factor 0.999f in
+ * the reference; 6/7 = 1/(|c1|+|c2|) is the 4th-order stability margin). */
+double or_lattice_time_3d(double minval, double omega0, double mu0,
+                          double dx, double dy, double dz, double factor,
+                          double *cmax_out)
+{
+    double cmax = sqrt(2.0 * omega0 / mu0 / minval);
+    double inv = 1.0 / pow(dx, 2) + 1.0 / pow(dy, 2) + 1.0 / pow(dz, 2);
+    double courant = 1.0 / cmax / sqrt(inv);
+    if (cmax_out)
+        *cmax_out = cmax;
+    return courant * 6.0 / 7.0 * factor;
+}
+
+or_state *or_create(const or_params *p)
+{
+    or_state *s = (or_state *)calloc(1, sizeof(or_state));
+    if (!s)
+        return NULL;
+    s->p = *p;
+    s->n = (long)p->ix * p->iy * p->iz;
+    for (int a = 0; a < OR_NARRAYS; ++a) {
+        s->vol[a] = (double *)calloc((size_t)s->n, sizeof(double));
+        if (!s->vol[a]) {
+            or_destroy(s);
+            return NULL;
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < OR_NPROFILES; ++q) {
+            s->prof[a][q] = (double *)calloc((size_t)axis_len(p, a), sizeof(double));
+            if (!s->prof[a][q]) {
+                or_destroy(s);
+                return NULL;
+            }
+        }
+    return s;
+}
+
+void or_destroy(or_state *s)
+{
+    if (!s)
+        return;
+    for (int a = 0; a < OR_NARRAYS; ++a)
+        free(s->vol[a]);
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < OR_NPROFILES; ++q)
+            free(s->prof[a][q]);
+    free(s);
+}
+
+double *or_array(or_state *s, int id)
+{
+    return (id >= 0 && id < OR_NARRAYS) ? s->vol[id] : NULL;
+}
+
+double *or_profile(or_state *s, int axis, int id)
+{
+    if (axis < 0 || axis > 2 || id < 0 || id >= OR_NPROFILES)
+        return NULL;
+    return s->prof[axis][id];
+}
+
+/* satan_i/fwi3d_cpu_cpmlabc_v1.4.c:7-175.  Cubic grading of sigma, linear
+ * alpha, kappa = 1 + (kappa_max-1)*ramp^nn, recursive-convolution constants
+ * b = exp(-(sigma/kappa+alpha)dt), c = sigma(b-1)/(sigma+kappa*alpha)/kappa.
+ * The ramps are evaluated on integer (E) and half-integer (H) offsets with the
+ * reference's float-typed offsets, so the edge cell exceeds sigma_max. */
+void or_init_cpml(or_state *s)
+{
+    const or_params *p = &s->p;
+    const int nc = p->ncpml;
+    const double expo = p->nn + p->order;
+    for (int a = 0; a < 3; ++a) {
+        const int n = axis_len(p, a);
+        const double h = axis_step(p, a);
+        const double delta = nc * h;
+        const double smax = (p->nn + p->order + 1.0) * p->cmax * log(1.0 / p->rcoef)
+                            / (2.0 * delta) * p->opt_to_max[a];
+        double **q = s->prof[a];
+        for (int i = 0; i < n; ++i) {
+            double re, rm, ae, am; /* ramp arguments (float-typed in the reference) */
+            int inside = 1;
+            if (i < nc) {
+                re = (double)(nc - i);
+                rm = (double)((float)(nc - i) - 0.5f);
+                ae = (double)i;
+                am = (double)((float)i + 0.5f);
+            } else if (i > n - nc - 1) {
+                re = (double)((float)(i - n) + 1.0f + (float)nc);
+                rm = (double)((float)(i - n) + 0.5f + (float)nc);
+                ae = (double)((float)(-i + n) - 1.0f);
+                am = (double)((float)(-i + n) - 0.5f);
+            } else {
+                inside = 0;
+                re = rm = ae = am = 0.0;
+            }
+            if (inside) {
+                const double den = (double)((float)nc - 1.0f);
+                q[OR_P_ESIG][i] = smax * pow(re / den, expo);
+                q[OR_P_MSIG][i] = smax * pow(rm / den, expo);
+                q[OR_P_EKAP][i] = 1.0 + (p->kappa_max - 1.0) * pow(re / den, p->nn);
+                q[OR_P_MKAP][i] = 1.0 + (p->kappa_max - 1.0) * pow(rm / den, p->nn);
+                q[OR_P_AE][i] = p->alpha_max * pow(ae / den, p->ma);
+                q[OR_P_AM][i] = p->alpha_max * pow(am / den, p->ma);
+                q[OR_P_BE][i] = exp(-(q[OR_P_ESIG][i] / q[OR_P_EKAP][i] + q[OR_P_AE][i]) * p->dt);
+                q[OR_P_BH][i] = exp(-(q[OR_P_MSIG][i] / q[OR_P_MKAP][i] + q[OR_P_AM][i]) * p->dt);
+                q[OR_P_CE][i] = q[OR_P_ESIG][i] * (q[OR_P_BE][i] - 1.0)
+                                / (q[OR_P_ESIG][i] + q[OR_P_EKAP][i] * q[OR_P_AE][i])
+                                / q[OR_P_EKAP][i];
+                q[OR_P_CH][i] = q[OR_P_MSIG][i] * (q[OR_P_BH][i] - 1.0)
+                                / (q[OR_P_MSIG][i] + q[OR_P_MKAP][i] * q[OR_P_AM][i])
+                                / q[OR_P_MKAP][i];
+            } else {
+                q[OR_P_ESIG][i] = q[OR_P_MSIG][i] = 0.0;
+                q[OR_P_EKAP][i] = q[OR_P_MKAP][i] = 1.0;
+                q[OR_P_AE][i] = q[OR_P_AM][i] = 0.0;
+                q[OR_P_BE][i] = q[OR_P_CE][i] = 0.0;
+                q[OR_P_BH][i] = q[OR_P_CH][i] = 0.0;
+            }
+            q[OR_P_KED][i] = q[OR_P_EKAP][i] * h;
+            q[OR_P_KHD][i] = q[OR_P_MKAP][i] * h;
+        }
+    }
+}
+
+/* satan_i/fwi3d_cpu_function_v1.4.c:325-435 (coef1..coef4): fictitious
+ * permittivity eps' = sigma/2/omega0 and the lossy-CPML update factors per
+ * component, each graded along the component's own axis. */
+void or_media_coeff_3d(or_state *s)
+{
+    const or_params *p = &s->p;
+    const int ix = p->ix, iy = p->iy, iz = p->iz;
+    const double dt = p->dt;
+#pragma omp parallel for
+    for (int k = 0; k < iz; ++k)
+        for (int j = 0; j < iy; ++j)
+            for (int i = 0; i < ix; ++i) {
+                const long c = ((long)k * iy + j) * ix + i;
+                const double eps2 = s->vol[OR_SIG][c] / 2.0 / p->omega0;
+                const int idx[3] = {i, j, k};
+                for (int d = 0; d < 3; ++d) {
+                    const double se = s->prof[d][OR_P_ESIG][idx[d]] * dt / (2.0 * eps2);
+                    const double sm = s->prof[d][OR_P_MSIG][idx[d]] * dt / (2.0 * eps2);
+                    s->vol[OR_CA_X + d][c] = (1.0 - se) / (1.0 + se);
+                    s->vol[OR_DA_X + d][c] = (1.0 - sm) / (1.0 + sm);
+                    s->vol[OR_CB_X + d][c] = dt / eps2 / (1.0 + se);
+                    s->vol[OR_DB_X + d][c] = dt / p->mu0 / (1.0 + sm);
+                }
+            }
+}
+
+/* satan_i/fwi3d_cpu_function_v1.4.c:964-986 */
+void or_set_zero_eh(or_state *s)
+{
+    static const int ids[] = {OR_EX, OR_EY, OR_EZ, OR_HX, OR_HY, OR_HZ,
+                              OR_PSI_EZX, OR_PSI_EYX, OR_PSI_EXY, OR_PSI_EZY,
+                              OR_PSI_EYZ, OR_PSI_EXZ, OR_PSI_HYX, OR_PSI_HZX,
+                              OR_PSI_HZY, OR_PSI_HXY, OR_PSI_HXZ, OR_PSI_HYZ};
+    for (size_t q = 0; q < sizeof(ids) / sizeof(ids[0]); ++q)
+        memset(s->vol[ids[q]], 0, (size_t)s->n * sizeof(double));
+}
+
+/* 4th-order staggered differences in the reference's evaluation order:
+ * backward (E update, function.c:1203) c1*F0 - c1*F-1 + c2*F+1 - c2*F-2,
+ * forward (H update, function.c:1257) c1*F+1 - c1*F0 + c2*F+2 - c2*F-1. */
+static inline double dminus(const double *F, long c, long st, double c1, double c2)
+{
+    return c1 * F[c] - c1 * F[c - st] + c2 * F[c + st] - c2 * F[c - 2 * st];
+}
+
+static inline double dplus(const double *F, long c, long st, double c1, double c2)
+{
+    return c1 * F[c + st] - c1 * F[c] + c2 * F[c + 2 * st] - c2 * F[c - st];
+}
+
+/* One component of satan_i/fwi3d_cpu_function_v1.4.c:1184-1236 (sign=+1) or
+ * the sign-flipped back-propagation variant :1292-1333 (sign=-1):
+ * E_d = ca_d*E_d +/- cb_d*(D-_a H_b / ked_a - D-_b H_a / ked_b), (d,a,b) cyclic.
+ * Range: own axis [1,n-1), the two others [2,n-1). */
+static void e_component(or_state *s, int d, int sign)
+{
+    const or_params *p = &s->p;
+    const int a = (d + 1) % 3, b = (d + 2) % 3;
+    const long st[3] = {1, p->ix, (long)p->ix * p->iy};
+    int lo[3], hi[3];
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = (q == d) ? 1 : 2;
+        hi[q] = axis_len(p, q) - 1;
+    }
+    double *E = s->vol[OR_EX + d];
+    const double *Ha = s->vol[OR_HX + a], *Hb = s->vol[OR_HX + b];
+    const double *ca = s->vol[OR_CA_X + d], *cb = s->vol[OR_CB_X + d];
+    const double *keda = s->prof[a][OR_P_KED], *kedb = s->prof[b][OR_P_KED];
+    const double c1 = p->c1, c2 = p->c2;
+#pragma omp parallel for
+    for (int k = lo[2]; k < hi[2]; ++k)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int i = lo[0]; i < hi[0]; ++i) {
+                const int idx[3] = {i, j, k};
+                const long c = k * st[2] + j * st[1] + i;
+                const double curl = dminus(Hb, c, st[a], c1, c2) / keda[idx[a]]
+                                  - dminus(Ha, c, st[b], c1, c2) / kedb[idx[b]];
+                if (sign > 0)
+                    E[c] = ca[c] * E[c] + cb[c] * curl;
+                else
+                    E[c] = ca[c] * E[c] - cb[c] * curl;
+            }
+}
+
+void or_e_field_cpml42(or_state *s, int sign)
+{
+    for (int d = 0; d < 3; ++d)
+        e_component(s, d, sign);
+}
+
+/* One component of satan_i/fwi3d_cpu_function_v1.4.c:1238-1290 (sign=+1) or
+ * :1335-1372 (sign=-1): H_d = da_d*H_d -/+ db_d*(D+_a E_b/khd_a - D+_b E_a/khd_b).
+ * Range: own axis [1,n-1), the two others [1,n-2). */
+static void h_component(or_state *s, int d, int sign)
+{
+    const or_params *p = &s->p;
+    const int a = (d + 1) % 3, b = (d + 2) % 3;
+    const long st[3] = {1, p->ix, (long)p->ix * p->iy};
+    int lo[3], hi[3];
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = 1;
+        hi[q] = axis_len(p, q) - ((q == d) ? 1 : 2);
+    }
+    double *H = s->vol[OR_HX + d];
+    const double *Ea = s->vol[OR_EX + a], *Eb = s->vol[OR_EX + b];
+    const double *da = s->vol[OR_DA_X + d], *db = s->vol[OR_DB_X + d];
+    const double *khda = s->prof[a][OR_P_KHD], *khdb = s->prof[b][OR_P_KHD];
+    const double c1 = p->c1, c2 = p->c2;
+#pragma omp parallel for
+    for (int k = lo[2]; k < hi[2]; ++k)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int i = lo[0]; i < hi[0]; ++i) {
+                const int idx[3] = {i, j, k};
+                const long c = k * st[2] + j * st[1] + i;
+                const double curl = dplus(Eb, c, st[a], c1, c2) / khda[idx[a]]
+                                  - dplus(Ea, c, st[b], c1, c2) / khdb[idx[b]];
+                if (sign > 0)
+                    H[c] = da[c] * H[c] - db[c] * curl;
+                else
+                    H[c] = da[c] * H[c] + db[c] * curl;
+            }
+}
+
+void or_h_field_cpml42(or_state *s, int sign)
+{
+    for (int d = 0; d < 3; ++d)
+        h_component(s, d, sign);
+}
+
+/* CPML memory update on one face slab of axis a, cells [r0,r1) along a and
+ * [0,n-1) along the two other axes (satan_i/fwi3d_cpu_cpmlabc_v1.4.c:402-679).
+ * With p=(a+1)%3, q=(a+2)%3:
+ *   E pass: psi0 <- D-_a H_p (E_q += cb_q psi0), psi1 <- D-_a H_q (E_p -= cb_p psi1)
+ *   H pass: psi0 <- D+_a E_q (H_p += db_p psi0), psi1 <- D+_a E_p (H_q -= db_q psi1)
+ * psi = b*psi + c*D/dl, with dl the plain grid step of axis a. */
+static void cpml_slab(or_state *s, int a, int r0, int r1, int is_h)
+{
+    const or_params *p = &s->p;
+    if (r1 <= r0)
+        return;
+    const int pa = (a + 1) % 3, qa = (a + 2) % 3;
+    const long st[3] = {1, p->ix, (long)p->ix * p->iy};
+    int lo[3] = {0, 0, 0}, hi[3];
+    for (int t = 0; t < 3; ++t)
+        hi[t] = axis_len(p, t) - 1;
+    lo[a] = r0;
+    hi[a] = r1;
+    const double c1 = p->c1, c2 = p->c2;
+    const double dl = axis_step(p, a);
+    const double *bb = s->prof[a][is_h ? OR_P_BH : OR_P_BE];
+    const double *cc = s->prof[a][is_h ? OR_P_CH : OR_P_CE];
+    double *psi0, *psi1, *F0, *F1;
+    const double *S0, *S1, *k0, *k1;
+    static const int pe[3][2] = {{OR_PSI_EZX, OR_PSI_EYX}, {OR_PSI_EXY, OR_PSI_EZY},
+                                 {OR_PSI_EYZ, OR_PSI_EXZ}};
+    static const int ph[3][2] = {{OR_PSI_HYX, OR_PSI_HZX}, {OR_PSI_HZY, OR_PSI_HXY},
+                                 {OR_PSI_HXZ, OR_PSI_HYZ}};
+    if (!is_h) {
+        psi0 = s->vol[pe[a][0]];
+        psi1 = s->vol[pe[a][1]];
+        S0 = s->vol[OR_HX + pa];
+        S1 = s->vol[OR_HX + qa];
+        F0 = s->vol[OR_EX + qa];
+        F1 = s->vol[OR_EX + pa];
+        k0 = s->vol[OR_CB_X + qa];
+        k1 = s->vol[OR_CB_X + pa];
+    } else {
+        psi0 = s->vol[ph[a][0]];
+        psi1 = s->vol[ph[a][1]];
+        S0 = s->vol[OR_EX + qa];
+        S1 = s->vol[OR_EX + pa];
+        F0 = s->vol[OR_HX + pa];
+        F1 = s->vol[OR_HX + qa];
+        k0 = s->vol[OR_DB_X + pa];
+        k1 = s->vol[OR_DB_X + qa];
+    }
+#pragma omp parallel for
+    for (int k = lo[2]; k < hi[2]; ++k)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int i = lo[0]; i < hi[0]; ++i) {
+                const int idx[3] = {i, j, k};
+                const int r = idx[a];
+                const long c = k * st[2] + j * st[1] + i;
+                const double g0 = is_h ? dplus(S0, c, st[a], c1, c2) : dminus(S0, c, st[a], c1, c2);
+                const double g1 = is_h ? dplus(S1, c, st[a], c1, c2) : dminus(S1, c, st[a], c1, c2);
+                psi0[c] = bb[r] * psi0[c] + cc[r] * g0 / dl;
+                psi1[c] = bb[r] * psi1[c] + cc[r] * g1 / dl;
+                F0[c] += k0[c] * psi0[c];
+                F1[c] -= k1[c] * psi1[c];
+            }
+}
+
+/* satan_i/fwi3d_cpu_cpmlabc_v1.4.c:191-196 with xe/ye/ze_pml4 (:402-543):
+ * slabs [2,nc-1)|[n-nc+1,n-2) on x, [2,nc)|[n-nc+1,n-2) on y and z. */
+void or_epml_abcs4(or_state *s)
+{
+    const int nc = s->p.ncpml;
+    for (int a = 0; a < 3; ++a) {
+        const int n = axis_len(&s->p, a);
+        cpml_slab(s, a, 2, a == 0 ? nc - 1 : nc, 0);
+        cpml_slab(s, a, n - nc + 1, n - 2, 0);
+    }
+}
+
+/* satan_i/fwi3d_cpu_cpmlabc_v1.4.c:198-203 with xh/yh/zh_pml4 (:545-679):
+ * slabs [1,nc-1)|[n-nc,n-2) on every axis. */
+void or_hpml_abcs4(or_state *s)
+{
+    const int nc = s->p.ncpml;
+    for (int a = 0; a < 3; ++a) {
+        const int n = axis_len(&s->p, a);
+        cpml_slab(s, a, 1, nc - 1, 1);
+        cpml_slab(s, a, n - nc, n - 2, 1);
+    }
+}
+
+/* satan_i/fwi3d_cpu_function_v1.4.c:514-529: half-step averaged current
+ * injected into E_comp at a 1-based cell, scaled by 1/eps' and the cell volume. */
+void or_read_source_3d(or_state *s, int comp, const double *signal,
+                       int jx, int jy, int jz, int step)
+{
+    const or_params *p = &s->p;
+    const long c = (long)p->ix * p->iy * (jz - 1) + (long)p->ix * (jy - 1) + jx - 1;
+    s->vol[OR_EX + comp][c] -= (signal[step] + signal[step + 1]) / 2.0 * p->dt * 2.0
+                               * p->omega0 / s->vol[OR_SIG][c] / p->dx / p->dy / p->dz;
+}
+
+/* satan_i/fwi3d_cpu_function_v1.4.c:765-785 (trace part; no file output). */
+void or_output(or_state *s, int nrec, const int *rec1, int it, int step,
+               double *tx, double *ty, double *tz)
+{
+    const or_params *p = &s->p;
+    for (int r = 0; r < nrec; ++r) {
+        const long c = (long)p->ix * p->iy * (rec1[3 * r + 2] - 1)
+                     + (long)p->ix * (rec1[3 * r + 1] - 1) + rec1[3 * r] - 1;
+        tx[(long)r * it + step] = s->vol[OR_EX][c];
+        ty[(long)r * it + step] = s->vol[OR_EY][c];
+        tz[(long)r * it + step] = s->vol[OR_EZ][c];
+    }
+}
+
+/* One forward step in the order of satan_i/fwi3d_cpu_rec2rec_v1.4.c:48-55, with
+ * the volume running DFT of E after the step (the reference's copytoEcal +
+ * laplaceToFreq_2 pair, function.c:1652-1686 and 2091-2200, accumulated per step
+ * instead of storing the history). */
+void or_forward(or_state *s, int comp, const double *signal,
+                int jx, int jy, int jz, int step0, int nsteps,
+                int nrec, const int *rec1, int it,
+                double *tx, double *ty, double *tz,
+                int nf, const double *w, double *acc)
+{
+    const long n = s->n;
+    for (int step = step0; step < step0 + nsteps; ++step) {
+        or_e_field_cpml42(s, 1);
+        or_epml_abcs4(s);
+        if (signal)
+            or_read_source_3d(s, comp, signal, jx, jy, jz, step);
+        or_h_field_cpml42(s, 1);
+        or_hpml_abcs4(s);
+        if (nrec > 0)
+            or_output(s, nrec, rec1, it, step, tx, ty, tz);
+        if (nf > 0) {
+            for (int d = 0; d < 3; ++d) {
+                const double *E = s->vol[OR_EX + d];
+                for (int f = 0; f < nf; ++f) {
+                    const double wr = w[2 * ((long)step * nf + f)];
+                    const double wi = w[2 * ((long)step * nf + f) + 1];
+                    double *A = acc + 2 * ((long)d * nf + f) * n;
+#pragma omp parallel for
+                    for (long c = 0; c < n; ++c) {
+                        A[2 * c] += E[c] * wr;
+                        A[2 * c + 1] += E[c] * wi;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* satan_i/fwi3d_cpu_waveform_v1.4.c:132-143: first derivative of a Gaussian,
+ * beta = pi*fmax^2, delay t0 = pi/fmax. */
+void or_firstderiv_gauss(int nt, double dt, double fmax_w, double *out)
+{
+    const double beta = M_PI * pow(fmax_w, 2);
+    const double t0 = M_PI / fmax_w;
+    for (int i = 0; i < nt; ++i)
+        out[i] = -2.0 * beta * (i * dt - t0) * sqrt(beta / M_PI)
+                 * exp(-beta * pow(i * dt - t0, 2.0));
+}
+
+/* satan_i/fwi3d_cpu_waveform_v1.4.c:420-438: Ricker wavelet, fp = 1 Hz,
+ * phase 1 s, sample offset 200. */
+void or_ricker(int nt, double dt, double *out)
+{
+    const double fp = 1.0, md = 1.0, phase = md / fp;
+    for (int i = 0; i < nt; ++i) {
+        const double al = pow(M_PI * fp, 2) * pow((double)(i - 200) * dt - phase, 2);
+        out[i] = (2.0 * al - 1.0) * exp(-al);
+    }
+}
+
+/* Fictitious->diffusive running-DFT weights at arbitrary angular frequencies:
+ * w[n][f] = dt*exp(-a*n*dt)*exp(i*a*n*dt), a = sqrt(omega0*omega_f)
+ * (the kernel of satan_i/fwi3d_cpu_function_v1.4.c:2041-2042 and
+ * ver0040/cpu_f_to_d_e.f90:140-142).  Output complex128 [it][nf]. */
+void or_dft_weights(int it, double dt, double omega0, int nf,
+                    const double *omega, double *w)
+{
+    for (int f = 0; f < nf; ++f) {
+        const double a = sqrt(omega0 * omega[f]);
+        for (int n = 0; n < it; ++n) {
+            const double x = a * n * dt;
+            const double e = dt * exp(-x);
+            w[2 * ((long)n * nf + f)] = e * cos(x);
+            w[2 * ((long)n * nf + f) + 1] = e * sin(x);
+        }
+    }
+}
+
+/* FFTW_BACKWARD definition (un-normalised, exp(+2*pi*i*j*k/n)) of a real
+ * sequence, as the reference applies it to the true waveform
+ * (function.c:2006-2007).  Naive O(n^2) with exact index reduction. */
+void or_fft_backward_real(int n, const double *in, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const long m = ((long)j * k) % n;
+            const double ang = 2.0 * M_PI * (double)m / (double)n;
+            re += in[j] * cos(ang);
+            im += in[j] * sin(ang);
+        }
+        out[2 * k] = re;
+        out[2 * k + 1] = im;
+    }
+}
+
+/* satan_i/fwi3d_cpu_function_v1.4.c:2038-2056 for one receiver component:
+ * bins omega_n = n*2*pi/(it*dt), n < nbins; E_n and the fictitious source
+ * spectrum J_n = csqrt(-2*omega0/(i*omega_n)) * sum J[k] * kernel, J_0 := 2*omega0;
+ * G = E/J; OBS = G * FFTW_BACKWARD(true waveform)[n].  All outputs complex128.
+ * The term is evaluated as ((x*dt)*exp(-a k dt))*cexp(i a k dt) like the
+ * reference so that the oracle reproduces it to the last bit. */
+void or_laplace_to_freq(int it, double dt, double omega0, int nbins,
+                        const double *trace, const double *jsig,
+                        const double *strue,
+                        double *e_w, double *j_w, double *g_w, double *obs)
+{
+    double *tru = (double *)malloc(sizeof(double) * 2 * (size_t)it);
+    or_fft_backward_real(it, strue, tru);
+    const double om = 2.0 * M_PI / it / dt;
+#pragma omp parallel for
+    for (int n = 0; n < nbins; ++n) {
+        const double a = sqrt(omega0 * om * n);
+        const double complex jf = csqrt(-2.0 * omega0 / I / om / n);
+        double complex ev = 0.0, jv = 0.0;
+        for (int k = 0; k < it; ++k) {
+            const double decay = exp(-a * k * dt);
+            const double complex osc = cexp(I * a * k * dt);
+            const double complex jk = jsig[k];
+            ev += trace[k] * dt * decay * osc;
+            jv += jf * jk * dt * decay * osc;
+        }
+        if (n == 0)
+            jv = 2.0 * omega0;
+        const double complex g = ev / jv;
+        const double complex o = g * (tru[2 * n] + I * tru[2 * n + 1]);
+        e_w[2 * n] = creal(ev);
+        e_w[2 * n + 1] = cimag(ev);
+        j_w[2 * n] = creal(jv);
+        j_w[2 * n + 1] = cimag(jv);
+        g_w[2 * n] = creal(g);
+        g_w[2 * n + 1] = cimag(g);
+        obs[2 * n] = creal(o);
+        obs[2 * n + 1] = cimag(o);
+    }
+    free(tru);
+}

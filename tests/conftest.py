@@ -1,0 +1,28 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+    config.addinivalue_line("markers", "slow: longer CPU test")
+
+
+def gpu_available() -> bool:
+    try:
+        import torch  # noqa: F401  (device query only)
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def hip_lib():
+    """The HIP library, loaded; GPU tests fail loudly if it is missing."""
+    from mittet_main_amd import _lib
+    return _lib.load()

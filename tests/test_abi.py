@@ -1,0 +1,93 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point
+include/mittet_fdtd.h declares, and its host-only functions and argument
+validation behave (no compute is launched here)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from mittet_main_amd import _lib
+from mittet_main_amd.fwi3d import Grid, MU0, f_to_d_h, lattice_time_3d, shot_bytes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mittet_fdtd.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fwi3d_\w+)\s*\(", txt)))
+
+
+def test_header_declarations_all_exported():
+    names = declared()
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+
+
+def test_ctypes_binding_covers_header():
+    assert sorted(_lib.exported_symbols()) == declared()
+
+
+def test_library_loads_without_device():
+    L = _lib.load()
+    assert L.fwi3d_version().decode().startswith("mittet_fdtd")
+
+
+def test_lattice_equals_oracle_bitwise():
+    for sig_min, d in [(0.01, (50.0, 50.0, 50.0)), (0.45, (30.0, 30.0, 30.0)), (0.2, (25.0, 60.0, 35.0))]:
+        for factor in (oracle.DT_FACTOR_REF, 0.99):
+            a = lattice_time_3d(sig_min, 2 * np.pi, *d, factor=factor)
+            b = oracle.lattice_time_3d(sig_min, 2 * np.pi, *d, factor=factor)
+            assert a == b
+
+
+def test_f_to_d_h_factor():
+    om = 2 * np.pi * np.array([0.1, 1.0, 4.0])
+    x = np.random.default_rng(0).standard_normal((5, 3)) + 1j
+    y = f_to_d_h(x, om, 2 * np.pi)
+    np.testing.assert_allclose(y, x * np.sqrt(-2 * 2 * np.pi / (1j * om)), rtol=1e-15)
+
+
+def grid(**kw):
+    g = dict(ix=40, iy=40, iz=40, ncpml=10, dx=50.0, dy=50.0, dz=50.0, dt=7.7e-4,
+             omega0=2 * np.pi, cmax=31622.0)
+    g.update(kw)
+    return Grid(**g)
+
+
+@pytest.mark.parametrize("bad", [dict(ncpml=1), dict(ix=20), dict(dt=0.0), dict(dx=-1.0),
+                                 dict(cmax=0.0)])
+def test_model_create_rejects_bad_grids(bad):
+    p = grid(**bad).to_c()
+    h = C.c_void_p()
+    rc = _lib.load().fwi3d_model_create(C.byref(p), None, C.byref(h))
+    assert rc == -1
+    assert _lib.load().fwi3d_last_error()
+    assert not h.value
+
+
+def test_shot_bytes_accounts_ring_and_accumulators():
+    g = grid()
+    n = 40 ** 3
+    plain = shot_bytes(g, it=100)
+    with_dft = shot_bytes(g, it=100, nf_vol=4, ring=16)
+    assert plain > 8 * 6 * n
+    # 15 extra ring slots of E (3 comps) + 4 complex accumulators x 3 comps + weights
+    assert with_dft - plain == 8 * 3 * n * 15 + 16 * 3 * 4 * n + 16 * 4 * 100
+
+
+def test_null_arguments_are_errors():
+    L = _lib.load()
+    assert L.fwi3d_step_n(None, 1) == -1
+    assert L.fwi3d_get_traces(None, None) == -1
+    assert L.fwi3d_batch_step(None, 1) == -1
+    assert L.fwi3d_model_destroy(None) == 0

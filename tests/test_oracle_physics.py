@@ -1,0 +1,267 @@
+"""Oracle functions whose reference source cannot be built here
+(satan_i/fwi3d_cpu_function_v1.4.c needs fftw3.h) are pinned by closed forms,
+invariants and brute force, each chosen so that a dropped term, a wrong sign, a
+wrong index or a swapped axis fails:
+
+* E/H update on linear fields: D-F = (c1 + 3 c2) * slope exactly, with
+  dx != dy != dz and a different slope per component;
+* discrete charge conservation: div-(eps' dE) = 0 and div+(dH) = 0 in the
+  interior for arbitrary fields (div(curl) = 0 of the staggered operators);
+* source scaling and the whole scheme: whole-space retarded dipole field
+  (the reference's own known answer efld_fic, satan_i/fwi3d_cpu_convert_v1.4.c:207-240);
+* transform: high-precision brute force (mpmath) of the laplaceToFreq sums,
+  FFTW_BACKWARD by numpy's inverse FFT, and the closed-form source spectrum
+  J(w) = 2 w0 exp((i-1) sqrt(w0 w) t0) exp(-i w0 w/(2 beta)) (convert.c:68-69);
+* waveform: antisymmetry and the derivative of the Gaussian.
+"""
+import mpmath as mp
+import numpy as np
+import pytest
+
+import oracle
+
+OM0 = 2 * np.pi
+
+
+def make_sim(ix=28, iy=24, iz=20, nc=5, dx=50.0, dy=40.0, dz=30.0, sig=None, seed=0):
+    sig = sig if sig is not None else np.exp(
+        np.random.default_rng(seed).uniform(np.log(0.05), np.log(2.0), (iz, iy, ix)))
+    dt, cmax = oracle.lattice_time_3d(float(sig.min()), OM0, dx, dy, dz, factor=0.99)
+    S = oracle.OracleSim(ix, iy, iz, nc, dx, dy, dz, dt, OM0, cmax)
+    S.set_sigma(sig)
+    S.init_cpml()
+    S.media_coeff_3d()
+    return S, sig, dt
+
+
+def interior(S, pad):
+    nc = S.p.ncpml
+    k = slice(nc + pad, S.p.iz - nc - pad)
+    j = slice(nc + pad, S.p.iy - nc - pad)
+    i = slice(nc + pad, S.p.ix - nc - pad)
+    return k, j, i
+
+
+def test_constants_are_the_reference_literals():
+    assert oracle.C1 == 1.125
+    assert oracle.C2 == -0.041669998317956924
+    assert oracle.RCOEF == 0.009999999776482582
+    assert oracle.ALPHA_MAX == 0.20000000298023224
+
+
+def test_e_update_on_linear_fields():
+    S, sig, dt = make_sim()
+    iz, iy, ix = S.shape
+    K, J, I = np.meshgrid(np.arange(iz), np.arange(iy), np.arange(ix), indexing="ij")
+    slopes = {"HX": (0.3, 0.7, 1.1), "HY": (1.3, -0.4, 0.9), "HZ": (-0.8, 0.5, 2.1)}
+    for nm, (a, b, c) in slopes.items():
+        S.arr(nm)[...] = a * I + b * J + c * K
+    e0 = {nm: np.random.default_rng(1).standard_normal(S.shape) for nm in ("EX", "EY", "EZ")}
+    for nm, v in e0.items():
+        S.arr(nm)[...] = v
+    S.e_field_cpml42(1)
+    g = oracle.C1 + 3 * oracle.C2          # D-(linear ramp of slope 1) per cell
+    eps = sig / 2.0 / OM0
+    dxs = (S.p.dx, S.p.dy, S.p.dz)
+    # d/dy Hz - d/dz Hy etc. in physical units
+    dHz = np.array(slopes["HZ"]) * g / dxs
+    dHy = np.array(slopes["HY"]) * g / dxs
+    dHx = np.array(slopes["HX"]) * g / dxs
+    curl = {"EX": dHz[1] - dHy[2], "EY": dHx[2] - dHz[0], "EZ": dHy[0] - dHx[1]}
+    sl = interior(S, 2)
+    for nm in ("EX", "EY", "EZ"):
+        want = e0[nm] + dt / eps * curl[nm]
+        np.testing.assert_allclose(S.arr(nm)[sl], want[sl], rtol=1e-12, atol=1e-12, err_msg=nm)
+
+
+def test_h_update_on_linear_fields():
+    S, sig, dt = make_sim(seed=2)
+    iz, iy, ix = S.shape
+    K, J, I = np.meshgrid(np.arange(iz), np.arange(iy), np.arange(ix), indexing="ij")
+    slopes = {"EX": (0.2, -0.9, 1.7), "EY": (1.4, 0.6, -0.3), "EZ": (-1.1, 0.8, 0.25)}
+    for nm, (a, b, c) in slopes.items():
+        S.arr(nm)[...] = (a * I + b * J + c * K) * 1e-3
+    h0 = {nm: np.random.default_rng(3).standard_normal(S.shape) for nm in ("HX", "HY", "HZ")}
+    for nm, v in h0.items():
+        S.arr(nm)[...] = v
+    S.h_field_cpml42(1)
+    g = oracle.C1 + 3 * oracle.C2
+    dxs = np.array((S.p.dx, S.p.dy, S.p.dz))
+    dE = {nm: np.array(v) * 1e-3 * g / dxs for nm, v in slopes.items()}
+    curl = {"HX": dE["EZ"][1] - dE["EY"][2], "HY": dE["EX"][2] - dE["EZ"][0],
+            "HZ": dE["EY"][0] - dE["EX"][1]}
+    db = dt / oracle.MU0_C
+    sl = interior(S, 2)
+    for nm in ("HX", "HY", "HZ"):
+        want = h0[nm] - db * curl[nm]
+        np.testing.assert_allclose(S.arr(nm)[sl], want[sl], rtol=1e-12, atol=1e-14, err_msg=nm)
+
+
+def dminus(F, axis, h):
+    c1, c2 = oracle.C1, oracle.C2
+    ax = {0: 2, 1: 1, 2: 0}[axis]    # x -> last array axis
+    r = lambda s: np.roll(F, s, axis=ax)
+    return (c1 * (F - r(1)) + c2 * (r(-1) - r(2))) / h
+
+
+def dplus(F, axis, h):
+    c1, c2 = oracle.C1, oracle.C2
+    ax = {0: 2, 1: 1, 2: 0}[axis]
+    r = lambda s: np.roll(F, s, axis=ax)
+    return (c1 * (r(-1) - F) + c2 * (r(-2) - r(1))) / h
+
+
+def test_e_update_conserves_discrete_charge():
+    S, sig, dt = make_sim(seed=4)
+    rng = np.random.default_rng(5)
+    for nm in oracle.FIELD_NAMES:
+        S.arr(nm)[...] = rng.standard_normal(S.shape)
+    e0 = [S.arr(nm).copy() for nm in ("EX", "EY", "EZ")]
+    S.e_field_cpml42(1)
+    eps = sig / 2.0 / OM0
+    d = [eps * (S.arr(nm) - e0[q]) for q, nm in enumerate(("EX", "EY", "EZ"))]
+    h = (S.p.dx, S.p.dy, S.p.dz)
+    div = dminus(d[0], 0, h[0]) + dminus(d[1], 1, h[1]) + dminus(d[2], 2, h[2])
+    scale = sum(np.abs(dminus(d[q], q, h[q])) for q in range(3))
+    sl = interior(S, 3)
+    assert np.max(np.abs(div[sl]) / scale[sl]) < 1e-12
+
+
+def test_h_update_conserves_discrete_flux():
+    S, sig, dt = make_sim(seed=6)
+    rng = np.random.default_rng(7)
+    for nm in oracle.FIELD_NAMES:
+        S.arr(nm)[...] = rng.standard_normal(S.shape)
+    h0 = [S.arr(nm).copy() for nm in ("HX", "HY", "HZ")]
+    S.h_field_cpml42(1)
+    d = [S.arr(nm) - h0[q] for q, nm in enumerate(("HX", "HY", "HZ"))]
+    h = (S.p.dx, S.p.dy, S.p.dz)
+    div = dplus(d[0], 0, h[0]) + dplus(d[1], 1, h[1]) + dplus(d[2], 2, h[2])
+    scale = sum(np.abs(dplus(d[q], q, h[q])) for q in range(3))
+    sl = interior(S, 3)
+    assert np.max(np.abs(div[sl]) / scale[sl]) < 1e-12
+
+
+def test_step_is_linear_and_zero_preserving():
+    S, sig, dt = make_sim(ix=24, iy=22, iz=20, nc=4, seed=8)
+    sigw = oracle.firstderiv_gauss(60, dt, 3.0)
+    src = (12, 11, 10)
+    rec = np.array([[8, 11, 10], [16, 11, 10]])
+    t1, _ = S.forward(0, sigw, src, 40, rec1=rec, it=60)
+    S.set_zero_eh()
+    t2, _ = S.forward(0, 2.0 * sigw, src, 40, rec1=rec, it=60)
+    np.testing.assert_allclose(t2, 2.0 * t1, rtol=1e-14, atol=1e-300)
+    S.set_zero_eh()
+    t0, _ = S.forward(0, np.zeros(60), src, 40, rec1=rec, it=60)
+    assert not np.any(t0)
+
+
+@pytest.mark.slow
+def test_whole_space_dipole_matches_retarded_field():
+    """x-dipole in a homogeneous sigma = 1 S/m whole space vs the retarded
+    dipole field of satan_i/fwi3d_cpu_convert_v1.4.c:207-240:
+    E_x = mu0/(4 pi r) [(nx^2-1) p'' + (3 nx^2 - 1)(c/r p' + (c/r)^2 p)],
+    p = sqrt(beta/pi) exp(-beta (t - r/c - t0)^2), p' = the source waveform."""
+    n, nc, h = 60, 10, 50.0
+    sig = np.ones((n, n, n))
+    S, _, dt = make_sim(ix=n, iy=n, iz=n, nc=nc, dx=h, dy=h, dz=h, sig=sig)
+    fmax = 3.0
+    nt = 260
+    s = oracle.firstderiv_gauss(nt + 1, dt, fmax)
+    c0 = n // 2
+    src1 = (c0 + 1, c0 + 1, c0 + 1)
+    offs = [(6, 0, 0), (0, 7, 0), (4, 4, 3), (-5, 3, -4)]
+    rec1 = np.array([[c0 + 1 + a, c0 + 1 + b, c0 + 1 + c] for a, b, c in offs])
+    tr, _ = S.forward(0, s, src1, nt, rec1=rec1, it=nt + 1)
+    cval = np.sqrt(2 * OM0 / oracle.MU0_C / 1.0)
+    beta, t0 = np.pi * fmax ** 2, np.pi / fmax
+    for r_i, (a, b, c) in enumerate(offs):
+        # Ex sits half a cell further along x than the cell centre
+        rx, ry, rz = (a) * h, b * h, c * h
+        r = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+        nx2 = (rx / r) ** 2
+        t = (np.arange(nt) + 1) * dt - r / cval - t0   # sample n holds E after n+1 updates
+        g = np.sqrt(beta / np.pi) * np.exp(-beta * t ** 2)
+        g1 = -2 * beta * t * g
+        g2 = g * (4 * beta ** 2 * t ** 2 - 2 * beta)
+        ex = oracle.MU0_C / (4 * np.pi * r) * ((nx2 - 1) * g2 + (3 * nx2 - 1) * (cval / r * g1 + (cval / r) ** 2 * g))
+        sim = tr[0, r_i, :nt]
+        err = np.linalg.norm(sim - ex) / np.linalg.norm(ex)
+        # 6-10 cells from a point source the lattice near field differs from the
+        # continuum one by a few percent; a sign or scale error gives >= 50 %.
+        assert err < 0.12, (offs[r_i], err)
+
+
+def test_dft_weights_closed_form():
+    dt = 7.7e-4
+    om = 2 * np.pi * np.array([0.1, 0.37, 1.0, 4.0])
+    w = oracle.dft_weights(3000, dt, OM0, om)
+    n = np.arange(3000)[:, None]
+    a = np.sqrt(OM0 * om)[None, :]
+    want = dt * np.exp((-1 + 1j) * a * n * dt)
+    np.testing.assert_allclose(w, want, rtol=2e-13, atol=0)
+
+
+def test_laplace_to_freq_bruteforce():
+    it, dt = 48, 3.3e-3
+    rng = np.random.default_rng(9)
+    tr = rng.standard_normal(it)
+    tr[-1] = 0.0
+    js = oracle.firstderiv_gauss(it, dt, 40.0)
+    strue = rng.standard_normal(it)
+    nb = 12
+    out = oracle.laplace_to_freq(tr, js, strue, dt, OM0, nb)
+    mp.mp.dps = 40
+    om = 2 * mp.pi / it / dt
+    tru = np.fft.ifft(strue) * it      # FFTW_BACKWARD definition
+    for n in range(nb):
+        a = mp.sqrt(OM0 * om * n)
+        ker = [dt * mp.exp((mp.mpc(-1, 1)) * a * k * dt) for k in range(it)]
+        e = mp.fsum(mp.mpf(tr[k]) * ker[k] for k in range(it))
+        if n == 0:
+            jv = mp.mpf(2 * OM0)
+        else:
+            jf = mp.sqrt(mp.mpf(-2 * OM0) / mp.mpc(0, 1) / (om * n))
+            jv = jf * mp.fsum(mp.mpf(js[k]) * ker[k] for k in range(it))
+        g = e / jv
+        o = g * mp.mpc(tru[n].real, tru[n].imag)
+        for key, v in (("E", e), ("J", jv), ("G", g), ("OBS", o)):
+            got = out[key][n]
+            ref = complex(v)
+            assert abs(got - ref) <= 1e-12 * abs(ref) + 1e-300, (key, n, got, ref)
+
+
+def test_source_spectrum_closed_form():
+    """Fictitious transform of the Gaussian-derivative current equals
+    2 w0 exp((i-1) sqrt(w0 w) t0) exp(-i w0 w / (2 beta)) up to O(dt^2)."""
+    fmax, dt, it = 3.0, 1e-3, 6000
+    js = oracle.firstderiv_gauss(it, dt, fmax)
+    out = oracle.laplace_to_freq(np.zeros(it), js, np.zeros(it), dt, OM0, 1)
+    om = 2 * np.pi / it / dt
+    beta, t0 = np.pi * fmax ** 2, np.pi / fmax
+    for n in (1, 3, 7, 12):
+        res = oracle.laplace_to_freq(np.zeros(it), js, np.zeros(it), dt, OM0, n + 1)["J"][n]
+        w = om * n
+        q = np.sqrt(OM0 * w)
+        want = 2 * OM0 * np.exp((1j - 1) * q * t0) * np.exp(-1j * OM0 * w / (2 * beta))
+        assert abs(res - want) / abs(want) < 5e-3, (n, res, want)
+    assert out["J"][0] == 2 * OM0
+
+
+def test_firstderiv_gauss_is_gaussian_derivative():
+    fmax, nt = 3.0, 25000
+    beta, t0 = np.pi * fmax ** 2, np.pi / fmax
+    dt = t0 / 10000.0                      # t0 on the sample grid
+    s = oracle.firstderiv_gauss(nt, dt, fmax)
+    t = np.arange(nt) * dt
+    g = np.sqrt(beta / np.pi) * np.exp(-beta * (t - t0) ** 2)
+    dg = np.gradient(g, dt)
+    np.testing.assert_allclose(s[1:-1], dg[1:-1], atol=1e-4 * np.abs(s).max())
+    assert abs(s[10000]) < 1e-9 * np.abs(s).max()
+    for m in (50, 700, 3000):
+        assert s[10000 + m] == pytest.approx(-s[10000 - m], rel=1e-9)
+
+
+def test_fft_backward_matches_numpy_ifft():
+    x = np.random.default_rng(11).standard_normal(37)
+    np.testing.assert_allclose(oracle.fft_backward_real(x), np.fft.ifft(x) * 37, atol=1e-12)
