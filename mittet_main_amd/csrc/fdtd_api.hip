@@ -483,8 +483,10 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
             for (int n = 0; n < sp->it; ++n) {
                 const double x = a * n * p.dt;
                 const double e = p.dt * std::exp(-x);
-                w[2 * ((size_t)n * s->nf + f)] = e * std::cos(x);
-                w[2 * ((size_t)n * s->nf + f) + 1] = e * std::sin(x);
+                double sn, cs;
+                sincos(x, &sn, &cs);  // one call: sin/cos and sincos may differ by an ulp
+                w[2 * ((size_t)n * s->nf + f)] = e * cs;
+                w[2 * ((size_t)n * s->nf + f) + 1] = e * sn;
             }
         }
         if (hipMemcpy(s->d_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice) != hipSuccess)

@@ -10,6 +10,7 @@
  * -ffp-contract=off: the evaluation order below is the reference's, and
  * fused multiply-adds would change the last bits.
  */
+#define _GNU_SOURCE
 #include "fwi3d_oracle.h"
 
 #include <complex.h>
@@ -486,8 +487,10 @@ void or_dft_weights(int it, double dt, double omega0, int nf,
         for (int n = 0; n < it; ++n) {
             const double x = a * n * dt;
             const double e = dt * exp(-x);
-            w[2 * ((long)n * nf + f)] = e * cos(x);
-            w[2 * ((long)n * nf + f) + 1] = e * sin(x);
+            double sn, cs;
+            sincos(x, &sn, &cs);
+            w[2 * ((long)n * nf + f)] = e * cs;
+            w[2 * ((long)n * nf + f) + 1] = e * sn;
         }
     }
 }

@@ -92,7 +92,7 @@ def test_backprop_sign_variants_bitwise():
         np.testing.assert_array_equal(shot.field(q), S.arr(nm), err_msg=nm)
 
 
-@pytest.mark.parametrize("ring,pieces", [(0, None), (8, (5, 3, 19, 8, 100)), (1, (7, 50))])
+@pytest.mark.parametrize("ring,pieces", [(0, None), (8, (5, 3, 19, 8, 100)), (1, (7, 50, 100))])
 def test_fused_run_bitwise(ring, pieces):
     it = 90
     m, shot, S, sig, signal, src, rec, om, dt = setup_pair(seed=5, it=it, ring=ring)
@@ -220,7 +220,12 @@ def test_full_size_256_linearity_and_zero():
     a.close()
     b = m.shot(81, 2.0 * wl.signal, wl.src, 0, rec)
     b.run()
-    np.testing.assert_array_equal(b.traces(), 2.0 * ta)
+    tb = b.traces()
+    # exact doubling wherever no intermediate underflows; subnormals (< 2.2e-308)
+    # carry absolute rounding only
+    normal = np.abs(ta) > 1e-290
+    np.testing.assert_array_equal(tb[normal], 2.0 * ta[normal])
+    assert np.abs(tb[~normal] - 2.0 * ta[~normal]).max(initial=0.0) < 1e-300
     b.close()
     z = m.shot(81, 0.0 * wl.signal, wl.src, 0, rec)
     z.run()
