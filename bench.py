@@ -92,10 +92,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--workload", default="roofline_256", choices=["roofline_256", "plain_64"])
     ap.add_argument("--ring", type=int, default=32)
+    ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes as C
+        from mittet_main_amd import _lib
+        ms = C.c_double(0.0)
+        nbytes = args.copy_calibration << 20
+        _lib.check(_lib.load().fwi3d_diag_copy(nbytes, 5, C.byref(ms)))
+        print(json.dumps({"copy_bytes": nbytes, "reps": 5, "ms": ms.value,
+                          "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shots shard across GPUs by construction; see DESIGN.md)", file=sys.stderr)
@@ -115,7 +127,7 @@ def main():
     model = Model(g)
     model.media_coeff_3d(wl.sigma)
     shot = model.shot(wl.it, wl.signal, wl.src, wl.src_comp, wl.receivers, wl.omegas,
-                      ring=args.ring)
+                      ring=args.ring, zchunk=args.zchunk)
     shot.step_n(args.warmup)
     shot.sync()
     log(f"[bench] warmup {args.warmup} steps done")
@@ -152,7 +164,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "interior": g.ix - 2 * g.ncpml, "ncpml": g.ncpml,
                    "cells": g.ncells, "nt": wl.it, "nf_vol": nf, "nrec": len(wl.receivers),
-                   "ring": args.ring, "sigma_sha256_16": configs.sigma_digest(wl.sigma)},
+                   "ring": args.ring, "zchunk": args.zchunk, "sigma_sha256_16": configs.sigma_digest(wl.sigma)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

@@ -73,6 +73,7 @@ typedef struct fwi3d_shot_params {
     int nf_vol;              /* volume E running-DFT frequencies (0: off) */
     const double *omega_vol; /* host, nf_vol angular frequencies (rad/s) */
     int ring;                /* E history slots between DFT flushes (0: library default) */
+    int zchunk;              /* z-cells per thread in the fused kernels (0: library default) */
 } fwi3d_shot_params;
 
 typedef struct fwi3d_model fwi3d_model;
@@ -137,6 +138,11 @@ int fwi3d_shot_sync(fwi3d_shot *s);
  * (its timing is a clock() around main, rec2rec.c:18,65). */
 int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms);
 int fwi3d_shot_step(const fwi3d_shot *s); /* steps done so far */
+/* Diagnostic with no counterpart in the reference: copies `bytes` bytes between
+ * two device buffers `reps` times with a plain streaming kernel (k_diag_copy),
+ * so hardware byte counters can be calibrated against a known transfer.
+ * ms_out: mean kernel time. */
+int fwi3d_diag_copy(long long bytes, int reps, double *ms_out);
 
 /* ---- readback -------------------------------------------------------------- */
 

@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmittet_fdtd.so")
+# MITTET_FDTD_LIB selects another build of the same library (A/B comparisons)
+LIB_PATH = os.environ.get("MITTET_FDTD_LIB") or os.path.join(HERE, "libmittet_fdtd.so")
 
 FWI3D_OK = 0
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "EBUDGET"}
@@ -38,7 +39,7 @@ class ShotParams(C.Structure):
                 ("signal", C.POINTER(C.c_double)),
                 ("nrec", C.c_int), ("rec_ijk", C.POINTER(C.c_int)),
                 ("nf_vol", C.c_int), ("omega_vol", C.POINTER(C.c_double)),
-                ("ring", C.c_int)]
+                ("ring", C.c_int), ("zchunk", C.c_int)]
 
 
 _SIGS = {
@@ -64,6 +65,7 @@ _SIGS = {
     "fwi3d_shot_sync": (C.c_int, [C.c_void_p]),
     "fwi3d_profile_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fwi3d_shot_step": (C.c_int, [C.c_void_p]),
+    "fwi3d_diag_copy": (C.c_int, [C.c_longlong, C.c_int, C.c_void_p]),
     "fwi3d_get_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fwi3d_set_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fwi3d_get_traces": (C.c_int, [C.c_void_p, C.c_void_p]),

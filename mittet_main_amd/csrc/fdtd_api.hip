@@ -9,6 +9,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -135,6 +136,19 @@ void pack_device_profile(const std::vector<double> &hp, int n, std::vector<doubl
 int tiles(int n, int t) { return (n + t - 1) / t; }
 
 constexpr int kDefaultChunk = 32;
+constexpr int kDefaultZChunk = 32;
+
+// Padding (doubles) between E ring slots and between DFT accumulator rows.
+// Unpadded, the slot and row strides of power-of-two-rich grids are multiples
+// of 256 KiB, so the many streams a DFT flush reads for one cell would share
+// their low address bits.  MITTET_FDTD_SLOT_PAD / MITTET_FDTD_ACC_PAD override.
+long long env_pad(const char *name, long long dflt)
+{
+    const char *v = std::getenv(name);
+    return v ? std::atoll(v) : dflt;
+}
+constexpr long long kSlotPad = 0;
+constexpr long long kAccPad = 0;
 
 }  // namespace
 
@@ -146,6 +160,7 @@ struct fwi3d_model {
     double *d_prof[3] = {nullptr, nullptr, nullptr};
     double *d_sig = nullptr;
     double *d_eps = nullptr;
+    double *d_cbi = nullptr;
     std::vector<double> sig;  // host copy (source scaling)
     int version = 0;          // bumps on every media_coeff_3d
     int nshots = 0;
@@ -164,7 +179,10 @@ struct fwi3d_shot {
     ShotDev desc{};
     ShotDev *d_desc = nullptr;
     bool desc_dirty = true;
-    int nblocks = 0, ncellblocks = 0;
+    bool zmarch = false;   // fused step uses the z-marching kernels (zchunk > 0)
+    int nblocks = 0;       // fused blocks (padded to a multiple of 8)
+    int nblocks_op = 0;    // per-cell blocks of the stand-alone operators
+    int ncellblocks = 0;
     int step = 0;          // steps done (host mirror of *d_step)
     int amp_version = -1;
     hipGraphExec_t graph = nullptr;
@@ -179,6 +197,7 @@ struct fwi3d_batch {
     ShotDev *d_desc = nullptr;
     int *d_pre = nullptr, *d_pre_cells = nullptr;
     int nblocks = 0, ncellblocks = 0;
+    bool zmarch = false;  // kernel family of the members (all equal)
     bool dirty = true;
     hipGraphExec_t graph = nullptr;
 };
@@ -228,12 +247,16 @@ int check_shot_ready(fwi3d_shot *s)
 
 // launches for local steps j in [0, L) followed by the DFT flush and step advance
 int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const int *pre_cells,
-                 int ncellblocks, int L, bool any_dft, bool any_rec, hipStream_t st)
+                 int ncellblocks, int L, bool any_dft, bool any_rec, bool zmarch, hipStream_t st)
 {
     for (int j = 0; j < L; ++j) {
-        HIP_TRY(mt::launch_efield(mt::M_UPD | mt::M_PSI | mt::M_SRC, d, ns, pre, nblocks, j, -1,
-                                  1, st));
-        HIP_TRY(mt::launch_hfield(mt::M_UPD | mt::M_PSI, d, ns, pre, nblocks, j, 1, st));
+        if (zmarch) {
+            HIP_TRY(mt::launch_efield_z(d, ns, pre, nblocks, j, 1, st));
+            HIP_TRY(mt::launch_hfield_z(d, ns, pre, nblocks, j, 1, st));
+        } else {
+            HIP_TRY(mt::launch_efield_f(d, ns, pre, nblocks, j, 1, st));
+            HIP_TRY(mt::launch_hfield_f(d, ns, pre, nblocks, j, 1, st));
+        }
         if (any_rec)
             HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
     }
@@ -245,11 +268,12 @@ int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const in
 
 int capture_chunk(hipGraphExec_t *out, const ShotDev *d, int ns, const int *pre, int nblocks,
                   const int *pre_cells, int ncellblocks, int L, bool any_dft, bool any_rec,
-                  hipStream_t st)
+                  bool zmarch, hipStream_t st)
 {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = launch_chunk(d, ns, pre, nblocks, pre_cells, ncellblocks, L, any_dft, any_rec, st);
+    int rc = launch_chunk(d, ns, pre, nblocks, pre_cells, ncellblocks, L, any_dft, any_rec, zmarch,
+                          st);
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc)
         return rc;
@@ -324,7 +348,8 @@ int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **o
         }
     }
     const size_t N = (size_t)p->ix * p->iy * p->iz;
-    if (dalloc(&m->d_sig, N) != hipSuccess || dalloc(&m->d_eps, N) != hipSuccess) {
+    if (dalloc(&m->d_sig, N) != hipSuccess || dalloc(&m->d_eps, N) != hipSuccess ||
+        dalloc(&m->d_cbi, N) != hipSuccess) {
         fwi3d_model_destroy(m);
         return fail(FWI3D_ENOMEM, "cannot allocate %zu-cell model", N);
     }
@@ -342,6 +367,7 @@ int fwi3d_model_destroy(fwi3d_model *m)
         dfree(q);
     dfree(m->d_sig);
     dfree(m->d_eps);
+    dfree(m->d_cbi);
     if (m->own_stream && m->stream)
         (void)hipStreamDestroy(m->stream);
     delete m;
@@ -368,7 +394,7 @@ int fwi3d_media_coeff_3d(fwi3d_model *m, const double *sigma)
                         sigma[c]);
     m->sig.assign(sigma, sigma + N);
     HIP_TRY(hipMemcpyAsync(m->d_sig, sigma, sizeof(double) * N, hipMemcpyHostToDevice, m->stream));
-    HIP_TRY(mt::launch_eps(m->d_sig, m->d_eps, (int)N, m->p.omega0, m->stream));
+    HIP_TRY(mt::launch_eps(m->d_sig, m->d_eps, m->d_cbi, (int)N, m->p.omega0, m->p.dt, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
     m->version++;
     return FWI3D_OK;
@@ -412,6 +438,8 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
             return fail(FWI3D_EINVAL, "receiver %d at (%d,%d,%d) outside the grid", r, q[0], q[1],
                         q[2]);
     }
+    if (sp->zchunk < 0)
+        return fail(FWI3D_EINVAL, "zchunk must be >= 0");
     if (sp->nf_vol < 0 || (sp->nf_vol > 0 && !sp->omega_vol))
         return fail(FWI3D_EINVAL, "bad volume DFT frequency list");
     for (int f = 0; f < sp->nf_vol; ++f)
@@ -447,7 +475,13 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     const size_t nc2 = 2 * (size_t)p.ncpml;
     const size_t slab[3] = {(size_t)p.iz * p.iy * nc2, (size_t)p.iz * nc2 * p.ix,
                             nc2 * p.iy * p.ix};
-    bool ok = dalloc(&s->d_e, 3 * N * ring) == hipSuccess &&
+    const long long sstride = 3 * (long long)N + (ring > 1 ? env_pad("MITTET_FDTD_SLOT_PAD", kSlotPad) : 0);
+    const long long astride = (long long)N + env_pad("MITTET_FDTD_ACC_PAD", kAccPad);
+    if (sstride < 3 * (long long)N || astride < (long long)N) {
+        delete s;
+        return fail(FWI3D_EINVAL, "negative stride padding");
+    }
+    bool ok = dalloc(&s->d_e, (size_t)sstride * ring) == hipSuccess &&
               dalloc(&s->d_h, 3 * N) == hipSuccess &&
               dalloc(&s->d_psi[0], mt::PSI_N * slab[0]) == hipSuccess &&
               dalloc(&s->d_psi[1], mt::PSI_N * slab[1]) == hipSuccess &&
@@ -459,7 +493,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
               dalloc(&s->d_desc, 1) == hipSuccess;
     if (ok && s->nf)
         ok = dalloc(&s->d_w, 2 * (size_t)s->nf * sp->it) == hipSuccess &&
-             dalloc(&s->d_acc, 2 * 3 * (size_t)s->nf * N) == hipSuccess;
+             dalloc(&s->d_acc, 2 * 3 * (size_t)s->nf * astride) == hipSuccess;
     if (!ok) {
         m->nshots++;
         fwi3d_shot_destroy(s);
@@ -520,7 +554,18 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
         d.slab[a] = (long long)slab[a];
     }
     d.eps = m->d_eps;
+    d.cbi = m->d_cbi;
+    d.kb = sp->zchunk > 0 ? std::min(sp->zchunk, p.iz) : std::min(kDefaultZChunk, p.iz);
+    d.ztx = tiles(p.ix, mt::ZX);
+    d.zty = tiles(p.iy, mt::ZY);
+    d.nkz = tiles(p.iz, d.kb);
+    d.zb = d.ztx * d.zty * d.nkz;
+    d.zb_pad = tiles(d.zb, 8) * 8;
+    d.fb = d.ztx * d.zty * p.iz;
+    d.fb_pad = tiles(d.fb, 8) * 8;
     d.e = s->d_e;
+    d.sstride = sstride;
+    d.astride = astride;
     d.ring = ring;
     d.cur = 0;
     d.h = s->d_h;
@@ -534,7 +579,9 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.w = s->d_w;
     d.acc = s->d_acc;
     d.step = s->d_step;
-    s->nblocks = d.nbx * d.nby * p.iz;
+    s->nblocks_op = d.nbx * d.nby * p.iz;
+    s->zmarch = sp->zchunk > 0;
+    s->nblocks = s->zmarch ? d.zb_pad : d.fb_pad;
     s->ncellblocks = tiles((int)N, 256);
     if ((rc = upload_desc(s)) || (rc = fwi3d_set_zero_eh(s))) {
         fwi3d_shot_destroy(s);
@@ -580,13 +627,13 @@ int fwi3d_set_zero_eh(fwi3d_shot *s)
         return fail(FWI3D_ESTATE, "shot is in a batch");
     const fwi3d_grid_params &p = s->m->p;
     const size_t N = (size_t)p.ix * p.iy * p.iz;
-    HIP_TRY(mt::launch_fill(s->d_e, 3 * N * s->ring, 0.0, s->st));
+    HIP_TRY(mt::launch_fill(s->d_e, (size_t)s->desc.sstride * s->ring, 0.0, s->st));
     HIP_TRY(mt::launch_fill(s->d_h, 3 * N, 0.0, s->st));
     for (int a = 0; a < 3; ++a)
         HIP_TRY(mt::launch_fill(s->d_psi[a], mt::PSI_N * (size_t)s->desc.slab[a], 0.0, s->st));
     HIP_TRY(mt::launch_fill(s->d_traces, std::max<size_t>(1, 6 * (size_t)s->nrec * s->it), 0.0, s->st));
     if (s->nf)
-        HIP_TRY(mt::launch_fill(s->d_acc, 2 * 3 * (size_t)s->nf * N, 0.0, s->st));
+        HIP_TRY(mt::launch_fill(s->d_acc, 2 * 3 * (size_t)s->nf * s->desc.astride, 0.0, s->st));
     static const int kZero = 0;
     HIP_TRY(hipMemcpyAsync(s->d_step, &kZero, sizeof(int), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -609,19 +656,19 @@ static int single_op(fwi3d_shot *s, int which, int sign, int step)
         return fail(FWI3D_EINVAL, "sign must be +1 or -1");
     switch (which) {
     case 0:
-        HIP_TRY(mt::launch_efield(mt::M_UPD | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+        HIP_TRY(mt::launch_efield(mt::M_UPD | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks_op, 0,
                                   0, sign, s->st));
         break;
     case 1:
-        HIP_TRY(mt::launch_efield(mt::M_PSI | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+        HIP_TRY(mt::launch_efield(mt::M_PSI | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks_op, 0,
                                   0, 1, s->st));
         break;
     case 2:
-        HIP_TRY(mt::launch_hfield(mt::M_UPD | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+        HIP_TRY(mt::launch_hfield(mt::M_UPD | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks_op, 0,
                                   sign, s->st));
         break;
     case 3:
-        HIP_TRY(mt::launch_hfield(mt::M_PSI | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks, 0,
+        HIP_TRY(mt::launch_hfield(mt::M_PSI | mt::M_INPLACE, s->d_desc, 1, s->d_pre, s->nblocks_op, 0,
                                   1, s->st));
         break;
     case 4:
@@ -662,12 +709,12 @@ int fwi3d_step_n(fwi3d_shot *s, int nsteps)
         if (L == chunk) {
             if (!s->graph &&
                 (rc = capture_chunk(&s->graph, s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre,
-                                    s->ncellblocks, L, dft, rec, s->st)))
+                                    s->ncellblocks, L, dft, rec, s->zmarch, s->st)))
                 return rc;
             HIP_TRY(hipGraphLaunch(s->graph, s->st));
         } else {
             if ((rc = launch_chunk(s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre, s->ncellblocks, L,
-                                   dft, rec, s->st)))
+                                   dft, rec, s->zmarch, s->st)))
                 return rc;
         }
         s->step += L;
@@ -712,11 +759,16 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
     while (left > 0 && !rc) {
         const int L = std::min(chunk, left);
         for (int j = 0; j < L && !rc; ++j) {
-            rc = timed(0, [&] { return mt::launch_efield(mt::M_UPD | mt::M_PSI | mt::M_SRC, s->d_desc, 1,
-                                                         s->d_pre, s->nblocks, j, -1, 1, s->st); });
+            rc = timed(0, [&] {
+                return s->zmarch ? mt::launch_efield_z(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st)
+                                 : mt::launch_efield_f(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st);
+            });
             if (!rc)
-                rc = timed(1, [&] { return mt::launch_hfield(mt::M_UPD | mt::M_PSI, s->d_desc, 1, s->d_pre,
-                                                             s->nblocks, j, 1, s->st); });
+                rc = timed(1, [&] {
+                    return s->zmarch
+                               ? mt::launch_hfield_z(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st)
+                               : mt::launch_hfield_f(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st);
+                });
             if (!rc && s->nrec)
                 rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
         }
@@ -743,6 +795,43 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
     return FWI3D_OK;
 }
 
+int fwi3d_diag_copy(long long bytes, int reps, double *ms_out)
+{
+    if (bytes < 16 || reps <= 0)
+        return fail(FWI3D_EINVAL, "bad copy size");
+    const size_t count = (size_t)(bytes / 16) * 2;
+    double *a = nullptr, *b = nullptr;
+    if (dalloc(&a, count) != hipSuccess || dalloc(&b, count) != hipSuccess) {
+        dfree(a);
+        return fail(FWI3D_ENOMEM, "copy buffers");
+    }
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FWI3D_OK;
+    float total = 0.f;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(FWI3D_EDEVICE, "stream/event creation");
+    if (!rc && mt::launch_fill(a, count, 1.0, st) != hipSuccess)
+        rc = fail(FWI3D_EDEVICE, "fill");
+    for (int r = 0; r < reps && !rc; ++r) {
+        float t = 0.f;
+        if (hipEventRecord(e0, st) != hipSuccess || mt::launch_copy(a, b, count, st) != hipSuccess ||
+            hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+            rc = fail(FWI3D_EDEVICE, "copy launch");
+        total += t;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    dfree(a);
+    dfree(b);
+    if (!rc && ms_out)
+        *ms_out = total / reps;
+    return rc;
+}
+
 int fwi3d_run(fwi3d_shot *s)
 {
     if (!s)
@@ -765,7 +854,7 @@ int fwi3d_get_field(fwi3d_shot *s, int which, double *out)
     if (!s || !out || which < 0 || which > 5)
         return fail(FWI3D_EINVAL, "bad field request");
     const size_t N = (size_t)s->desc.n;
-    const double *src = which < 3 ? s->d_e + ((size_t)s->desc.cur * 3 + which) * N
+    const double *src = which < 3 ? s->d_e + (size_t)s->desc.cur * s->desc.sstride + (size_t)which * N
                                   : s->d_h + (size_t)(which - 3) * N;
     HIP_TRY(hipMemcpyAsync(out, src, sizeof(double) * N, hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -777,7 +866,7 @@ int fwi3d_set_field(fwi3d_shot *s, int which, const double *in)
     if (!s || !in || which < 0 || which > 5)
         return fail(FWI3D_EINVAL, "bad field request");
     const size_t N = (size_t)s->desc.n;
-    double *dst = which < 3 ? s->d_e + ((size_t)s->desc.cur * 3 + which) * N
+    double *dst = which < 3 ? s->d_e + (size_t)s->desc.cur * s->desc.sstride + (size_t)which * N
                             : s->d_h + (size_t)(which - 3) * N;
     HIP_TRY(hipMemcpyAsync(dst, in, sizeof(double) * N, hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -802,8 +891,10 @@ int fwi3d_get_volume_dft(fwi3d_shot *s, double *out)
         return fail(FWI3D_EINVAL, "null argument");
     if (!s->nf)
         return fail(FWI3D_ESTATE, "shot has no volume DFT frequencies");
-    HIP_TRY(hipMemcpyAsync(out, s->d_acc, sizeof(double) * 2 * 3 * (size_t)s->nf * s->desc.n,
-                           hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpy2DAsync(out, sizeof(double) * 2 * (size_t)s->desc.n, s->d_acc,
+                             sizeof(double) * 2 * (size_t)s->desc.astride,
+                             sizeof(double) * 2 * (size_t)s->desc.n, 3 * (size_t)s->nf,
+                             hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     return FWI3D_OK;
 }
@@ -1006,6 +1097,9 @@ int fwi3d_batch_add(fwi3d_batch *b, fwi3d_shot *s)
         return fail(FWI3D_EBUDGET, "batch holds at most %d shots", b->max_shots);
     if (s->ring != 1 && s->ring != b->chunk)
         return fail(FWI3D_EINVAL, "shot ring %d must equal the batch chunk %d", s->ring, b->chunk);
+    if (!b->shots.empty() && s->zmarch != b->zmarch)
+        return fail(FWI3D_EINVAL, "batch members must all use (or all not use) z-marching kernels");
+    b->zmarch = s->zmarch;
     int rc = ensure_amp(s);
     if (rc)
         return rc;
@@ -1081,11 +1175,12 @@ int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
         const int L = std::min(b->chunk, left);
         if (L == b->chunk) {
             if (!b->graph && (rc = capture_chunk(&b->graph, b->d_desc, ns, b->d_pre, b->nblocks,
-                                                 b->d_pre_cells, b->ncellblocks, L, dft, rec, b->st)))
+                                                 b->d_pre_cells, b->ncellblocks, L, dft, rec,
+                                                 b->zmarch, b->st)))
                 return rc;
             HIP_TRY(hipGraphLaunch(b->graph, b->st));
         } else if ((rc = launch_chunk(b->d_desc, ns, b->d_pre, b->nblocks, b->d_pre_cells,
-                                      b->ncellblocks, L, dft, rec, b->st))) {
+                                      b->ncellblocks, L, dft, rec, b->zmarch, b->st))) {
             return rc;
         }
         left -= L;

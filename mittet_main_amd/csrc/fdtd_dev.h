@@ -16,6 +16,8 @@ enum { PSI_EP = 0, PSI_EM, PSI_HP, PSI_HM, PSI_N };
 constexpr int TX = 64;   // threads along x (one wavefront row, 512 B coalesced)
 constexpr int TY = 4;    // rows per block
 constexpr int RING_MAX = 32;
+constexpr int ZX = 32;   // z-marching tile: 32 x 8 columns, one thread per column
+constexpr int ZY = 8;
 
 // Everything a kernel needs about one shot.  Lives in device memory; blocks
 // find their shot through an exclusive prefix of per-shot block counts.
@@ -28,7 +30,14 @@ struct ShotDev {
     double dl[3];
     const double *prof[3];    // [PR_N][n_a]
     const double *eps;        // eps' = sigma/2/omega0 per cell
-    double *e;                // E ring: slot q, comp d at e + (q*3 + d)*n
+    const double *cbi;        // dt/eps' per cell (cb where the CPML sigma is zero)
+    int kb;                   // z-chunk length of the z-marching kernels
+    int ztx, zty, nkz;        // z-marching tiles along x, y and z-chunks
+    int zb, zb_pad;           // z-marching blocks, padded to a multiple of 8
+    int fb, fb_pad;           // per-cell fused blocks (one plane each), padded to 8
+    double *e;                // E ring: slot q, comp d at e + q*sstride + d*n
+    long long sstride;        // doubles between ring slots (3n + padding)
+    long long astride;        // complex elements between accumulator rows (n + padding)
     int ring, cur;            // ring slots; slot that holds E at chunk start
     double *h;                // H [3][n], updated in place
     double *psi[3];           // [PSI_N][slab[a]]
@@ -41,7 +50,7 @@ struct ShotDev {
     double *traces;           // [6][nrec][it]
     int nf;
     const double *w;          // complex [it][nf]
-    double *acc;              // complex [3][nf][n]
+    double *acc;              // complex [3][nf] rows of astride
     int *step;                // absolute step of local index j = 0 of the current chunk
 };
 
@@ -54,6 +63,14 @@ hipError_t launch_efield(int mode, const ShotDev *d, int ns, const int *pre, int
 hipError_t launch_hfield(int mode, const ShotDev *d, int ns, const int *pre, int nblocks,
                          int j, int sign, hipStream_t st);
 hipError_t launch_source(const ShotDev *d, int n, hipStream_t st);
+hipError_t launch_efield_z(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
+hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
+hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
+hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
 hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
                             hipStream_t st);
 hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int L,
@@ -65,6 +82,8 @@ hipError_t launch_receiver_dft(const double *traces, int ncomp_rows, int it, dou
 hipError_t launch_bin_dft(const double *traces, int nrows, int it, double dt, double omega0,
                           int nbins, double *out, hipStream_t st);
 hipError_t launch_fill(double *p, size_t count, double v, hipStream_t st);
-hipError_t launch_eps(const double *sig, double *eps, int n, double omega0, hipStream_t st);
+hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t st);
+hipError_t launch_eps(const double *sig, double *eps, double *cbi, int n, double omega0, double dt,
+                      hipStream_t st);
 
 }  // namespace mt

@@ -17,16 +17,43 @@
 // with -ffp-contract=off so results equal the reference's bit for bit.
 #include "fdtd_dev.h"
 
+// Minimum resident waves per SIMD requested for the streaming kernels; it caps
+// their VGPR budget (512 / waves) so enough waves are in flight to cover HBM
+// latency.
+#ifndef MT_Z_WAVES
+#define MT_Z_WAVES 6
+#endif
+#ifndef MT_FLUSH_WAVES
+#define MT_FLUSH_WAVES 5
+#endif
+#ifndef MT_FLUSH_G
+#define MT_FLUSH_G 8
+#endif
+#ifndef MT_TILE_ORDER
+#define MT_TILE_ORDER 1
+#endif
+
+// Pointers read out of the shot descriptor are generic; casting them to the
+// global address space lets the compiler emit global_load/store instead of
+// flat accesses (which also count against the LDS wait counter).
+#define GLOBAL __attribute__((address_space(1)))
+
 namespace mt {
 
-__device__ __forceinline__ double dminus(const double *__restrict__ F, int c, int s, double c1,
-                                         double c2)
+__device__ __forceinline__ const GLOBAL double *gc(const double *p) { return (const GLOBAL double *)p; }
+__device__ __forceinline__ GLOBAL double *gm(double *p) { return (GLOBAL double *)p; }
+__device__ __forceinline__ const GLOBAL int *gc(const int *p) { return (const GLOBAL int *)p; }
+__device__ __forceinline__ const GLOBAL double *gc(const GLOBAL double *p) { return p; }
+__device__ __forceinline__ GLOBAL double *gm(GLOBAL double *p) { return p; }
+
+__device__ __forceinline__ double dminus(const GLOBAL double *__restrict__ F, int c, int s,
+                                         double c1, double c2)
 {
     return c1 * F[c] - c1 * F[c - s] + c2 * F[c + s] - c2 * F[c - 2 * s];
 }
 
-__device__ __forceinline__ double dplus(const double *__restrict__ F, int c, int s, double c1,
-                                        double c2)
+__device__ __forceinline__ double dplus(const GLOBAL double *__restrict__ F, int c, int s,
+                                        double c1, double c2)
 {
     return c1 * F[c + s] - c1 * F[c] + c2 * F[c + 2 * s] - c2 * F[c - s];
 }
@@ -45,47 +72,26 @@ __device__ __forceinline__ int slab_pos(int r, int n, int nc)
     return r < nc ? r : r - (n - 2 * nc);
 }
 
+// One cell of the E update + CPML-E corrections + source, general path.
 template <int MODE>
-__global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D, int ns,
-                                                   const int *__restrict__ pre, int j, int nover,
-                                                   int sign)
+__device__ __forceinline__ void efield_cell(const ShotDev &S, int i, int jj, int k, int n, int si,
+                                            int so, int sign)
 {
-    const int s = find_shot(pre, ns, blockIdx.x);
-    const ShotDev &S = D[s];
-    int n = nover;
-    if (n < 0) {
-        n = *S.step + j;
-        if (n >= S.it - 1)
-            return;  // this shot has finished its time loop
-    }
-    const int b = blockIdx.x - pre[s];
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc, N = S.n;
-    const int bx = b % S.nbx, t = b / S.nbx;
-    const int by = t % S.nby, k = t / S.nby;
-    const int i = bx * TX + threadIdx.x, jj = by * TY + threadIdx.y;
-    if (i >= ix || jj >= iy)
-        return;
     const int sy = ix, sz = ix * iy;
     const int c = (k * iy + jj) * ix + i;
 
-    int si, so;
-    if (MODE & M_INPLACE) {
-        si = so = S.cur;
-    } else {
-        si = (S.cur + j) % S.ring;
-        so = (S.cur + j + 1) % S.ring;
-    }
-    const double *__restrict__ Ein = S.e + (size_t)si * 3 * N;
-    double *__restrict__ Eout = S.e + (size_t)so * 3 * N;
-    const double *__restrict__ Hx = S.h;
-    const double *__restrict__ Hy = S.h + N;
-    const double *__restrict__ Hz = S.h + 2 * (size_t)N;
+    const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
+    GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Hx = gc(S.h);
+    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
+    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * (size_t)N);
     double ex = Ein[c], ey = Ein[N + c], ez = Ein[2 * (size_t)N + c];
 
     const double c1 = S.c1, c2 = S.c2, dt = S.dt;
-    const double *__restrict__ px = S.prof[0];
-    const double *__restrict__ py = S.prof[1];
-    const double *__restrict__ pz = S.prof[2];
+    const GLOBAL double *__restrict__ px = gc(S.prof[0]);
+    const GLOBAL double *__restrict__ py = gc(S.prof[1]);
+    const GLOBAL double *__restrict__ pz = gc(S.prof[2]);
 
     const bool rx1 = i >= 1 && i < ix - 1, rx2 = i >= 2 && i < ix - 1;
     const bool ry1 = jj >= 1 && jj < iy - 1, ry2 = jj >= 2 && jj < iy - 1;
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
     if (upz || fx) hyx = dminus(Hy, c, 1, c1, c2);
     if (upz || fy) hxy = dminus(Hx, c, sy, c1, c2);
 
-    const double eps2 = S.eps[c];
+    const double eps2 = gc(S.eps)[c];
     const double sex = px[PR_ESIG * ix + i] * dt / (2.0 * eps2);
     const double sey = py[PR_ESIG * iy + jj] * dt / (2.0 * eps2);
     const double sez = pz[PR_ESIG * iz + k] * dt / (2.0 * eps2);
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
         ez = sign > 0 ? ca * ez + cbz * cu : ca * ez - cbz * cu;
     }
     if (fx) {
-        double *__restrict__ P = S.psi[0];
+        GLOBAL double *__restrict__ P = gm(S.psi[0]);
         const size_t L = (size_t)S.slab[0];
         const size_t q = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
         const double be = px[PR_BE * ix + i], ce = px[PR_CE * ix + i], h = S.dl[0];
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
         ey -= cby * p1;
     }
     if (fy) {
-        double *__restrict__ P = S.psi[1];
+        GLOBAL double *__restrict__ P = gm(S.psi[1]);
         const size_t L = (size_t)S.slab[1];
         const size_t q = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
         const double be = py[PR_BE * iy + jj], ce = py[PR_CE * iy + jj], h = S.dl[1];
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
         ez -= cbz * p1;
     }
     if (fz) {
-        double *__restrict__ P = S.psi[2];
+        GLOBAL double *__restrict__ P = gm(S.psi[2]);
         const size_t L = (size_t)S.slab[2];
         const size_t q = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
         const double be = pz[PR_BE * iz + k], ce = pz[PR_CE * iz + k], h = S.dl[2];
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
         ex -= cbx * p1;
     }
     if ((MODE & M_SRC) && c == S.src_cell) {
-        const double a = S.amp[n];
+        const double a = gc(S.amp)[n];
         if (S.src_comp == 0)
             ex -= a;
         else if (S.src_comp == 1)
@@ -182,41 +188,55 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
     Eout[2 * (size_t)N + c] = ez;
 }
 
+
 template <int MODE>
-__global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D, int ns,
-                                                   const int *__restrict__ pre, int j, int sign)
+__global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D, int ns,
+                                                   const int *__restrict__ pre, int j, int nover,
+                                                   int sign)
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    int so;
-    if (MODE & M_INPLACE) {
-        so = S.cur;
-    } else {
-        if (*S.step + j >= S.it - 1)
-            return;
-        so = (S.cur + j + 1) % S.ring;
+    int n = nover;
+    if (n < 0) {
+        n = *S.step + j;
+        if (n >= S.it - 1)
+            return;  // this shot has finished its time loop
     }
     const int b = blockIdx.x - pre[s];
-    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc, N = S.n;
     const int bx = b % S.nbx, t = b / S.nbx;
     const int by = t % S.nby, k = t / S.nby;
     const int i = bx * TX + threadIdx.x, jj = by * TY + threadIdx.y;
-    if (i >= ix || jj >= iy)
+    if (i >= S.ix || jj >= S.iy)
         return;
+    int si, so;
+    if (MODE & M_INPLACE) {
+        si = so = S.cur;
+    } else {
+        si = (S.cur + j) % S.ring;
+        so = (S.cur + j + 1) % S.ring;
+    }
+    efield_cell<MODE>(S, i, jj, k, n, si, so, sign);
+}
+
+// One cell of the H update + CPML-H corrections, general path.
+template <int MODE>
+__device__ __forceinline__ void hfield_cell(const ShotDev &S, int i, int jj, int k, int so, int sign)
+{
+    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc, N = S.n;
     const int sy = ix, sz = ix * iy;
     const int c = (k * iy + jj) * ix + i;
 
-    const double *__restrict__ Ex = S.e + (size_t)so * 3 * N;
-    const double *__restrict__ Ey = Ex + N;
-    const double *__restrict__ Ez = Ex + 2 * (size_t)N;
-    double *__restrict__ Hx = S.h;
-    double *__restrict__ Hy = S.h + N;
-    double *__restrict__ Hz = S.h + 2 * (size_t)N;
+    const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Ey = gc(Ex + N);
+    const GLOBAL double *__restrict__ Ez = gc(Ex + 2 * (size_t)N);
+    GLOBAL double *__restrict__ Hx = gm(S.h);
+    GLOBAL double *__restrict__ Hy = gm(S.h + N);
+    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * (size_t)N);
 
     const double c1 = S.c1, c2 = S.c2, dt = S.dt;
-    const double *__restrict__ px = S.prof[0];
-    const double *__restrict__ py = S.prof[1];
-    const double *__restrict__ pz = S.prof[2];
+    const GLOBAL double *__restrict__ px = gc(S.prof[0]);
+    const GLOBAL double *__restrict__ py = gc(S.prof[1]);
+    const GLOBAL double *__restrict__ pz = gc(S.prof[2]);
 
     const bool ax1 = i >= 1 && i < ix - 1, ax2 = i >= 1 && i < ix - 2;
     const bool ay1 = jj >= 1 && jj < iy - 1, ay2 = jj >= 1 && jj < iy - 2;
@@ -247,7 +267,7 @@ __global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D
     const double msx = px[PR_MSIG * ix + i], msy = py[PR_MSIG * iy + jj], msz = pz[PR_MSIG * iz + k];
     double smx = 0.0, smy = 0.0, smz = 0.0;
     if (msx != 0.0 || msy != 0.0 || msz != 0.0) {
-        const double eps2 = S.eps[c];
+        const double eps2 = gc(S.eps)[c];
         smx = msx * dt / (2.0 * eps2);
         smy = msy * dt / (2.0 * eps2);
         smz = msz * dt / (2.0 * eps2);
@@ -273,7 +293,7 @@ __global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D
         hz = sign > 0 ? da * hz - dbz * cu : da * hz + dbz * cu;
     }
     if (gx) {
-        double *__restrict__ P = S.psi[0];
+        GLOBAL double *__restrict__ P = gm(S.psi[0]);
         const size_t L = (size_t)S.slab[0];
         const size_t q = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
         const double bh = px[PR_BH * ix + i], ch = px[PR_CH * ix + i], h = S.dl[0];
@@ -285,7 +305,7 @@ __global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D
         hz -= dbz * p1;
     }
     if (gy) {
-        double *__restrict__ P = S.psi[1];
+        GLOBAL double *__restrict__ P = gm(S.psi[1]);
         const size_t L = (size_t)S.slab[1];
         const size_t q = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
         const double bh = py[PR_BH * iy + jj], ch = py[PR_CH * iy + jj], h = S.dl[1];
@@ -297,7 +317,7 @@ __global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D
         hx -= dbx * p1;
     }
     if (gz) {
-        double *__restrict__ P = S.psi[2];
+        GLOBAL double *__restrict__ P = gm(S.psi[2]);
         const size_t L = (size_t)S.slab[2];
         const size_t q = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
         const double bh = pz[PR_BH * iz + k], ch = pz[PR_CH * iz + k], h = S.dl[2];
@@ -313,14 +333,38 @@ __global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D
     Hz[c] = hz;
 }
 
+
+template <int MODE>
+__global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D, int ns,
+                                                   const int *__restrict__ pre, int j, int sign)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    int so;
+    if (MODE & M_INPLACE) {
+        so = S.cur;
+    } else {
+        if (*S.step + j >= S.it - 1)
+            return;
+        so = (S.cur + j + 1) % S.ring;
+    }
+    const int b = blockIdx.x - pre[s];
+    const int bx = b % S.nbx, t = b / S.nbx;
+    const int by = t % S.nby, k = t / S.nby;
+    const int i = bx * TX + threadIdx.x, jj = by * TY + threadIdx.y;
+    if (i >= S.ix || jj >= S.iy)
+        return;
+    hfield_cell<MODE>(S, i, jj, k, so, sign);
+}
+
 // read_source_3d as a stand-alone operator (in place on the current slot)
 __global__ void k_source(const ShotDev *__restrict__ D, int n)
 {
     const ShotDev &S = D[0];
     if (S.src_cell < 0 || n < 0 || n >= S.it - 1)
         return;
-    double *E = S.e + ((size_t)S.cur * 3 + S.src_comp) * S.n;
-    E[S.src_cell] -= S.amp[n];
+    double *E = S.e + (size_t)S.cur * S.sstride + (size_t)S.src_comp * S.n;
+    E[S.src_cell] -= gc(S.amp)[n];
 }
 
 // Receiver gather: E (from the slot written this step) and H after the step.
@@ -339,47 +383,127 @@ __global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int
     for (int q = threadIdx.x; q < tot; q += blockDim.x) {
         const int comp = q / S.nrec, r = q - comp * S.nrec;
         const int cell = S.rec_cell[r];
-        const double v = comp < 3 ? S.e[((size_t)so * 3 + comp) * S.n + cell]
+        const double v = comp < 3 ? S.e[(size_t)so * S.sstride + (size_t)comp * S.n + cell]
                                   : S.h[(size_t)(comp - 3) * S.n + cell];
         S.traces[((size_t)comp * S.nrec + r) * S.it + n] = v;
     }
 }
 
-// Volume running DFT over the last L ring slots: acc += E[n] * w[n] in step order.
-__global__ __launch_bounds__(256) void k_flush(const ShotDev *__restrict__ D, int ns,
-                                               const int *__restrict__ pre, int L)
+// Volume running DFT over the last L ring slots: acc += E[n] * w[n] in step
+// order.  All NF accumulators of a component stay in registers while the slots
+// stream past, so every frequency is an independent dependency chain; each
+// accumulator still receives its terms in ascending step order (bit-identical
+// to the per-step running sum).
+// The chunk's weights (<= RING_MAX steps x FLUSH_NF_MAX frequencies) are staged
+// in LDS once per block and read back as wave-uniform LDS broadcasts.
+constexpr int FLUSH_NF_MAX = 8;
+
+template <int NF>
+__device__ __forceinline__ void flush_cell(const ShotDev &S, int c, int Lv,
+                                           const double2 *__restrict__ wsh)
 {
+    constexpr int G = MT_FLUSH_G;  // E samples loaded together: G loads in flight per lane
+    const int N = S.n, ring = S.ring, cur = S.cur;
+    GLOBAL double *__restrict__ A = gm(S.acc);
+    for (int d = 0; d < 3; ++d) {
+        double2 a[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            a[f].x = A[2 * (((size_t)d * NF + f) * S.astride + c)];
+            a[f].y = A[2 * (((size_t)d * NF + f) * S.astride + c) + 1];
+        }
+        for (int t0 = 0; t0 < Lv; t0 += G) {
+            double e[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int slot = (cur + 1 + t0 + g) % ring;
+                e[g] = t0 + g < Lv ? gc(S.e)[(size_t)slot * S.sstride + (size_t)d * N + c] : 0.0;
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (t0 + g < Lv) {
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) {
+                        const double2 w = wsh[(t0 + g) * NF + f];
+                        a[f].x += e[g] * w.x;
+                        a[f].y += e[g] * w.y;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            A[2 * (((size_t)d * NF + f) * S.astride + c)] = a[f].x;
+            A[2 * (((size_t)d * NF + f) * S.astride + c) + 1] = a[f].y;
+        }
+    }
+}
+
+// any frequency count: groups of 4 accumulators, weights from global memory
+__device__ __forceinline__ void flush_cell_any(const ShotDev &S, int c, int n0, int Lv)
+{
+    const int N = S.n, ring = S.ring, cur = S.cur, nf = S.nf;
+    const GLOBAL double *__restrict__ W = gc(S.w) + 2 * (size_t)n0 * nf;
+    GLOBAL double *__restrict__ A = gm(S.acc);
+    for (int d = 0; d < 3; ++d)
+        for (int f0 = 0; f0 < nf; f0 += 4) {
+            const int nq = nf - f0 < 4 ? nf - f0 : 4;
+            double2 a[4];
+            for (int q = 0; q < nq; ++q) {
+                a[q].x = A[2 * (((size_t)d * nf + f0 + q) * S.astride + c)];
+                a[q].y = A[2 * (((size_t)d * nf + f0 + q) * S.astride + c) + 1];
+            }
+            int slot = cur;
+            for (int t = 0; t < Lv; ++t) {
+                slot = slot + 1 == ring ? 0 : slot + 1;
+                const double e = gc(S.e)[(size_t)slot * S.sstride + (size_t)d * N + c];
+                for (int q = 0; q < nq; ++q) {
+                    const double wx = W[2 * ((size_t)t * nf + f0 + q)];
+                    const double wy = W[2 * ((size_t)t * nf + f0 + q) + 1];
+                    a[q].x += e * wx;
+                    a[q].y += e * wy;
+                }
+            }
+            for (int q = 0; q < nq; ++q) {
+                A[2 * (((size_t)d * nf + f0 + q) * S.astride + c)] = a[q].x;
+                A[2 * (((size_t)d * nf + f0 + q) * S.astride + c) + 1] = a[q].y;
+            }
+        }
+}
+
+__global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__restrict__ D, int ns,
+                                                               const int *__restrict__ pre, int L)
+{
+    __shared__ double2 wsh[RING_MAX * FLUSH_NF_MAX];
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    const int c = (blockIdx.x - pre[s]) * blockDim.x + threadIdx.x;
-    if (c >= S.n || S.nf == 0)
-        return;
+    const int nf = S.nf;
     const int n0 = *S.step;
     int Lv = L;
     if (n0 + Lv > S.it - 1)
         Lv = S.it - 1 - n0;
-    if (Lv <= 0)
+    if (nf == 0 || Lv <= 0)
+        return;  // uniform over the block
+    const int c = (blockIdx.x - pre[s]) * blockDim.x + threadIdx.x;
+    if (nf > FLUSH_NF_MAX || (nf != 1 && nf != 2 && nf != 3 && nf != 4 && nf != 8)) {
+        if (c < S.n)
+            flush_cell_any(S, c, n0, Lv);
         return;
-    const int N = S.n, nf = S.nf, ring = S.ring, cur = S.cur;
-    const double2 *__restrict__ W = reinterpret_cast<const double2 *>(S.w) + (size_t)n0 * nf;
-    double2 *__restrict__ A = reinterpret_cast<double2 *>(S.acc);
-    for (int d = 0; d < 3; ++d) {
-        double ev[RING_MAX];
-#pragma unroll
-        for (int t = 0; t < RING_MAX; ++t)
-            ev[t] = t < Lv ? S.e[((size_t)((cur + 1 + t) % ring) * 3 + d) * N + c] : 0.0;
-        for (int f = 0; f < nf; ++f) {
-            double2 a = A[((size_t)d * nf + f) * N + c];
-#pragma unroll
-            for (int t = 0; t < RING_MAX; ++t) {
-                if (t < Lv) {
-                    const double2 w = W[(size_t)t * nf + f];
-                    a.x += ev[t] * w.x;
-                    a.y += ev[t] * w.y;
-                }
-            }
-            A[((size_t)d * nf + f) * N + c] = a;
-        }
+    }
+    const GLOBAL double *__restrict__ W = gc(S.w) + 2 * (size_t)n0 * nf;
+    for (int q = threadIdx.x; q < Lv * nf; q += blockDim.x) {
+        wsh[q].x = W[2 * q];
+        wsh[q].y = W[2 * q + 1];
+    }
+    __syncthreads();
+    if (c >= S.n)
+        return;
+    switch (nf) {
+    case 1: flush_cell<1>(S, c, Lv, wsh); break;
+    case 2: flush_cell<2>(S, c, Lv, wsh); break;
+    case 3: flush_cell<3>(S, c, Lv, wsh); break;
+    case 4: flush_cell<4>(S, c, Lv, wsh); break;
+    default: flush_cell<8>(S, c, Lv, wsh); break;
     }
 }
 
@@ -427,12 +551,536 @@ __global__ void k_fill(double *p, size_t count, double v)
         p[q] = v;
 }
 
-// eps' = sigma/2/omega0 (satan_i/fwi3d_cpu_function_v1.4.c:335)
-__global__ void k_eps(const double *__restrict__ sig, double *__restrict__ eps, int n, double omega0)
+__global__ void k_diag_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n2;
+         q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = src[q];
+}
+
+// eps' = sigma/2/omega0 (satan_i/fwi3d_cpu_function_v1.4.c:335) and dt/eps', the
+// E coefficient cb wherever the CPML conductivity is zero (function.c:415-417
+// with esig = 0 reduces to dt/eps'/1.0).
+__global__ void k_eps(const double *__restrict__ sig, double *__restrict__ eps,
+                      double *__restrict__ cbi, int n, double omega0, double dt)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n)
-        eps[c] = sig[c] / 2.0 / omega0;
+    if (c < n) {
+        const double e = sig[c] / 2.0 / omega0;
+        eps[c] = e;
+        cbi[c] = dt / e;
+    }
+}
+
+
+// ------------------------------------------------------------------------
+// z-marching kernels of the fused step.  One thread per (i, j) column of a
+// 32 x 8 tile walks a z-chunk; the z-stencil operands stay in a 4-deep register
+// window, x/y neighbours come through L1/L2.  Blocks are numbered so that each
+// XCD (blockIdx % 8) owns a contiguous run of tiles: neighbouring tiles share
+// an L2.  Arithmetic is the reference's, operation for operation; where the
+// CPML conductivity of a component is zero its coefficients reduce exactly to
+// ca = 1, cb = dt/eps' (E) and da = 1, db = dt/mu0 (H), taken without divisions.
+
+struct ZCol {
+    int i, jj, k0, k1;
+};
+
+__device__ __forceinline__ bool zcol(const ShotDev &S, int b, ZCol &z)
+{
+    const int L = (b & 7) * (S.zb_pad >> 3) + (b >> 3);
+    if (L >= S.zb)
+        return false;
+    const int tx = L % S.ztx, r = L / S.ztx;
+    const int ty = r % S.zty, kz = r / S.zty;
+    z.i = tx * ZX + (threadIdx.x & (ZX - 1));
+    z.jj = ty * ZY + (threadIdx.x / ZX);
+    z.k0 = kz * S.kb;
+    z.k1 = min(z.k0 + S.kb, S.iz);
+    return z.i < S.ix && z.jj < S.iy;
+}
+
+__global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_efield_z(const ShotDev *__restrict__ D, int ns,
+                                                     const int *__restrict__ pre, int j, int sign)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    const int n = *S.step + j;
+    if (n >= S.it - 1)
+        return;
+    ZCol z;
+    if (!zcol(S, blockIdx.x - pre[s], z))
+        return;
+    const int i = z.i, jj = z.jj;
+    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
+    const size_t N = (size_t)S.n;
+    const int sy = ix, sz = ix * iy, cxy = jj * ix + i;
+    const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
+    const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
+    GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Hx = gc(S.h);
+    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
+    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * N);
+    const double c1 = S.c1, c2 = S.c2, dt = S.dt;
+    const GLOBAL double *__restrict__ px = gc(S.prof[0]);
+    const GLOBAL double *__restrict__ py = gc(S.prof[1]);
+    const GLOBAL double *__restrict__ pz = gc(S.prof[2]);
+
+    const bool rx1 = i >= 1 && i < ix - 1, rx2 = i >= 2 && i < ix - 1;
+    const bool ry1 = jj >= 1 && jj < iy - 1, ry2 = jj >= 2 && jj < iy - 1;
+    const bool fxc = ((i >= 2 && i < nc - 1) || (i >= ix - nc + 1 && i < ix - 2)) && jj < iy - 1;
+    const bool fyc = ((jj >= 2 && jj < nc) || (jj >= iy - nc + 1 && jj < iy - 2)) && i < ix - 1;
+    const double esx = px[PR_ESIG * ix + i], esy = py[PR_ESIG * iy + jj];
+    const double kedx = px[PR_KED * ix + i], kedy = py[PR_KED * iy + jj];
+
+    // z window of Hx and Hy at k-2 .. k+1
+    double xm2 = 0.0, xm1 = 0.0, x0, ym2 = 0.0, ym1 = 0.0, y0;
+    {
+        const int c = z.k0 * sz + cxy;
+        if (z.k0 >= 2) {
+            xm2 = Hx[c - 2 * sz];
+            ym2 = Hy[c - 2 * sz];
+        }
+        if (z.k0 >= 1) {
+            xm1 = Hx[c - sz];
+            ym1 = Hy[c - sz];
+        }
+        x0 = Hx[c];
+        y0 = Hy[c];
+    }
+    for (int k = z.k0; k < z.k1; ++k) {
+        const int c = k * sz + cxy;
+        double xp1 = 0.0, yp1 = 0.0;
+        if (k + 1 < iz) {
+            xp1 = Hx[c + sz];
+            yp1 = Hy[c + sz];
+        }
+        const bool rz1 = k >= 1 && k < iz - 1, rz2 = k >= 2 && k < iz - 1;
+        const bool upx = rx1 && ry2 && rz2, upy = rx2 && ry1 && rz2, upz = rx2 && ry2 && rz1;
+        const bool fx = fxc && k < iz - 1, fy = fyc && k < iz - 1;
+        const bool fz = ((k >= 2 && k < nc) || (k >= iz - nc + 1 && k < iz - 2)) && i < ix - 1 &&
+                        jj < iy - 1;
+        double ex = Ein[c], ey = Ein[N + c], ez = Ein[2 * N + c];
+
+        double hzy = 0.0, hyz = 0.0, hxz = 0.0, hzx = 0.0, hyx = 0.0, hxy = 0.0;
+        if (upx || fy) hzy = dminus(Hz, c, sy, c1, c2);
+        if (upx || fz) hyz = c1 * y0 - c1 * ym1 + c2 * yp1 - c2 * ym2;
+        if (upy || fz) hxz = c1 * x0 - c1 * xm1 + c2 * xp1 - c2 * xm2;
+        if (upy || fx) hzx = dminus(Hz, c, 1, c1, c2);
+        if (upz || fx) hyx = c1 * y0 - c1 * Hy[c - 1] + c2 * Hy[c + 1] - c2 * Hy[c - 2];
+        if (upz || fy) hxy = c1 * x0 - c1 * Hx[c - sy] + c2 * Hx[c + sy] - c2 * Hx[c - 2 * sy];
+
+        const double esz = pz[PR_ESIG * iz + k];
+        const double cbi = gc(S.cbi)[c];
+        double cbx = cbi, cby = cbi, cbz = cbi, sex = 0.0, sey = 0.0, sez = 0.0;
+        if (esx != 0.0 || esy != 0.0 || esz != 0.0) {
+            const double eps2 = gc(S.eps)[c];
+            sex = esx * dt / (2.0 * eps2);
+            sey = esy * dt / (2.0 * eps2);
+            sez = esz * dt / (2.0 * eps2);
+            cbx = dt / eps2 / (1.0 + sex);
+            cby = dt / eps2 / (1.0 + sey);
+            cbz = dt / eps2 / (1.0 + sez);
+        }
+        const double kedz = pz[PR_KED * iz + k];
+        if (upx) {
+            const double ca = esx != 0.0 ? (1.0 - sex) / (1.0 + sex) : 1.0;
+            const double cu = hzy / kedy - hyz / kedz;
+            ex = sign > 0 ? ca * ex + cbx * cu : ca * ex - cbx * cu;
+        }
+        if (upy) {
+            const double ca = esy != 0.0 ? (1.0 - sey) / (1.0 + sey) : 1.0;
+            const double cu = hxz / kedz - hzx / kedx;
+            ey = sign > 0 ? ca * ey + cby * cu : ca * ey - cby * cu;
+        }
+        if (upz) {
+            const double ca = esz != 0.0 ? (1.0 - sez) / (1.0 + sez) : 1.0;
+            const double cu = hyx / kedx - hxy / kedy;
+            ez = sign > 0 ? ca * ez + cbz * cu : ca * ez - cbz * cu;
+        }
+        if (fx) {
+            GLOBAL double *__restrict__ P = gm(S.psi[0]);
+            const size_t L = (size_t)S.slab[0];
+            const size_t q = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
+            const double be = px[PR_BE * ix + i], ce = px[PR_CE * ix + i], h = S.dl[0];
+            const double p0 = be * P[q] + ce * hyx / h;
+            const double p1 = be * P[L + q] + ce * hzx / h;
+            P[q] = p0;
+            P[L + q] = p1;
+            ez += cbz * p0;
+            ey -= cby * p1;
+        }
+        if (fy) {
+            GLOBAL double *__restrict__ P = gm(S.psi[1]);
+            const size_t L = (size_t)S.slab[1];
+            const size_t q = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
+            const double be = py[PR_BE * iy + jj], ce = py[PR_CE * iy + jj], h = S.dl[1];
+            const double p0 = be * P[q] + ce * hzy / h;
+            const double p1 = be * P[L + q] + ce * hxy / h;
+            P[q] = p0;
+            P[L + q] = p1;
+            ex += cbx * p0;
+            ez -= cbz * p1;
+        }
+        if (fz) {
+            GLOBAL double *__restrict__ P = gm(S.psi[2]);
+            const size_t L = (size_t)S.slab[2];
+            const size_t q = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
+            const double be = pz[PR_BE * iz + k], ce = pz[PR_CE * iz + k], h = S.dl[2];
+            const double p0 = be * P[q] + ce * hxz / h;
+            const double p1 = be * P[L + q] + ce * hyz / h;
+            P[q] = p0;
+            P[L + q] = p1;
+            ey += cby * p0;
+            ex -= cbx * p1;
+        }
+        if (c == S.src_cell) {
+            const double a = gc(S.amp)[n];
+            if (S.src_comp == 0)
+                ex -= a;
+            else if (S.src_comp == 1)
+                ey -= a;
+            else
+                ez -= a;
+        }
+        Eout[c] = ex;
+        Eout[N + c] = ey;
+        Eout[2 * N + c] = ez;
+        xm2 = xm1;
+        xm1 = x0;
+        x0 = xp1;
+        ym2 = ym1;
+        ym1 = y0;
+        y0 = yp1;
+    }
+}
+
+__global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_hfield_z(const ShotDev *__restrict__ D, int ns,
+                                                     const int *__restrict__ pre, int j, int sign)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    if (*S.step + j >= S.it - 1)
+        return;
+    ZCol z;
+    if (!zcol(S, blockIdx.x - pre[s], z))
+        return;
+    const int i = z.i, jj = z.jj;
+    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
+    const size_t N = (size_t)S.n;
+    const int sy = ix, sz = ix * iy, cxy = jj * ix + i;
+    const int so = (S.cur + j + 1) % S.ring;
+    const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Ey = gc(Ex + N);
+    const GLOBAL double *__restrict__ Ez = gc(Ex + 2 * N);
+    GLOBAL double *__restrict__ Hx = gm(S.h);
+    GLOBAL double *__restrict__ Hy = gm(S.h + N);
+    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * N);
+    const double c1 = S.c1, c2 = S.c2, dt = S.dt;
+    const double dbi = dt / S.mu0 / 1.0;   // db where the magnetic CPML sigma is zero
+    const GLOBAL double *__restrict__ px = gc(S.prof[0]);
+    const GLOBAL double *__restrict__ py = gc(S.prof[1]);
+    const GLOBAL double *__restrict__ pz = gc(S.prof[2]);
+
+    const bool ax1 = i >= 1 && i < ix - 1, ax2 = i >= 1 && i < ix - 2;
+    const bool ay1 = jj >= 1 && jj < iy - 1, ay2 = jj >= 1 && jj < iy - 2;
+    const bool gxc = ((i >= 1 && i < nc - 1) || (i >= ix - nc && i < ix - 2)) && jj < iy - 1;
+    const bool gyc = ((jj >= 1 && jj < nc - 1) || (jj >= iy - nc && jj < iy - 2)) && i < ix - 1;
+    const double msx = px[PR_MSIG * ix + i], msy = py[PR_MSIG * iy + jj];
+    const double khdx = px[PR_KHD * ix + i], khdy = py[PR_KHD * iy + jj];
+
+    // z window of Ex and Ey at k-1 .. k+2
+    double xm1 = 0.0, x0, xp1 = 0.0, ym1 = 0.0, y0, yp1 = 0.0;
+    {
+        const int c = z.k0 * sz + cxy;
+        if (z.k0 >= 1) {
+            xm1 = Ex[c - sz];
+            ym1 = Ey[c - sz];
+        }
+        x0 = Ex[c];
+        y0 = Ey[c];
+        if (z.k0 + 1 < iz) {
+            xp1 = Ex[c + sz];
+            yp1 = Ey[c + sz];
+        }
+    }
+    for (int k = z.k0; k < z.k1; ++k) {
+        const int c = k * sz + cxy;
+        double xp2 = 0.0, yp2 = 0.0;
+        if (k + 2 < iz) {
+            xp2 = Ex[c + 2 * sz];
+            yp2 = Ey[c + 2 * sz];
+        }
+        const bool az1 = k >= 1 && k < iz - 1, az2 = k >= 1 && k < iz - 2;
+        const bool upx = ax1 && ay2 && az2, upy = ax2 && ay1 && az2, upz = ax2 && ay2 && az1;
+        const bool gx = gxc && k < iz - 1, gy = gyc && k < iz - 1;
+        const bool gz = ((k >= 1 && k < nc - 1) || (k >= iz - nc && k < iz - 2)) && i < ix - 1 &&
+                        jj < iy - 1;
+        if (upx || upy || upz || gx || gy || gz) {
+            double ezy = 0.0, eyz = 0.0, exz = 0.0, ezx = 0.0, eyx = 0.0, exy = 0.0;
+            if (upx || gy) ezy = dplus(Ez, c, sy, c1, c2);
+            if (upx || gz) eyz = c1 * yp1 - c1 * y0 + c2 * yp2 - c2 * ym1;
+            if (upy || gz) exz = c1 * xp1 - c1 * x0 + c2 * xp2 - c2 * xm1;
+            if (upy || gx) ezx = dplus(Ez, c, 1, c1, c2);
+            if (upz || gx) eyx = c1 * Ey[c + 1] - c1 * y0 + c2 * Ey[c + 2] - c2 * Ey[c - 1];
+            if (upz || gy) exy = c1 * Ex[c + sy] - c1 * x0 + c2 * Ex[c + 2 * sy] - c2 * Ex[c - sy];
+
+            const double msz = pz[PR_MSIG * iz + k];
+            double smx = 0.0, smy = 0.0, smz = 0.0, dbx = dbi, dby = dbi, dbz = dbi;
+            if (msx != 0.0 || msy != 0.0 || msz != 0.0) {
+                const double eps2 = gc(S.eps)[c];
+                smx = msx * dt / (2.0 * eps2);
+                smy = msy * dt / (2.0 * eps2);
+                smz = msz * dt / (2.0 * eps2);
+                dbx = dt / S.mu0 / (1.0 + smx);
+                dby = dt / S.mu0 / (1.0 + smy);
+                dbz = dt / S.mu0 / (1.0 + smz);
+            }
+            const double khdz = pz[PR_KHD * iz + k];
+            double hx = Hx[c], hy = Hy[c], hz = Hz[c];
+            if (upx) {
+                const double da = msx != 0.0 ? (1.0 - smx) / (1.0 + smx) : 1.0;
+                const double cu = ezy / khdy - eyz / khdz;
+                hx = sign > 0 ? da * hx - dbx * cu : da * hx + dbx * cu;
+            }
+            if (upy) {
+                const double da = msy != 0.0 ? (1.0 - smy) / (1.0 + smy) : 1.0;
+                const double cu = exz / khdz - ezx / khdx;
+                hy = sign > 0 ? da * hy - dby * cu : da * hy + dby * cu;
+            }
+            if (upz) {
+                const double da = msz != 0.0 ? (1.0 - smz) / (1.0 + smz) : 1.0;
+                const double cu = eyx / khdx - exy / khdy;
+                hz = sign > 0 ? da * hz - dbz * cu : da * hz + dbz * cu;
+            }
+            if (gx) {
+                GLOBAL double *__restrict__ P = gm(S.psi[0]);
+                const size_t L = (size_t)S.slab[0];
+                const size_t q = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
+                const double bh = px[PR_BH * ix + i], ch = px[PR_CH * ix + i], h = S.dl[0];
+                const double p0 = bh * P[2 * L + q] + ch * ezx / h;
+                const double p1 = bh * P[3 * L + q] + ch * eyx / h;
+                P[2 * L + q] = p0;
+                P[3 * L + q] = p1;
+                hy += dby * p0;
+                hz -= dbz * p1;
+            }
+            if (gy) {
+                GLOBAL double *__restrict__ P = gm(S.psi[1]);
+                const size_t L = (size_t)S.slab[1];
+                const size_t q = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
+                const double bh = py[PR_BH * iy + jj], ch = py[PR_CH * iy + jj], h = S.dl[1];
+                const double p0 = bh * P[2 * L + q] + ch * exy / h;
+                const double p1 = bh * P[3 * L + q] + ch * ezy / h;
+                P[2 * L + q] = p0;
+                P[3 * L + q] = p1;
+                hz += dbz * p0;
+                hx -= dbx * p1;
+            }
+            if (gz) {
+                GLOBAL double *__restrict__ P = gm(S.psi[2]);
+                const size_t L = (size_t)S.slab[2];
+                const size_t q = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
+                const double bh = pz[PR_BH * iz + k], ch = pz[PR_CH * iz + k], h = S.dl[2];
+                const double p0 = bh * P[2 * L + q] + ch * eyz / h;
+                const double p1 = bh * P[3 * L + q] + ch * exz / h;
+                P[2 * L + q] = p0;
+                P[3 * L + q] = p1;
+                hx += dbx * p0;
+                hy -= dby * p1;
+            }
+            Hx[c] = hx;
+            Hy[c] = hy;
+            Hz[c] = hz;
+        }
+        xm1 = x0;
+        x0 = xp1;
+        xp1 = xp2;
+        ym1 = y0;
+        y0 = yp1;
+        yp1 = yp2;
+    }
+}
+
+hipError_t launch_efield_z(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_efield_z, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_hfield_z, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
+}
+
+
+// ------------------------------------------------------------------------
+// Per-cell kernels of the fused step (32 x 8 tiles, one plane per block,
+// blocks numbered so each XCD owns a contiguous run of tiles).  Waves whose
+// cells all lie outside every CPML shell take a branch-free path with all 25
+// loads issued up front; there the reference's coefficients reduce exactly to
+// ca = 1, cb = dt/eps' (E) and da = 1, db = dt/mu0 (H), so no division by the
+// coefficient denominators is needed and the result is bit-identical.  Any
+// other wave runs the general per-cell code.
+
+__device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, int &k)
+{
+    const int L = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
+    if (L >= S.fb)
+        return false;
+    const int tx = L % S.ztx, r = L / S.ztx;
+#if MT_TILE_ORDER == 1
+    // x fastest, then z, then y: an XCD sweeps an 8-row band through every
+    // plane, so the z-neighbour planes it re-reads are still in its L2
+    k = r % S.iz;
+    const int ty = r / S.iz;
+#else
+    const int ty = r % S.zty;
+    k = r / S.zty;
+#endif
+    i = tx * ZX + (threadIdx.x & (ZX - 1));
+    jj = ty * ZY + threadIdx.x / ZX;
+    return i < S.ix && jj < S.iy;
+}
+
+__global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__ D, int ns,
+                                                     const int *__restrict__ pre, int j, int sign)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    const int n = *S.step + j;
+    if (n >= S.it - 1)
+        return;
+    int i, jj, k;
+    if (!fcell(S, blockIdx.x - pre[s], i, jj, k))
+        return;
+    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
+    const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
+    const bool inner = i >= nc && i < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
+    if (!__all(inner)) {
+        efield_cell<M_UPD | M_PSI | M_SRC>(S, i, jj, k, n, si, so, sign);
+        return;
+    }
+    const size_t N = (size_t)S.n;
+    const int sy = ix, sz = ix * iy;
+    const int c = (k * iy + jj) * ix + i;
+    const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
+    GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Hx = gc(S.h);
+    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
+    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * N);
+    const double ex0 = Ein[c], ey0 = Ein[N + c], ez0 = Ein[2 * N + c], cb = gc(S.cbi)[c];
+    const double z0 = Hz[c], zym1 = Hz[c - sy], zyp1 = Hz[c + sy], zym2 = Hz[c - 2 * sy];
+    const double zxm1 = Hz[c - 1], zxp1 = Hz[c + 1], zxm2 = Hz[c - 2];
+    const double y0 = Hy[c], yzm1 = Hy[c - sz], yzp1 = Hy[c + sz], yzm2 = Hy[c - 2 * sz];
+    const double yxm1 = Hy[c - 1], yxp1 = Hy[c + 1], yxm2 = Hy[c - 2];
+    const double x0 = Hx[c], xzm1 = Hx[c - sz], xzp1 = Hx[c + sz], xzm2 = Hx[c - 2 * sz];
+    const double xym1 = Hx[c - sy], xyp1 = Hx[c + sy], xym2 = Hx[c - 2 * sy];
+    const double kedx = gc(S.prof[0])[PR_KED * ix + i];
+    const double kedy = gc(S.prof[1])[PR_KED * iy + jj];
+    const double kedz = gc(S.prof[2])[PR_KED * iz + k];
+    const double c1 = S.c1, c2 = S.c2;
+    const double hzy = c1 * z0 - c1 * zym1 + c2 * zyp1 - c2 * zym2;
+    const double hyz = c1 * y0 - c1 * yzm1 + c2 * yzp1 - c2 * yzm2;
+    const double hxz = c1 * x0 - c1 * xzm1 + c2 * xzp1 - c2 * xzm2;
+    const double hzx = c1 * z0 - c1 * zxm1 + c2 * zxp1 - c2 * zxm2;
+    const double hyx = c1 * y0 - c1 * yxm1 + c2 * yxp1 - c2 * yxm2;
+    const double hxy = c1 * x0 - c1 * xym1 + c2 * xyp1 - c2 * xym2;
+    const double cux = hzy / kedy - hyz / kedz;
+    const double cuy = hxz / kedz - hzx / kedx;
+    const double cuz = hyx / kedx - hxy / kedy;
+    double ex = sign > 0 ? ex0 + cb * cux : ex0 - cb * cux;
+    double ey = sign > 0 ? ey0 + cb * cuy : ey0 - cb * cuy;
+    double ez = sign > 0 ? ez0 + cb * cuz : ez0 - cb * cuz;
+    if (c == S.src_cell) {
+        const double a = gc(S.amp)[n];
+        if (S.src_comp == 0)
+            ex -= a;
+        else if (S.src_comp == 1)
+            ey -= a;
+        else
+            ez -= a;
+    }
+    Eout[c] = ex;
+    Eout[N + c] = ey;
+    Eout[2 * N + c] = ez;
+}
+
+__global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__ D, int ns,
+                                                     const int *__restrict__ pre, int j, int sign)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    if (*S.step + j >= S.it - 1)
+        return;
+    int i, jj, k;
+    if (!fcell(S, blockIdx.x - pre[s], i, jj, k))
+        return;
+    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
+    const int so = (S.cur + j + 1) % S.ring;
+    const bool inner = i >= nc && i < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
+    if (!__all(inner)) {
+        hfield_cell<M_UPD | M_PSI>(S, i, jj, k, so, sign);
+        return;
+    }
+    const size_t N = (size_t)S.n;
+    const int sy = ix, sz = ix * iy;
+    const int c = (k * iy + jj) * ix + i;
+    const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Ey = gc(Ex + N);
+    const GLOBAL double *__restrict__ Ez = gc(Ex + 2 * N);
+    GLOBAL double *__restrict__ Hx = gm(S.h);
+    GLOBAL double *__restrict__ Hy = gm(S.h + N);
+    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * N);
+    const double hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
+    const double z0 = Ez[c], zyp1 = Ez[c + sy], zyp2 = Ez[c + 2 * sy], zym1 = Ez[c - sy];
+    const double zxp1 = Ez[c + 1], zxp2 = Ez[c + 2], zxm1 = Ez[c - 1];
+    const double y0 = Ey[c], yzp1 = Ey[c + sz], yzp2 = Ey[c + 2 * sz], yzm1 = Ey[c - sz];
+    const double yxp1 = Ey[c + 1], yxp2 = Ey[c + 2], yxm1 = Ey[c - 1];
+    const double x0 = Ex[c], xzp1 = Ex[c + sz], xzp2 = Ex[c + 2 * sz], xzm1 = Ex[c - sz];
+    const double xyp1 = Ex[c + sy], xyp2 = Ex[c + 2 * sy], xym1 = Ex[c - sy];
+    const double khdx = gc(S.prof[0])[PR_KHD * ix + i];
+    const double khdy = gc(S.prof[1])[PR_KHD * iy + jj];
+    const double khdz = gc(S.prof[2])[PR_KHD * iz + k];
+    const double c1 = S.c1, c2 = S.c2;
+    const double db = S.dt / S.mu0 / 1.0;
+    const double ezy = c1 * zyp1 - c1 * z0 + c2 * zyp2 - c2 * zym1;
+    const double eyz = c1 * yzp1 - c1 * y0 + c2 * yzp2 - c2 * yzm1;
+    const double exz = c1 * xzp1 - c1 * x0 + c2 * xzp2 - c2 * xzm1;
+    const double ezx = c1 * zxp1 - c1 * z0 + c2 * zxp2 - c2 * zxm1;
+    const double eyx = c1 * yxp1 - c1 * y0 + c2 * yxp2 - c2 * yxm1;
+    const double exy = c1 * xyp1 - c1 * x0 + c2 * xyp2 - c2 * xym1;
+    const double cux = ezy / khdy - eyz / khdz;
+    const double cuy = exz / khdz - ezx / khdx;
+    const double cuz = eyx / khdx - exy / khdy;
+    Hx[c] = sign > 0 ? hx0 - db * cux : hx0 + db * cux;
+    Hy[c] = sign > 0 ? hy0 - db * cuy : hy0 + db * cuy;
+    Hz[c] = sign > 0 ? hz0 - db * cuz : hz0 + db * cuz;
+}
+
+hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_efield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_hfield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
 }
 
 #define LAUNCH_E(M)                                                                          \
@@ -524,9 +1172,21 @@ hipError_t launch_fill(double *p, size_t count, double v, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_eps(const double *sig, double *eps, int n, double omega0, hipStream_t st)
+hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_eps, dim3((n + 255) / 256), dim3(256), 0, st, sig, eps, n, omega0);
+    const size_t n2 = count / 2;
+    size_t blocks = (n2 + 255) / 256;
+    if (blocks > 262144)
+        blocks = 262144;
+    hipLaunchKernelGGL(k_diag_copy, dim3((unsigned)blocks), dim3(256), 0, st,
+                       reinterpret_cast<const double2 *>(src), reinterpret_cast<double2 *>(dst), n2);
+    return hipGetLastError();
+}
+
+hipError_t launch_eps(const double *sig, double *eps, double *cbi, int n, double omega0, double dt,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL(k_eps, dim3((n + 255) / 256), dim3(256), 0, st, sig, eps, cbi, n, omega0, dt);
     return hipGetLastError();
 }
 

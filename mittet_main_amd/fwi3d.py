@@ -135,13 +135,13 @@ class Model:
 
     def shot(self, it: int, signal: Optional[np.ndarray] = None, src=(0, 0, 0), src_comp: int = 0,
              receivers: Optional[Sequence] = None, omega_vol: Optional[Sequence[float]] = None,
-             ring: int = 0) -> "Shot":
-        s = Shot(self, it, signal, src, src_comp, receivers, omega_vol, ring)
+             ring: int = 0, zchunk: int = 0) -> "Shot":
+        s = Shot(self, it, signal, src, src_comp, receivers, omega_vol, ring, zchunk)
         self._shots.append(s)
         return s
 
 
-def shot_params(it, signal, src, src_comp, receivers, omega_vol, ring, keep):
+def shot_params(it, signal, src, src_comp, receivers, omega_vol, ring, keep, zchunk=0):
     sig = None if signal is None else np.ascontiguousarray(signal, dtype=np.float64)
     if sig is not None and sig.size < it:
         raise ValueError(f"signal has {sig.size} samples, it={it}")
@@ -153,7 +153,7 @@ def shot_params(it, signal, src, src_comp, receivers, omega_vol, ring, keep):
     return ShotParams(int(it), int(src_comp), int(src[0]), int(src[1]), int(src[2]),
                       None if sig is None else sig.ctypes.data_as(C.POINTER(C.c_double)),
                       rec.shape[0], rec.ctypes.data_as(C.POINTER(C.c_int)),
-                      om.size, om.ctypes.data_as(C.POINTER(C.c_double)), int(ring))
+                      om.size, om.ctypes.data_as(C.POINTER(C.c_double)), int(ring), int(zchunk))
 
 
 def shot_bytes(grid: Grid, it: int, nrec: int = 0, nf_vol: int = 0, ring: int = 0) -> int:
@@ -166,11 +166,12 @@ def shot_bytes(grid: Grid, it: int, nrec: int = 0, nf_vol: int = 0, ring: int = 
 class Shot:
     """Fields, psi shells, receivers and DFT accumulators of one source."""
 
-    def __init__(self, model: Model, it, signal, src, src_comp, receivers, omega_vol, ring):
+    def __init__(self, model: Model, it, signal, src, src_comp, receivers, omega_vol, ring,
+                 zchunk=0):
         self.model = model
         self.it = int(it)
         keep = []
-        sp = shot_params(it, signal, src, src_comp, receivers, omega_vol, ring, keep)
+        sp = shot_params(it, signal, src, src_comp, receivers, omega_vol, ring, keep, zchunk)
         self.nrec = sp.nrec
         self.nf = sp.nf_vol
         self.omega_vol = np.array(keep[2])
