@@ -85,8 +85,82 @@ def cpu_baseline(wl, steps):
                       f"{dt:.2f} s"}
 
 
+def serve_mode(args):
+    """Served forward modelling: Poisson stream (mean 20 req/s), grids
+    {96,128,192,256}^3 + 12-cell CPML, 1-16 shots, Nt = 30 x n, receiver spectra
+    at 4 frequencies, 200 GB admission budget; vs the sequential plain path."""
+    from mittet_main_amd import serving
+    reqs = serving.request_stream(args.seed, args.requests, max_shots=args.max_shots)
+    log(f"[serve] {len(reqs)} requests, {sum(r.nshots for r in reqs)} shots, sizes "
+        f"{[r.n_int for r in reqs]}")
+    srv = serving.Server(budget_bytes=200e9, chunk=32)
+    served = srv.run(reqs, realtime=True)
+    srv.close()
+    log(f"[serve] served: {served['shots_per_s']:.3f} shots/s, p50 {served['p50_s']:.2f} s")
+    seq = serving.run_sequential(reqs, mode="fused", realtime=True)
+    log(f"[serve] sequential fused: {seq['shots_per_s']:.3f} shots/s")
+    out = {"mode": "serve", "requests": len(reqs), "shots": served["shots"],
+           "served": {k: v for k, v in served.items() if k != "results"},
+           "sequential_fused": {k: v for k, v in seq.items() if k != "results"},
+           "speedup_vs_sequential_fused": served["shots_per_s"] / seq["shots_per_s"]}
+    if args.ops_requests:
+        sub = reqs[:args.ops_requests]
+        ops = serving.run_sequential(sub, mode="ops", realtime=False)
+        fus = serving.run_sequential(sub, mode="fused", realtime=False)
+        out["per_operator_launches_subset"] = {
+            "requests": len(sub), "ops_shots_per_s": ops["shots_per_s"],
+            "fused_shots_per_s": fus["shots_per_s"]}
+    same = all(np.array_equal(a, b) for rid, r in seq["results"].items()
+               for a, b in zip(r.spectra, served["results"][rid].spectra))
+    out["served_equals_alone_bitwise"] = bool(same)
+    print(json.dumps(out), flush=True)
+
+
+def gradient_mode(args):
+    """Adjoint gradient: 128^3 + 12-cell CPML, 8 x-dipole shots, 64 receivers,
+    L2 misfit on complex receiver E at 4 frequencies, Nt = 30 x 128."""
+    from mittet_main_amd import configs
+    from mittet_main_amd.adjoint import GradientProblem, ShotSpec
+    n_int, nc, it = 128, 12, args.grad_nt
+    sig_true, k_top = configs.layered_box_sigma(n_int, nc, args.seed)
+    n = n_int + 2 * nc
+    sig0 = np.full_like(sig_true, float(np.exp(np.log(sig_true).mean())))
+    dt, cmax = configs.dt_for(50.0, 50.0, 50.0)
+    from mittet_main_amd import Grid
+    g = Grid(n, n, n, nc, 50.0, 50.0, 50.0, dt, configs.OMEGA0, cmax)
+    sigw = configs.firstderiv_gauss(it, dt, configs.FMAX_W)
+    rng = np.random.default_rng(args.seed)
+    shots = [ShotSpec((int(x), int(y), k_top + 2), sigw)
+             for x, y in rng.integers(nc + 8, n - nc - 8, size=(8, 2))]
+    gg = np.linspace(nc + 4, n - nc - 5, 8).round().astype(int)
+    X, Y = np.meshgrid(gg, gg, indexing="xy")
+    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, k_top)], axis=1)
+    om = 2 * np.pi * np.geomspace(0.25, 2.0, 4)
+    P = GradientProblem(g, shots, rec, it, om, segment=args.segment)
+    t0 = time.perf_counter()
+    P.d_obs = P.forward_data(sig_true)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, grad = P.value_and_grad(np.log(sig0))
+    t_grad = time.perf_counter() - t0
+    out = {"mode": "gradient", "workload": "adjoint_gradient_128cube_fp64", "cells": g.ncells,
+           "nt": it, "shots": len(shots), "nrec": len(rec), "nf": len(om),
+           "forward_all_shots_s": t_fwd, "gradient_s": t_grad, "gradients_per_s": 1.0 / t_grad,
+           "misfit": J, "grad_norm": float(np.linalg.norm(grad)),
+           "checkpoint_storage_gb": P.storage_bytes() / 1e9, "segment": args.segment}
+    P.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", default="forward", choices=["forward", "serve", "gradient"])
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--requests", type=int, default=6)
+    ap.add_argument("--max-shots", type=int, default=16)
+    ap.add_argument("--ops-requests", type=int, default=0)
+    ap.add_argument("--grad-nt", type=int, default=3840)
+    ap.add_argument("--segment", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=64)
@@ -112,6 +186,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shots shard across GPUs by construction; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "serve":
+        return serve_mode(args)
+    if args.mode == "gradient":
+        return gradient_mode(args)
 
     from mittet_main_amd import Model, configs
 

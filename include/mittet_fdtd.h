@@ -180,6 +180,25 @@ int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal
 int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue,
                           double *e_w, double *j_w, double *g_w, double *obs);
 
+/* ---- differentiation ------------------------------------------------------- */
+
+/* Exact discrete adjoint of the forward step for a receiver misfit.  No
+ * counterpart in the reference, whose gradient (fwi3d_cpu_inv_fwi_v1.4.c:36-85,
+ * function.c:2246-2384) is a reciprocity correlation and not the derivative of
+ * its discrete scheme.  The shot must have no volume DFT (nf_vol = 0).
+ *   fwi3d_grad_create: checkpoint every `segment` steps (0: library default).
+ *   fwi3d_grad_forward: zeroes the shot and runs its whole time loop, storing
+ *       checkpoints; traces are then readable with fwi3d_get_traces.
+ *   fwi3d_grad_backward: adjoint sweep driven by the receiver-trace sensitivities
+ *       g = dJ/dE_trace, layout [3][nrec][it] (sample n = after step n); replays
+ *       each segment forward from its checkpoint.  out: dJ/dsigma per cell. */
+typedef struct fwi3d_grad fwi3d_grad;
+int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out);
+int fwi3d_grad_destroy(fwi3d_grad *g);
+long long fwi3d_grad_bytes(fwi3d_grad *g);
+int fwi3d_grad_forward(fwi3d_grad *g);
+int fwi3d_grad_backward(fwi3d_grad *g, const double *dj_dtrace, double *dj_dsigma);
+
 /* ---- multi-shot serving (persistent multi-shot launches) ------------------ */
 
 /* A batch advances all member shots (any models/grid sizes) with one launch per

@@ -76,6 +76,11 @@ _SIGS = {
                                         C.c_void_p, C.c_void_p]),
     "fwi3d_laplace_to_freq": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p]),
+    "fwi3d_grad_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "fwi3d_grad_destroy": (C.c_int, [C.c_void_p]),
+    "fwi3d_grad_bytes": (C.c_longlong, [C.c_void_p]),
+    "fwi3d_grad_forward": (C.c_int, [C.c_void_p]),
+    "fwi3d_grad_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fwi3d_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "fwi3d_batch_destroy": (C.c_int, [C.c_void_p]),
     "fwi3d_batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),

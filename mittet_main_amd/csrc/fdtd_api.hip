@@ -179,7 +179,7 @@ struct fwi3d_shot {
     ShotDev desc{};
     ShotDev *d_desc = nullptr;
     bool desc_dirty = true;
-    bool zmarch = false;   // fused step uses the z-marching kernels (zchunk > 0)
+    int kind = 0;          // fused kernel family (see launch_eh)
     int nblocks = 0;       // fused blocks (padded to a multiple of 8)
     int nblocks_op = 0;    // per-cell blocks of the stand-alone operators
     int ncellblocks = 0;
@@ -197,7 +197,7 @@ struct fwi3d_batch {
     ShotDev *d_desc = nullptr;
     int *d_pre = nullptr, *d_pre_cells = nullptr;
     int nblocks = 0, ncellblocks = 0;
-    bool zmarch = false;  // kernel family of the members (all equal)
+    int kind = 0;         // fused kernel family of the members (all equal)
     bool dirty = true;
     hipGraphExec_t graph = nullptr;
 };
@@ -246,17 +246,32 @@ int check_shot_ready(fwi3d_shot *s)
 }
 
 // launches for local steps j in [0, L) followed by the DFT flush and step advance
-int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const int *pre_cells,
-                 int ncellblocks, int L, bool any_dft, bool any_rec, bool zmarch, hipStream_t st)
+// fused kernel family: 0 per-cell, 1 per-cell x-pairs, 2 z-marching
+int launch_eh(int kind, const ShotDev *d, int ns, const int *pre, int nblocks, int j, bool h,
+              hipStream_t st)
 {
+    hipError_t e;
+    if (kind == 2)
+        e = h ? mt::launch_hfield_z(d, ns, pre, nblocks, j, 1, st)
+              : mt::launch_efield_z(d, ns, pre, nblocks, j, 1, st);
+    else if (kind == 1)
+        e = h ? mt::launch_hfield_p(d, ns, pre, nblocks, j, 1, st)
+              : mt::launch_efield_p(d, ns, pre, nblocks, j, 1, st);
+    else
+        e = h ? mt::launch_hfield_f(d, ns, pre, nblocks, j, 1, st)
+              : mt::launch_efield_f(d, ns, pre, nblocks, j, 1, st);
+    HIP_TRY(e);
+    return FWI3D_OK;
+}
+
+int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const int *pre_cells,
+                 int ncellblocks, int L, bool any_dft, bool any_rec, int kind, hipStream_t st)
+{
+    int rc;
     for (int j = 0; j < L; ++j) {
-        if (zmarch) {
-            HIP_TRY(mt::launch_efield_z(d, ns, pre, nblocks, j, 1, st));
-            HIP_TRY(mt::launch_hfield_z(d, ns, pre, nblocks, j, 1, st));
-        } else {
-            HIP_TRY(mt::launch_efield_f(d, ns, pre, nblocks, j, 1, st));
-            HIP_TRY(mt::launch_hfield_f(d, ns, pre, nblocks, j, 1, st));
-        }
+        if ((rc = launch_eh(kind, d, ns, pre, nblocks, j, false, st)) ||
+            (rc = launch_eh(kind, d, ns, pre, nblocks, j, true, st)))
+            return rc;
         if (any_rec)
             HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
     }
@@ -268,11 +283,11 @@ int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const in
 
 int capture_chunk(hipGraphExec_t *out, const ShotDev *d, int ns, const int *pre, int nblocks,
                   const int *pre_cells, int ncellblocks, int L, bool any_dft, bool any_rec,
-                  bool zmarch, hipStream_t st)
+                  int kind, hipStream_t st)
 {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = launch_chunk(d, ns, pre, nblocks, pre_cells, ncellblocks, L, any_dft, any_rec, zmarch,
+    int rc = launch_chunk(d, ns, pre, nblocks, pre_cells, ncellblocks, L, any_dft, any_rec, kind,
                           st);
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc)
@@ -561,7 +576,12 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.nkz = tiles(p.iz, d.kb);
     d.zb = d.ztx * d.zty * d.nkz;
     d.zb_pad = tiles(d.zb, 8) * 8;
-    d.fb = d.ztx * d.zty * p.iz;
+    // x-pair kernels (two cells per lane) are opt-in: on 288^3 they run ~1.3x
+    // slower than one cell per lane (more registers, fewer waves in flight)
+    d.pair = (p.ix % 2 == 0 && env_pad("MITTET_FDTD_PAIR", 0)) ? 1 : 0;
+    d.ftx = d.pair ? tiles(p.ix, 32) : tiles(p.ix, mt::ZX);
+    d.fty = d.pair ? tiles(p.iy, 16) : tiles(p.iy, mt::ZY);
+    d.fb = d.ftx * d.fty * p.iz;
     d.fb_pad = tiles(d.fb, 8) * 8;
     d.e = s->d_e;
     d.sstride = sstride;
@@ -580,8 +600,8 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.acc = s->d_acc;
     d.step = s->d_step;
     s->nblocks_op = d.nbx * d.nby * p.iz;
-    s->zmarch = sp->zchunk > 0;
-    s->nblocks = s->zmarch ? d.zb_pad : d.fb_pad;
+    s->kind = sp->zchunk > 0 ? 2 : d.pair;
+    s->nblocks = s->kind == 2 ? d.zb_pad : d.fb_pad;
     s->ncellblocks = tiles((int)N, 256);
     if ((rc = upload_desc(s)) || (rc = fwi3d_set_zero_eh(s))) {
         fwi3d_shot_destroy(s);
@@ -709,12 +729,12 @@ int fwi3d_step_n(fwi3d_shot *s, int nsteps)
         if (L == chunk) {
             if (!s->graph &&
                 (rc = capture_chunk(&s->graph, s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre,
-                                    s->ncellblocks, L, dft, rec, s->zmarch, s->st)))
+                                    s->ncellblocks, L, dft, rec, s->kind, s->st)))
                 return rc;
             HIP_TRY(hipGraphLaunch(s->graph, s->st));
         } else {
             if ((rc = launch_chunk(s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre, s->ncellblocks, L,
-                                   dft, rec, s->zmarch, s->st)))
+                                   dft, rec, s->kind, s->st)))
                 return rc;
         }
         s->step += L;
@@ -760,14 +780,13 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
         const int L = std::min(chunk, left);
         for (int j = 0; j < L && !rc; ++j) {
             rc = timed(0, [&] {
-                return s->zmarch ? mt::launch_efield_z(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st)
-                                 : mt::launch_efield_f(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st);
+                return launch_eh(s->kind, s->d_desc, 1, s->d_pre, s->nblocks, j, false, s->st)
+                           ? hipErrorUnknown : hipSuccess;
             });
             if (!rc)
                 rc = timed(1, [&] {
-                    return s->zmarch
-                               ? mt::launch_hfield_z(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st)
-                               : mt::launch_hfield_f(s->d_desc, 1, s->d_pre, s->nblocks, j, 1, s->st);
+                    return launch_eh(s->kind, s->d_desc, 1, s->d_pre, s->nblocks, j, true, s->st)
+                               ? hipErrorUnknown : hipSuccess;
                 });
             if (!rc && s->nrec)
                 rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
@@ -1037,6 +1056,232 @@ int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue, double 
     return FWI3D_OK;
 }
 
+
+// ---------------------------------------------------------------- gradients
+
+struct fwi3d_grad {
+    fwi3d_shot *s = nullptr;
+    int seg = 0, nseg = 0, nt = 0;
+    size_t state = 0;                 // doubles per stored state
+    size_t off_h = 0, off_psi[3] = {0, 0, 0};
+    double *ckpt = nullptr;           // nseg states
+    double *buf = nullptr;            // seg + 1 states
+    double *adj = nullptr;            // adjoint state (one state)
+    double *tmp = nullptr;            // 3N + psi slabs
+    double *grad = nullptr;           // N
+    double *grad_src = nullptr;       // 1
+    double *g = nullptr;              // [it][3][nrec] trace sensitivities
+};
+
+static void state_layout(fwi3d_grad *g)
+{
+    const ShotDev &d = g->s->desc;
+    const size_t N = (size_t)d.n;
+    g->off_h = 3 * N;
+    size_t o = 6 * N;
+    for (int a = 0; a < 3; ++a) {
+        g->off_psi[a] = o;
+        o += mt::PSI_N * (size_t)d.slab[a];
+    }
+    g->state = o;
+}
+
+// copy the shot's live state to / from a stored state
+static int state_copy(fwi3d_grad *g, double *dst_state, bool save)
+{
+    fwi3d_shot *s = g->s;
+    const size_t N = (size_t)s->desc.n;
+    double *E = s->d_e;  // ring == 1: slot 0
+    auto cp = [&](double *live, double *stored, size_t count) -> hipError_t {
+        return save ? hipMemcpyAsync(stored, live, sizeof(double) * count, hipMemcpyDeviceToDevice, s->st)
+                    : hipMemcpyAsync(live, stored, sizeof(double) * count, hipMemcpyDeviceToDevice, s->st);
+    };
+    HIP_TRY(cp(E, dst_state, 3 * N));
+    HIP_TRY(cp(s->d_h, dst_state + g->off_h, 3 * N));
+    for (int a = 0; a < 3; ++a)
+        HIP_TRY(cp(s->d_psi[a], dst_state + g->off_psi[a], mt::PSI_N * (size_t)s->desc.slab[a]));
+    return FWI3D_OK;
+}
+
+static int set_step(fwi3d_shot *s, int n)
+{
+    HIP_TRY(hipMemcpyAsync(s->d_step, &n, sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->step = n;
+    return FWI3D_OK;
+}
+
+extern "C" {
+
+int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out)
+{
+    if (!s || !out)
+        return fail(FWI3D_EINVAL, "null argument");
+    *out = nullptr;
+    if (s->nf || s->ring != 1)
+        return fail(FWI3D_EINVAL, "gradient shots must not carry a volume DFT");
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot is in a batch");
+    const int nt = s->it - 1;
+    fwi3d_grad *g = new fwi3d_grad();
+    g->s = s;
+    g->nt = nt;
+    g->seg = segment > 0 ? std::min(segment, nt) : std::max(1, (int)std::ceil(std::sqrt((double)nt)));
+    g->nseg = (nt + g->seg - 1) / g->seg;
+    state_layout(g);
+    const size_t N = (size_t)s->desc.n;
+    size_t tmp = 3 * N;
+    for (int a = 0; a < 3; ++a)
+        tmp += mt::PSI_N * (size_t)s->desc.slab[a];
+    if (dalloc(&g->ckpt, g->state * g->nseg) != hipSuccess ||
+        dalloc(&g->buf, g->state * (g->seg + 1)) != hipSuccess ||
+        dalloc(&g->adj, g->state) != hipSuccess || dalloc(&g->tmp, tmp) != hipSuccess ||
+        dalloc(&g->grad, N) != hipSuccess || dalloc(&g->grad_src, 1) != hipSuccess ||
+        dalloc(&g->g, std::max<size_t>(1, 3 * (size_t)s->nrec * s->it)) != hipSuccess) {
+        fwi3d_grad_destroy(g);
+        return fail(FWI3D_ENOMEM, "gradient storage (%zu states of %zu doubles)",
+                    (size_t)g->nseg + g->seg + 2, g->state);
+    }
+    *out = g;
+    return FWI3D_OK;
+}
+
+int fwi3d_grad_destroy(fwi3d_grad *g)
+{
+    if (!g)
+        return FWI3D_OK;
+    if (g->s)
+        (void)hipStreamSynchronize(g->s->st);
+    dfree(g->ckpt);
+    dfree(g->buf);
+    dfree(g->adj);
+    dfree(g->tmp);
+    dfree(g->grad);
+    dfree(g->grad_src);
+    dfree(g->g);
+    delete g;
+    return FWI3D_OK;
+}
+
+long long fwi3d_grad_bytes(fwi3d_grad *g)
+{
+    if (!g)
+        return -1;
+    return 8LL * (long long)g->state * (g->nseg + g->seg + 2);
+}
+
+int fwi3d_grad_forward(fwi3d_grad *g)
+{
+    if (!g)
+        return fail(FWI3D_EINVAL, "null gradient");
+    fwi3d_shot *s = g->s;
+    int rc = fwi3d_set_zero_eh(s);
+    for (int q = 0; q < g->nseg && !rc; ++q) {
+        if ((rc = state_copy(g, g->ckpt + g->state * q, true)))
+            break;
+        rc = fwi3d_step_n(s, std::min(g->seg, g->nt - q * g->seg));
+    }
+    if (!rc)
+        rc = fwi3d_shot_sync(s);
+    return rc;
+}
+
+int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
+{
+    if (!g || !djdt || !djds)
+        return fail(FWI3D_EINVAL, "null argument");
+    fwi3d_shot *s = g->s;
+    fwi3d_model *m = s->m;
+    int rc = check_shot_ready(s);
+    if (rc)
+        return rc;
+    const int nrec = s->nrec, it = s->it;
+    const size_t N = (size_t)s->desc.n;
+    // sensitivities regrouped per step: g[n][comp][rec]
+    std::vector<double> gt(std::max<size_t>(1, 3 * (size_t)nrec * it));
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < nrec; ++r)
+            for (int n = 0; n < it; ++n)
+                gt[((size_t)n * 3 + c) * nrec + r] = djdt[((size_t)c * nrec + r) * it + n];
+    HIP_TRY(hipMemcpyAsync(g->g, gt.data(), sizeof(double) * gt.size(), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(mt::launch_fill(g->adj, g->state, 0.0, s->st));
+    HIP_TRY(mt::launch_fill(g->grad, N, 0.0, s->st));
+    HIP_TRY(mt::launch_fill(g->grad_src, 1, 0.0, s->st));
+    std::vector<double> amp(it, 0.0);
+    const double sig_src = s->src_cell >= 0 ? m->sig[(size_t)s->src_cell] : 1.0;
+    if (s->src_cell >= 0) {
+        const fwi3d_grid_params &p = m->p;
+        for (int n = 0; n + 1 < it; ++n)
+            amp[n] = (s->signal[n] + s->signal[n + 1]) / 2.0 * p.dt * 2.0 * p.omega0 / sig_src / p.dx /
+                     p.dy / p.dz;
+    }
+    mt::AdjArgs A{};
+    const ShotDev &d = s->desc;
+    A.ix = d.ix;
+    A.iy = d.iy;
+    A.iz = d.iz;
+    A.nc = d.nc;
+    A.n = d.n;
+    A.dt = d.dt;
+    A.mu0 = d.mu0;
+    A.c1 = d.c1;
+    A.c2 = d.c2;
+    for (int a = 0; a < 3; ++a) {
+        A.dl[a] = d.dl[a];
+        A.prof[a] = d.prof[a];
+        A.slab[a] = d.slab[a];
+        A.lpsi[a] = g->adj + g->off_psi[a];
+    }
+    A.eps = d.eps;
+    A.lE = g->adj;
+    A.lH = g->adj + g->off_h;
+    A.t3 = g->tmp;
+    size_t o = 3 * N;
+    for (int a = 0; a < 3; ++a) {
+        A.tpsi[a] = g->tmp + o;
+        o += mt::PSI_N * (size_t)d.slab[a];
+    }
+    A.grad = g->grad;
+    A.grad_src = g->grad_src;
+    A.src_cell = s->src_cell;
+    A.src_comp = s->src_comp;
+    A.sig_src = sig_src;
+    for (int q = g->nseg - 1; q >= 0; --q) {
+        const int n0 = q * g->seg, L = std::min(g->seg, g->nt - n0);
+        // replay the segment from its checkpoint, keeping every state
+        if ((rc = state_copy(g, g->ckpt + g->state * q, false)) || (rc = set_step(s, n0)) ||
+            (rc = state_copy(g, g->buf, true)))
+            return rc;
+        for (int t = 0; t < L; ++t)
+            if ((rc = fwi3d_step_n(s, 1)) || (rc = state_copy(g, g->buf + g->state * (t + 1), true)))
+                return rc;
+        for (int t = L - 1; t >= 0; --t) {
+            const int n = n0 + t;
+            const double *u0 = g->buf + g->state * t, *u1 = g->buf + g->state * (t + 1);
+            A.uE0 = u0;
+            A.uH0 = u0 + g->off_h;
+            A.uE1 = u1;
+            for (int a = 0; a < 3; ++a)
+                A.upsi1[a] = u1 + g->off_psi[a];
+            A.amp_n = amp[n];
+            HIP_TRY(mt::launch_adjoint_step(A, g->g + (size_t)n * 3 * nrec, nrec, s->d_rec, s->st));
+        }
+    }
+    std::vector<double> ge(N);
+    double gs = 0.0;
+    HIP_TRY(hipMemcpyAsync(ge.data(), g->grad, sizeof(double) * N, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpyAsync(&gs, g->grad_src, sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    const double de = 1.0 / (2.0 * m->p.omega0);
+    for (size_t c = 0; c < N; ++c)
+        djds[c] = ge[c] * de;
+    if (s->src_cell >= 0)
+        djds[s->src_cell] += gs;
+    return FWI3D_OK;
+}
+
+}  // extern "C"
+
 // ---------------------------------------------------------------- batches
 
 int fwi3d_batch_create(int max_shots, int chunk, void *stream, fwi3d_batch **out)
@@ -1097,9 +1342,10 @@ int fwi3d_batch_add(fwi3d_batch *b, fwi3d_shot *s)
         return fail(FWI3D_EBUDGET, "batch holds at most %d shots", b->max_shots);
     if (s->ring != 1 && s->ring != b->chunk)
         return fail(FWI3D_EINVAL, "shot ring %d must equal the batch chunk %d", s->ring, b->chunk);
-    if (!b->shots.empty() && s->zmarch != b->zmarch)
-        return fail(FWI3D_EINVAL, "batch members must all use (or all not use) z-marching kernels");
-    b->zmarch = s->zmarch;
+    if (!b->shots.empty() && s->kind != b->kind)
+        return fail(FWI3D_EINVAL, "batch members must use the same fused kernel family "
+                                  "(even/odd ix, zchunk)");
+    b->kind = s->kind;
     int rc = ensure_amp(s);
     if (rc)
         return rc;
@@ -1176,11 +1422,11 @@ int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
         if (L == b->chunk) {
             if (!b->graph && (rc = capture_chunk(&b->graph, b->d_desc, ns, b->d_pre, b->nblocks,
                                                  b->d_pre_cells, b->ncellblocks, L, dft, rec,
-                                                 b->zmarch, b->st)))
+                                                 b->kind, b->st)))
                 return rc;
             HIP_TRY(hipGraphLaunch(b->graph, b->st));
         } else if ((rc = launch_chunk(b->d_desc, ns, b->d_pre, b->nblocks, b->d_pre_cells,
-                                      b->ncellblocks, L, dft, rec, b->zmarch, b->st))) {
+                                      b->ncellblocks, L, dft, rec, b->kind, b->st))) {
             return rc;
         }
         left -= L;

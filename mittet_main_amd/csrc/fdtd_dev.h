@@ -35,6 +35,8 @@ struct ShotDev {
     int ztx, zty, nkz;        // z-marching tiles along x, y and z-chunks
     int zb, zb_pad;           // z-marching blocks, padded to a multiple of 8
     int fb, fb_pad;           // per-cell fused blocks (one plane each), padded to 8
+    int ftx, fty;             // per-cell fused tiles along x and y
+    int pair;                 // 1: fused kernels update x-pairs of cells (even ix)
     double *e;                // E ring: slot q, comp d at e + q*sstride + d*n
     long long sstride;        // doubles between ring slots (3n + padding)
     long long astride;        // complex elements between accumulator rows (n + padding)
@@ -56,6 +58,27 @@ struct ShotDev {
 
 enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
 
+// Arguments of one adjoint step (fdtd_adjoint.hip), passed by value.
+struct AdjArgs {
+    int ix, iy, iz, nc, n;
+    double dt, mu0, c1, c2;
+    double dl[3];
+    const double *prof[3];
+    const double *eps;
+    long long slab[3];
+    double *lE, *lH, *lpsi[3];       // adjoint state (forward layout)
+    double *t3, *tpsi[3];            // per-step weights for the transposed stencils
+    const double *uE0, *uH0;         // forward state before the step
+    const double *uE1, *upsi1[3];    // forward state after the step
+    double *grad;                    // dJ/d eps' per cell (accumulated)
+    double *grad_src;                // dJ/d sigma[src] from the source scaling
+    int src_cell, src_comp;
+    double amp_n, sig_src;
+};
+
+hipError_t launch_adjoint_step(const AdjArgs &A, const double *g, int nrec, const int *rec,
+                               hipStream_t st);
+
 // Kernel launchers (fdtd_kernels.hip).  `pre` is the exclusive prefix of
 // per-shot block counts (device), `nblocks` its total.
 hipError_t launch_efield(int mode, const ShotDev *d, int ns, const int *pre, int nblocks,
@@ -70,6 +93,10 @@ hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks
 hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
+hipError_t launch_efield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
+hipError_t launch_hfield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
                             hipStream_t st);

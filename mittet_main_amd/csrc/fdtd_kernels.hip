@@ -30,7 +30,7 @@
 #define MT_FLUSH_G 8
 #endif
 #ifndef MT_TILE_ORDER
-#define MT_TILE_ORDER 1
+#define MT_TILE_ORDER 0
 #endif
 
 // Pointers read out of the shot descriptor are generic; casting them to the
@@ -108,12 +108,22 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, int i, int jj, int
                     i < ix - 1 && jj < iy - 1;
 
     double hzy = 0.0, hyz = 0.0, hxz = 0.0, hzx = 0.0, hyx = 0.0, hxy = 0.0;
-    if (upx || fy) hzy = dminus(Hz, c, sy, c1, c2);
-    if (upx || fz) hyz = dminus(Hy, c, sz, c1, c2);
-    if (upy || fz) hxz = dminus(Hx, c, sz, c1, c2);
-    if (upy || fx) hzx = dminus(Hz, c, 1, c1, c2);
-    if (upz || fx) hyx = dminus(Hy, c, 1, c1, c2);
-    if (upz || fy) hxy = dminus(Hx, c, sy, c1, c2);
+    if (i >= 2 && i < ix - 1 && jj >= 2 && jj < iy - 1 && k >= 2 && k < iz - 1) {
+        // every backward stencil is in bounds: issue all loads up front
+        hzy = dminus(Hz, c, sy, c1, c2);
+        hyz = dminus(Hy, c, sz, c1, c2);
+        hxz = dminus(Hx, c, sz, c1, c2);
+        hzx = dminus(Hz, c, 1, c1, c2);
+        hyx = dminus(Hy, c, 1, c1, c2);
+        hxy = dminus(Hx, c, sy, c1, c2);
+    } else {
+        if (upx || fy) hzy = dminus(Hz, c, sy, c1, c2);
+        if (upx || fz) hyz = dminus(Hy, c, sz, c1, c2);
+        if (upy || fz) hxz = dminus(Hx, c, sz, c1, c2);
+        if (upy || fx) hzx = dminus(Hz, c, 1, c1, c2);
+        if (upz || fx) hyx = dminus(Hy, c, 1, c1, c2);
+        if (upz || fy) hxy = dminus(Hx, c, sy, c1, c2);
+    }
 
     const double eps2 = gc(S.eps)[c];
     const double sex = px[PR_ESIG * ix + i] * dt / (2.0 * eps2);
@@ -255,12 +265,22 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, int i, int jj, int
         return;
 
     double ezy = 0.0, eyz = 0.0, exz = 0.0, ezx = 0.0, eyx = 0.0, exy = 0.0;
-    if (upx || gy) ezy = dplus(Ez, c, sy, c1, c2);
-    if (upx || gz) eyz = dplus(Ey, c, sz, c1, c2);
-    if (upy || gz) exz = dplus(Ex, c, sz, c1, c2);
-    if (upy || gx) ezx = dplus(Ez, c, 1, c1, c2);
-    if (upz || gx) eyx = dplus(Ey, c, 1, c1, c2);
-    if (upz || gy) exy = dplus(Ex, c, sy, c1, c2);
+    if (i >= 1 && i < ix - 2 && jj >= 1 && jj < iy - 2 && k >= 1 && k < iz - 2) {
+        // every forward stencil is in bounds: issue all loads up front
+        ezy = dplus(Ez, c, sy, c1, c2);
+        eyz = dplus(Ey, c, sz, c1, c2);
+        exz = dplus(Ex, c, sz, c1, c2);
+        ezx = dplus(Ez, c, 1, c1, c2);
+        eyx = dplus(Ey, c, 1, c1, c2);
+        exy = dplus(Ex, c, sy, c1, c2);
+    } else {
+        if (upx || gy) ezy = dplus(Ez, c, sy, c1, c2);
+        if (upx || gz) eyz = dplus(Ey, c, sz, c1, c2);
+        if (upy || gz) exz = dplus(Ex, c, sz, c1, c2);
+        if (upy || gx) ezx = dplus(Ez, c, 1, c1, c2);
+        if (upz || gx) eyx = dplus(Ey, c, 1, c1, c2);
+        if (upz || gy) exy = dplus(Ex, c, sy, c1, c2);
+    }
 
     // da = 1 and db = dt/mu0 wherever the magnetic CPML conductivity is zero;
     // eps' is only needed inside the shells.
@@ -935,15 +955,15 @@ __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, 
     const int L = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
     if (L >= S.fb)
         return false;
-    const int tx = L % S.ztx, r = L / S.ztx;
+    const int tx = L % S.ftx, r = L / S.ftx;
 #if MT_TILE_ORDER == 1
     // x fastest, then z, then y: an XCD sweeps an 8-row band through every
     // plane, so the z-neighbour planes it re-reads are still in its L2
     k = r % S.iz;
     const int ty = r / S.iz;
 #else
-    const int ty = r % S.zty;
-    k = r / S.zty;
+    const int ty = r % S.fty;
+    k = r / S.fty;
 #endif
     i = tx * ZX + (threadIdx.x & (ZX - 1));
     jj = ty * ZY + threadIdx.x / ZX;
@@ -1080,6 +1100,213 @@ hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks
     if (nblocks <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_hfield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
+}
+
+
+// ------------------------------------------------------------------------
+// Pair variant of the fused per-cell kernels (even ix): every lane updates the
+// x-pair (i, i+1) with 16-byte loads, halving the load instructions of the
+// stencil.  Tiles are 32 x 16 cells (16 x 16 lanes); same arithmetic, same
+// fast/general split (a wave with any non-interior pair runs the general code
+// for both of its cells).
+
+__device__ __forceinline__ double2 ld2(const GLOBAL double *p, size_t q)
+{
+    const GLOBAL double2 *v = (const GLOBAL double2 *)(p + q);
+    double2 r;
+    r.x = (*v).x;
+    r.y = (*v).y;
+    return r;
+}
+
+__device__ __forceinline__ void st2(GLOBAL double *p, size_t q, double a, double b)
+{
+    p[q] = a;
+    p[q + 1] = b;
+}
+
+__device__ __forceinline__ bool fpair(const ShotDev &S, int b, int &i, int &jj, int &k)
+{
+    const int L = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
+    if (L >= S.fb)
+        return false;
+    const int tx = L % S.ftx, r = L / S.ftx;
+    const int ty = r % S.fty;
+    k = r / S.fty;
+    i = tx * 32 + 2 * (threadIdx.x & 15);
+    jj = ty * 16 + (threadIdx.x >> 4);
+    return i < S.ix && jj < S.iy;
+}
+
+__global__ __launch_bounds__(256) void k_efield_p(const ShotDev *__restrict__ D, int ns,
+                                                  const int *__restrict__ pre, int j, int sign)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    const int n = *S.step + j;
+    if (n >= S.it - 1)
+        return;
+    int i, jj, k;
+    if (!fpair(S, blockIdx.x - pre[s], i, jj, k))
+        return;
+    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
+    const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
+    const bool inner =
+        i >= nc && i + 1 < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
+    if (!__all(inner)) {
+        efield_cell<M_UPD | M_PSI | M_SRC>(S, i, jj, k, n, si, so, sign);
+        efield_cell<M_UPD | M_PSI | M_SRC>(S, i + 1, jj, k, n, si, so, sign);
+        return;
+    }
+    const size_t N = (size_t)S.n;
+    const int sy = ix, sz = ix * iy;
+    const size_t c = ((size_t)k * iy + jj) * ix + i;
+    const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
+    GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Hx = gc(S.h);
+    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
+    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * N);
+    const double2 ex0 = ld2(Ein, c), ey0 = ld2(Ein, N + c), ez0 = ld2(Ein, 2 * N + c);
+    const double2 cb = ld2(gc(S.cbi), c);
+    const double2 z0 = ld2(Hz, c), zym1 = ld2(Hz, c - sy), zyp1 = ld2(Hz, c + sy),
+                  zym2 = ld2(Hz, c - 2 * sy), zxm = ld2(Hz, c - 2), zxp = ld2(Hz, c + 2);
+    const double2 y0 = ld2(Hy, c), yzm1 = ld2(Hy, c - sz), yzp1 = ld2(Hy, c + sz),
+                  yzm2 = ld2(Hy, c - 2 * sz), yxm = ld2(Hy, c - 2), yxp = ld2(Hy, c + 2);
+    const double2 x0 = ld2(Hx, c), xzm1 = ld2(Hx, c - sz), xzp1 = ld2(Hx, c + sz),
+                  xzm2 = ld2(Hx, c - 2 * sz), xym1 = ld2(Hx, c - sy), xyp1 = ld2(Hx, c + sy),
+                  xym2 = ld2(Hx, c - 2 * sy);
+    const double2 kedx = ld2(gc(S.prof[0]), (size_t)PR_KED * ix + i);
+    const double kedy = gc(S.prof[1])[PR_KED * iy + jj];
+    const double kedz = gc(S.prof[2])[PR_KED * iz + k];
+    const double c1 = S.c1, c2 = S.c2;
+    // cell a = i
+    const double hzy_a = c1 * z0.x - c1 * zym1.x + c2 * zyp1.x - c2 * zym2.x;
+    const double hyz_a = c1 * y0.x - c1 * yzm1.x + c2 * yzp1.x - c2 * yzm2.x;
+    const double hxz_a = c1 * x0.x - c1 * xzm1.x + c2 * xzp1.x - c2 * xzm2.x;
+    const double hzx_a = c1 * z0.x - c1 * zxm.y + c2 * z0.y - c2 * zxm.x;
+    const double hyx_a = c1 * y0.x - c1 * yxm.y + c2 * y0.y - c2 * yxm.x;
+    const double hxy_a = c1 * x0.x - c1 * xym1.x + c2 * xyp1.x - c2 * xym2.x;
+    // cell b = i + 1
+    const double hzy_b = c1 * z0.y - c1 * zym1.y + c2 * zyp1.y - c2 * zym2.y;
+    const double hyz_b = c1 * y0.y - c1 * yzm1.y + c2 * yzp1.y - c2 * yzm2.y;
+    const double hxz_b = c1 * x0.y - c1 * xzm1.y + c2 * xzp1.y - c2 * xzm2.y;
+    const double hzx_b = c1 * z0.y - c1 * z0.x + c2 * zxp.x - c2 * zxm.y;
+    const double hyx_b = c1 * y0.y - c1 * y0.x + c2 * yxp.x - c2 * yxm.y;
+    const double hxy_b = c1 * x0.y - c1 * xym1.y + c2 * xyp1.y - c2 * xym2.y;
+    const double cux_a = hzy_a / kedy - hyz_a / kedz, cux_b = hzy_b / kedy - hyz_b / kedz;
+    const double cuy_a = hxz_a / kedz - hzx_a / kedx.x, cuy_b = hxz_b / kedz - hzx_b / kedx.y;
+    const double cuz_a = hyx_a / kedx.x - hxy_a / kedy, cuz_b = hyx_b / kedx.y - hxy_b / kedy;
+    double exa, exb, eya, eyb, eza, ezb;
+    if (sign > 0) {
+        exa = ex0.x + cb.x * cux_a; exb = ex0.y + cb.y * cux_b;
+        eya = ey0.x + cb.x * cuy_a; eyb = ey0.y + cb.y * cuy_b;
+        eza = ez0.x + cb.x * cuz_a; ezb = ez0.y + cb.y * cuz_b;
+    } else {
+        exa = ex0.x - cb.x * cux_a; exb = ex0.y - cb.y * cux_b;
+        eya = ey0.x - cb.x * cuy_a; eyb = ey0.y - cb.y * cuy_b;
+        eza = ez0.x - cb.x * cuz_a; ezb = ez0.y - cb.y * cuz_b;
+    }
+    if ((long long)c == S.src_cell || (long long)c + 1 == S.src_cell) {
+        const double a = gc(S.amp)[n];
+        const bool second = (long long)c + 1 == S.src_cell;
+        if (S.src_comp == 0) {
+            if (second) exb -= a; else exa -= a;
+        } else if (S.src_comp == 1) {
+            if (second) eyb -= a; else eya -= a;
+        } else {
+            if (second) ezb -= a; else eza -= a;
+        }
+    }
+    st2(Eout, c, exa, exb);
+    st2(Eout, N + c, eya, eyb);
+    st2(Eout, 2 * N + c, eza, ezb);
+}
+
+__global__ __launch_bounds__(256) void k_hfield_p(const ShotDev *__restrict__ D, int ns,
+                                                  const int *__restrict__ pre, int j, int sign)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    if (*S.step + j >= S.it - 1)
+        return;
+    int i, jj, k;
+    if (!fpair(S, blockIdx.x - pre[s], i, jj, k))
+        return;
+    const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
+    const int so = (S.cur + j + 1) % S.ring;
+    const bool inner =
+        i >= nc && i + 1 < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
+    if (!__all(inner)) {
+        hfield_cell<M_UPD | M_PSI>(S, i, jj, k, so, sign);
+        hfield_cell<M_UPD | M_PSI>(S, i + 1, jj, k, so, sign);
+        return;
+    }
+    const size_t N = (size_t)S.n;
+    const int sy = ix, sz = ix * iy;
+    const size_t c = ((size_t)k * iy + jj) * ix + i;
+    const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *__restrict__ Ey = Ex + N;
+    const GLOBAL double *__restrict__ Ez = Ex + 2 * N;
+    GLOBAL double *__restrict__ Hx = gm(S.h);
+    GLOBAL double *__restrict__ Hy = gm(S.h + N);
+    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * N);
+    const double2 hx0 = ld2(Hx, c), hy0 = ld2(Hy, c), hz0 = ld2(Hz, c);
+    const double2 z0 = ld2(Ez, c), zyp1 = ld2(Ez, c + sy), zyp2 = ld2(Ez, c + 2 * sy),
+                  zym1 = ld2(Ez, c - sy), zxm = ld2(Ez, c - 2), zxp = ld2(Ez, c + 2);
+    const double2 y0 = ld2(Ey, c), yzp1 = ld2(Ey, c + sz), yzp2 = ld2(Ey, c + 2 * sz),
+                  yzm1 = ld2(Ey, c - sz), yxm = ld2(Ey, c - 2), yxp = ld2(Ey, c + 2);
+    const double2 x0 = ld2(Ex, c), xzp1 = ld2(Ex, c + sz), xzp2 = ld2(Ex, c + 2 * sz),
+                  xzm1 = ld2(Ex, c - sz), xyp1 = ld2(Ex, c + sy), xyp2 = ld2(Ex, c + 2 * sy),
+                  xym1 = ld2(Ex, c - sy);
+    const double2 khdx = ld2(gc(S.prof[0]), (size_t)PR_KHD * ix + i);
+    const double khdy = gc(S.prof[1])[PR_KHD * iy + jj];
+    const double khdz = gc(S.prof[2])[PR_KHD * iz + k];
+    const double c1 = S.c1, c2 = S.c2;
+    const double db = S.dt / S.mu0 / 1.0;
+    // cell a = i: forward x stencil uses i-1 .. i+2
+    const double ezy_a = c1 * zyp1.x - c1 * z0.x + c2 * zyp2.x - c2 * zym1.x;
+    const double eyz_a = c1 * yzp1.x - c1 * y0.x + c2 * yzp2.x - c2 * yzm1.x;
+    const double exz_a = c1 * xzp1.x - c1 * x0.x + c2 * xzp2.x - c2 * xzm1.x;
+    const double ezx_a = c1 * z0.y - c1 * z0.x + c2 * zxp.x - c2 * zxm.y;
+    const double eyx_a = c1 * y0.y - c1 * y0.x + c2 * yxp.x - c2 * yxm.y;
+    const double exy_a = c1 * xyp1.x - c1 * x0.x + c2 * xyp2.x - c2 * xym1.x;
+    // cell b = i + 1: i .. i+3
+    const double ezy_b = c1 * zyp1.y - c1 * z0.y + c2 * zyp2.y - c2 * zym1.y;
+    const double eyz_b = c1 * yzp1.y - c1 * y0.y + c2 * yzp2.y - c2 * yzm1.y;
+    const double exz_b = c1 * xzp1.y - c1 * x0.y + c2 * xzp2.y - c2 * xzm1.y;
+    const double ezx_b = c1 * zxp.x - c1 * z0.y + c2 * zxp.y - c2 * z0.x;
+    const double eyx_b = c1 * yxp.x - c1 * y0.y + c2 * yxp.y - c2 * y0.x;
+    const double exy_b = c1 * xyp1.y - c1 * x0.y + c2 * xyp2.y - c2 * xym1.y;
+    const double cux_a = ezy_a / khdy - eyz_a / khdz, cux_b = ezy_b / khdy - eyz_b / khdz;
+    const double cuy_a = exz_a / khdz - ezx_a / khdx.x, cuy_b = exz_b / khdz - ezx_b / khdx.y;
+    const double cuz_a = eyx_a / khdx.x - exy_a / khdy, cuz_b = eyx_b / khdx.y - exy_b / khdy;
+    if (sign > 0) {
+        st2(Hx, c, hx0.x - db * cux_a, hx0.y - db * cux_b);
+        st2(Hy, c, hy0.x - db * cuy_a, hy0.y - db * cuy_b);
+        st2(Hz, c, hz0.x - db * cuz_a, hz0.y - db * cuz_b);
+    } else {
+        st2(Hx, c, hx0.x + db * cux_a, hx0.y + db * cux_b);
+        st2(Hy, c, hy0.x + db * cuy_a, hy0.y + db * cuy_b);
+        st2(Hz, c, hz0.x + db * cuz_a, hz0.y + db * cuz_b);
+    }
+}
+
+hipError_t launch_efield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_efield_p, dim3(nblocks), dim3(256), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_hfield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_hfield_p, dim3(nblocks), dim3(256), 0, st, d, ns, pre, j, sign);
     return hipGetLastError();
 }
 

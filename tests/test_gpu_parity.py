@@ -233,3 +233,35 @@ def test_full_size_256_linearity_and_zero():
     assert not np.any(z.field(0))
     z.close()
     assert np.abs(ta).max() > 0
+
+
+@pytest.mark.parametrize("dims,pair", [((29, 26, 22), 0), ((31, 23, 25), 0), ((30, 26, 22), 1),
+                                       ((32, 23, 25), 1)])
+def test_fused_run_bitwise_tilings(dims, pair, monkeypatch):
+    """Ragged tiles of the one-cell-per-lane fused kernels, and the opt-in
+    two-cells-per-lane (x-pair) kernels, give the oracle's bits."""
+    monkeypatch.setenv("MITTET_FDTD_PAIR", str(pair))
+    ix, iy, iz = dims
+    it = 70
+    m, shot, S, sig, signal, src, rec, om, dt = setup_pair(ix=ix, iy=iy, iz=iz, seed=21, it=it,
+                                                            ring=8)
+    shot.run()
+    w = oracle.dft_weights(it, dt, OM0, om)
+    tr, acc = S.forward(0, signal, tuple(c + 1 for c in src), it - 1, rec1=rec + 1, it=it, weights=w)
+    np.testing.assert_array_equal(shot.traces()[:3], tr)
+    for q, nm in enumerate(oracle.FIELD_NAMES):
+        np.testing.assert_array_equal(shot.field(q), S.arr(nm), err_msg=nm)
+    np.testing.assert_array_equal(shot.volume_dft().reshape(3, len(om), -1), acc)
+
+
+def test_zmarch_kernels_bitwise():
+    """The z-marching kernel family (zchunk > 0) gives the same bits."""
+    it = 60
+    m, shot, S, sig, signal, src, rec, om, dt = setup_pair(seed=22, it=it, ring=8)
+    shot.close()
+    shot = m.shot(it, signal, src, 0, rec, om, ring=8, zchunk=5)
+    shot.run()
+    w = oracle.dft_weights(it, dt, OM0, om)
+    tr, acc = S.forward(0, signal, tuple(c + 1 for c in src), it - 1, rec1=rec + 1, it=it, weights=w)
+    np.testing.assert_array_equal(shot.traces()[:3], tr)
+    np.testing.assert_array_equal(shot.volume_dft().reshape(3, len(om), -1), acc)
