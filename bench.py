@@ -33,6 +33,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, ring, kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy kernel), or None
+    when there is none for this workload and ring or the kernel sources have
+    changed since it was measured."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", f"{workload}_pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    src = b"".join(open(os.path.join(ROOT, "mittet_main_amd", "csrc", n), "rb").read()
+                   for n in ("fdtd_kernels.hip", "fdtd_dev.h"))
+    if rec.get("ring") != ring or rec.get("sources_sha256_16") != hashlib.sha256(src).hexdigest()[:16]:
+        return None
+    k = rec["kernels"].get(kernel)
+    return None if k is None else int(round(k["traffic_bytes"]))
+
+
 def slab_cells(g):
     nc2 = 2 * g.ncpml
     return g.iz * g.iy * nc2, g.iz * nc2 * g.ix, nc2 * g.iy * g.ix
@@ -60,7 +79,9 @@ def survey_step_bytes(g, nf, nrec):
     return N * (6 * 16 + 8) + 3 * 2 * g.ncpml * n2 * 64 + N * 3 * nf * 32 + nrec * 3 * nf * 32
 
 
-def cpu_baseline(wl, steps):
+def cpu_baseline(wl, seconds):
+    """The oracle (reference loop structure, OpenMP) on the same workload: one
+    untimed warm-up step, then whole steps until `seconds` of wall time."""
     import oracle
     g = wl.grid
     S = oracle.OracleSim(g.ix, g.iy, g.iz, g.ncpml, g.dx, g.dy, g.dz, g.dt, g.omega0, g.cmax)
@@ -73,9 +94,12 @@ def cpu_baseline(wl, steps):
     src1 = tuple(c + 1 for c in wl.src)
     S.forward(0, wl.signal, src1, 1, rec1=wl.receivers + 1, it=wl.it, weights=w, acc=acc,
               traces=traces)                      # warm-up step
+    steps = 0
     t0 = time.perf_counter()
-    S.forward(0, wl.signal, src1, steps, rec1=wl.receivers + 1, it=wl.it, step0=1, weights=w,
-              acc=acc, traces=traces)
+    while steps < 2 or time.perf_counter() - t0 < seconds:
+        S.forward(0, wl.signal, src1, 1, rec1=wl.receivers + 1, it=wl.it, step0=1 + steps,
+                  weights=w, acc=acc, traces=traces)
+        steps += 1
     dt = time.perf_counter() - t0
     return {"value": g.ncells * steps / dt / 1e9, "unit": "Gcell-updates/s",
             "cores": oracle.num_threads(), "kind": "port",
@@ -88,28 +112,38 @@ def cpu_baseline(wl, steps):
 def serve_mode(args):
     """Served forward modelling: Poisson stream (mean 20 req/s), grids
     {96,128,192,256}^3 + 12-cell CPML, 1-16 shots, Nt = 30 x n, receiver spectra
-    at 4 frequencies, 200 GB admission budget; vs the sequential plain path."""
+    at 4 frequencies, 200 GB HBM budget.  Compared with the sequential plain
+    path (one shot at a time in arrival order, graph-captured fused loop) on the
+    whole stream, and with the reference-named per-step operators on the
+    smallest requests."""
     from mittet_main_amd import serving
     reqs = serving.request_stream(args.seed, args.requests, max_shots=args.max_shots)
-    log(f"[serve] {len(reqs)} requests, {sum(r.nshots for r in reqs)} shots, sizes "
-        f"{[r.n_int for r in reqs]}")
+    log(f"[serve] {len(reqs)} requests, {sum(r.nshots for r in reqs)} shots, "
+        f"(n_int, shots): {[(r.n_int, r.nshots) for r in reqs]}")
+    done = lambda r: log(f"[serve]   request {r.rid} done, latency {r.latency:.2f} s")
     srv = serving.Server(budget_bytes=200e9, chunk=32)
-    served = srv.run(reqs, realtime=True)
+    served = srv.run(reqs, realtime=True, on_done=done)
     srv.close()
-    log(f"[serve] served: {served['shots_per_s']:.3f} shots/s, p50 {served['p50_s']:.2f} s")
-    seq = serving.run_sequential(reqs, mode="fused", realtime=True)
-    log(f"[serve] sequential fused: {seq['shots_per_s']:.3f} shots/s")
-    out = {"mode": "serve", "requests": len(reqs), "shots": served["shots"],
-           "served": {k: v for k, v in served.items() if k != "results"},
-           "sequential_fused": {k: v for k, v in seq.items() if k != "results"},
-           "speedup_vs_sequential_fused": served["shots_per_s"] / seq["shots_per_s"]}
+    log(f"[serve] served: {served['shots_per_s']:.3f} shots/s, p50 {served['p50_s']:.2f} s, "
+        f"p99 {served['p99_s']:.2f} s")
+    seq = serving.run_sequential(reqs, mode="fused", realtime=True, on_done=done)
+    log(f"[serve] sequential fused: {seq['shots_per_s']:.3f} shots/s, p50 {seq['p50_s']:.2f} s")
+    strip = lambda d: {k: (round(v, 4) if isinstance(v, float) else v)
+                       for k, v in d.items() if k != "results"}
+    out = {"mode": "serve", "metric": "served shots/s, p50/p95/p99 latency",
+           "requests": len(reqs), "shots": served["shots"], "seed": args.seed,
+           "served": strip(served), "sequential_fused": strip(seq),
+           "throughput_vs_sequential": round(served["shots_per_s"] / seq["shots_per_s"], 3),
+           "p50_vs_sequential": round(served["p50_s"] / seq["p50_s"], 3)}
     if args.ops_requests:
-        sub = reqs[:args.ops_requests]
+        sub = sorted(reqs, key=serving.request_work)[:args.ops_requests]
         ops = serving.run_sequential(sub, mode="ops", realtime=False)
         fus = serving.run_sequential(sub, mode="fused", realtime=False)
-        out["per_operator_launches_subset"] = {
-            "requests": len(sub), "ops_shots_per_s": ops["shots_per_s"],
-            "fused_shots_per_s": fus["shots_per_s"]}
+        out["per_operator_vs_fused_subset"] = {
+            "requests": [(r.n_int, r.nshots) for r in sub],
+            "ops_shots_per_s": round(ops["shots_per_s"], 4),
+            "fused_shots_per_s": round(fus["shots_per_s"], 4)}
+        log(f"[serve] subset: ops {ops['shots_per_s']:.3f} vs fused {fus['shots_per_s']:.3f} shots/s")
     same = all(np.array_equal(a, b) for rid, r in seq["results"].items()
                for a, b in zip(r.spectra, served["results"][rid].spectra))
     out["served_equals_alone_bitwise"] = bool(same)
@@ -117,37 +151,37 @@ def serve_mode(args):
 
 
 def gradient_mode(args):
-    """Adjoint gradient: 128^3 + 12-cell CPML, 8 x-dipole shots, 64 receivers,
-    L2 misfit on complex receiver E at 4 frequencies, Nt = 30 x 128."""
+    """Adjoint gradient: 128^3 interior + 12-cell CPML, 8 x-dipole shots, 64
+    receivers, L2 misfit on complex receiver E at 4 frequencies, Nt = 30 x 128;
+    observed data from the true model, gradient at the smoothed start model."""
     from mittet_main_amd import configs
     from mittet_main_amd.adjoint import GradientProblem, ShotSpec
-    n_int, nc, it = 128, 12, args.grad_nt
-    sig_true, k_top = configs.layered_box_sigma(n_int, nc, args.seed)
-    n = n_int + 2 * nc
-    sig0 = np.full_like(sig_true, float(np.exp(np.log(sig_true).mean())))
-    dt, cmax = configs.dt_for(50.0, 50.0, 50.0)
-    from mittet_main_amd import Grid
-    g = Grid(n, n, n, nc, 50.0, 50.0, 50.0, dt, configs.OMEGA0, cmax)
-    sigw = configs.firstderiv_gauss(it, dt, configs.FMAX_W)
-    rng = np.random.default_rng(args.seed)
-    shots = [ShotSpec((int(x), int(y), k_top + 2), sigw)
-             for x, y in rng.integers(nc + 8, n - nc - 8, size=(8, 2))]
-    gg = np.linspace(nc + 4, n - nc - 5, 8).round().astype(int)
-    X, Y = np.meshgrid(gg, gg, indexing="xy")
-    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, k_top)], axis=1)
-    om = 2 * np.pi * np.geomspace(0.25, 2.0, 4)
-    P = GradientProblem(g, shots, rec, it, om, segment=args.segment)
+    wl = configs.gradient_128(it=args.grad_nt)
+    g = wl.grid
+    shots = [ShotSpec(tuple(s), wl.signal) for s in wl.sources]
+    P = GradientProblem(g, shots, wl.receivers, wl.it, wl.omegas, segment=args.segment)
+    log(f"[grad] {wl.name}: {g.ix}^3 cells, it={wl.it}, {len(shots)} shots, "
+        f"{len(wl.receivers)} receivers, {len(wl.omegas)} frequencies")
     t0 = time.perf_counter()
-    P.d_obs = P.forward_data(sig_true)
+    P.d_obs = P.forward_data(wl.sigma_true)
     t_fwd = time.perf_counter() - t0
+    log(f"[grad] observed data: {t_fwd:.2f} s")
+    logs0 = np.log(wl.sigma_start)
+    P.value_and_grad(logs0)                       # warm-up (allocations, graphs)
     t0 = time.perf_counter()
-    J, grad = P.value_and_grad(np.log(sig0))
+    J, grad = P.value_and_grad(logs0)
     t_grad = time.perf_counter() - t0
-    out = {"mode": "gradient", "workload": "adjoint_gradient_128cube_fp64", "cells": g.ncells,
-           "nt": it, "shots": len(shots), "nrec": len(rec), "nf": len(om),
-           "forward_all_shots_s": t_fwd, "gradient_s": t_grad, "gradients_per_s": 1.0 / t_grad,
+    log(f"[grad] gradient: {t_grad:.2f} s")
+    cell_steps = g.ncells * (wl.it - 1) * len(shots)
+    out = {"mode": "gradient", "workload": wl.name, "cells": g.ncells, "nt": wl.it,
+           "shots": len(shots), "nrec": len(wl.receivers), "nf": len(wl.omegas),
+           "forward_all_shots_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+           "gradients_per_s": round(1.0 / t_grad, 5),
+           "gradient_over_forward": round(t_grad / t_fwd, 3),
+           "forward_gcell_updates_per_s": round(cell_steps / t_fwd / 1e9, 3),
            "misfit": J, "grad_norm": float(np.linalg.norm(grad)),
-           "checkpoint_storage_gb": P.storage_bytes() / 1e9, "segment": args.segment}
+           "checkpoint_storage_gb": round(P.storage_bytes() / 1e9, 3), "segment": args.segment,
+           "dtype": "f64", "data": "synthetic"}
     P.close()
     print(json.dumps(out), flush=True)
 
@@ -156,10 +190,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mode", default="forward", choices=["forward", "serve", "gradient"])
     ap.add_argument("--seed", type=int, default=11)
-    ap.add_argument("--requests", type=int, default=6)
+    ap.add_argument("--requests", type=int, default=4)
     ap.add_argument("--max-shots", type=int, default=16)
-    ap.add_argument("--ops-requests", type=int, default=0)
-    ap.add_argument("--grad-nt", type=int, default=3840)
+    ap.add_argument("--ops-requests", type=int, default=1)
+    ap.add_argument("--grad-nt", type=int, default=0, help="time steps (default 30 x 128)")
     ap.add_argument("--segment", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -168,7 +202,8 @@ def main():
     ap.add_argument("--ring", type=int, default=32)
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="wall time of the CPU-oracle sample (whole steps, at least 2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (counter calibration)")
@@ -245,7 +280,8 @@ def main():
                    "ring": args.ring, "zchunk": args.zchunk, "sigma_sha256_16": configs.sigma_digest(wl.sigma)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(wl.name, args.ring, dom),
                      "bytes_per_launch": kb[dom], "ms_per_launch": round(dom_ms, 5)},
         "kernel_ms": {k: round(v, 5) for k, v in prof.items()},
         "step_vs_survey_bytes": {"survey_bytes_per_step": sb,
@@ -255,8 +291,8 @@ def main():
     shot.close()
     model.close()
     if not args.no_cpu_baseline:
-        log(f"[bench] cpu baseline: {args.cpu_steps} steps of the oracle")
-        result["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+        log(f"[bench] cpu baseline: {args.cpu_seconds:.0f} s of oracle steps")
+        result["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result), flush=True)
 

@@ -54,43 +54,46 @@ class ShotSpec:
 class GradientProblem:
     """Misfit J(sigma) over several shots sharing a grid, receivers and frequencies.
 
-    Shots, checkpoint storage and the model stay resident between evaluations;
-    each evaluation uploads the new sigma (media_coeff_3d) and runs, per shot,
-    the checkpointed forward pass and the adjoint sweep."""
+    The model and the shots stay resident between evaluations; each evaluation
+    uploads the new sigma (media_coeff_3d).  Misfit-only evaluations run each
+    shot's plain time loop.  A gradient evaluation takes the shots one at a
+    time: it attaches checkpoint storage to the shot (one state every `segment`
+    steps plus one replayed segment), runs the checkpointed forward pass and the
+    adjoint sweep, and frees the storage, so the HBM held at once is one shot's
+    checkpoints whatever the number of shots."""
 
     def __init__(self, grid: Grid, shots: Sequence[ShotSpec], receivers, it: int, omegas,
                  d_obs: Optional[List[np.ndarray]] = None, segment: int = 0):
         self.grid = grid
         self.it = int(it)
+        self.segment = int(segment)
         self.omegas = np.asarray(omegas, dtype=np.float64)
         self.receivers = np.ascontiguousarray(np.asarray(receivers, dtype=np.int32).reshape(-1, 3))
         self.model = Model(grid)
         self.specs = list(shots)
         self.shots = [self.model.shot(self.it, sp.signal, sp.src, sp.src_comp, self.receivers)
                       for sp in self.specs]
-        self.grads = []
-        for sh in self.shots:
-            h = C.c_void_p()
-            check(_lib.load().fwi3d_grad_create(sh._h, int(segment), C.byref(h)))
-            self.grads.append(h)
         self.w = dft_weights(self.it, grid.dt, grid.omega0, self.omegas)
         self.d_obs = d_obs
+        self.peak_storage = 0
 
     def close(self):
-        for h in self.grads:
-            check(_lib.load().fwi3d_grad_destroy(h))
-        self.grads = []
+        for sh in self.shots:
+            sh.close()
+        self.shots = []
         self.model.close()
 
     def storage_bytes(self) -> int:
-        return sum(int(_lib.load().fwi3d_grad_bytes(h)) for h in self.grads)
+        """Largest checkpoint storage held during a gradient evaluation so far."""
+        return self.peak_storage
 
     def forward_data(self, sigma: np.ndarray) -> List[np.ndarray]:
         """Complex receiver spectra (3, nrec, nf) of every shot for sigma."""
         self.model.media_coeff_3d(sigma)
         out = []
-        for sh, h in zip(self.shots, self.grads):
-            check(_lib.load().fwi3d_grad_forward(h))
+        for sh in self.shots:
+            sh.set_zero_eh()
+            sh.run()
             out.append(receiver_spectra(sh.traces()[:3], self.w))
         return out
 
@@ -102,16 +105,22 @@ class GradientProblem:
         """J and dJ/dlog(sigma) (same shape as log_sigma)."""
         sigma = np.exp(np.asarray(log_sigma, dtype=np.float64))
         self.model.media_coeff_3d(sigma)
+        L = _lib.load()
         total, terms = np.zeros(sigma.size), []
-        for sh, h, obs in zip(self.shots, self.grads, self.d_obs):
-            check(_lib.load().fwi3d_grad_forward(h))
-            tr = sh.traces()[:3]
-            J, g = misfit_terms(tr, obs, self.w)
-            terms.append(J)
-            dj = np.zeros(sigma.size)
-            gg = np.ascontiguousarray(g)
-            check(_lib.load().fwi3d_grad_backward(h, gg.ctypes.data_as(C.c_void_p),
-                                                  dj.ctypes.data_as(C.c_void_p)))
+        for sh, obs in zip(self.shots, self.d_obs):
+            h = C.c_void_p()
+            check(L.fwi3d_grad_create(sh._h, self.segment, C.byref(h)))
+            try:
+                self.peak_storage = max(self.peak_storage, int(L.fwi3d_grad_bytes(h)))
+                check(L.fwi3d_grad_forward(h))
+                J, g = misfit_terms(sh.traces()[:3], obs, self.w)
+                terms.append(J)
+                dj = np.zeros(sigma.size)
+                gg = np.ascontiguousarray(g)
+                check(L.fwi3d_grad_backward(h, gg.ctypes.data_as(C.c_void_p),
+                                            dj.ctypes.data_as(C.c_void_p)))
+            finally:
+                check(L.fwi3d_grad_destroy(h))
             total += dj
         return math.fsum(terms), (total * sigma.reshape(-1)).reshape(np.shape(log_sigma))
 

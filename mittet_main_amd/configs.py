@@ -121,3 +121,60 @@ def scaled(n_int: int, ncpml: int, it: int, nf: int = 4, seed: int = 3, nrec_sid
     freqs = list(np.geomspace(0.25, 2.0, nf))
     return make(f"forward_{n_int}cube_fp64", n_int, ncpml, it, freqs, nrec_side=nrec_side,
                 seed=seed, vol_dft=vol_dft)
+
+
+def smoothed_background(n_int: int, ncpml: int, seed: int, half_width: int = 0) -> np.ndarray:
+    """Starting model of the gradient workload: the three layers of
+    layered_box_sigma (same seed, so the same layer values) without the boxes,
+    log(sigma) smoothed along z by a moving average over 2*half_width+1 cells
+    (default half_width = n_int // 16)."""
+    rng = np.random.default_rng(seed)
+    n = n_int + 2 * ncpml
+    layers = np.exp(rng.uniform(np.log(SIG_MIN), np.log(SIG_MAX), 3))
+    k1 = ncpml + n_int // 3
+    k2 = ncpml + (2 * n_int) // 3
+    col = np.empty(n)
+    col[:k1], col[k1:k2], col[k2:] = np.log(layers[0]), np.log(layers[1]), np.log(layers[2])
+    hw = half_width or max(1, n_int // 16)
+    pad = np.concatenate([np.full(hw, col[0]), col, np.full(hw, col[-1])])
+    smooth = np.convolve(pad, np.full(2 * hw + 1, 1.0 / (2 * hw + 1)), mode="valid")
+    return np.broadcast_to(np.exp(smooth)[:, None, None], (n, n, n)).copy()
+
+
+@dataclass
+class GradientWorkload:
+    name: str
+    grid: Grid
+    sigma_true: np.ndarray
+    sigma_start: np.ndarray
+    it: int
+    sources: np.ndarray            # (nshots, 3) 0-based x-dipole cells
+    receivers: np.ndarray          # (nrec, 3)
+    freqs_hz: List[float]
+    signal: np.ndarray
+
+    @property
+    def omegas(self) -> np.ndarray:
+        return 2.0 * np.pi * np.asarray(self.freqs_hz)
+
+
+def gradient_128(seed: int = 4, it: int = 0, nshots: int = 8) -> GradientWorkload:
+    """128^3 interior + 12-cell CPML; true model = layered_box_sigma, start =
+    smoothed_background (same seed); 8 x-dipoles 2 cells below the top
+    interface at seeded positions; 64 receivers on an 8x8 grid on the
+    interface; L2 misfit at 4 frequencies (0.25-2 Hz); Nt = 30 x 128."""
+    n_int, nc = 128, 12
+    it = it or 30 * n_int
+    sig, k_top = layered_box_sigma(n_int, nc, seed)
+    sig0 = smoothed_background(n_int, nc, seed)
+    n = n_int + 2 * nc
+    dt, cmax = dt_for(50.0, 50.0, 50.0)
+    grid = Grid(n, n, n, nc, 50.0, 50.0, 50.0, dt, OMEGA0, cmax)
+    rng = np.random.default_rng(seed + 1000)
+    xy = rng.integers(nc + 8, n - nc - 8, size=(nshots, 2))
+    src = np.concatenate([xy, np.full((nshots, 1), k_top + 2)], axis=1).astype(np.int32)
+    g = np.linspace(nc + 4, n - nc - 5, 8).round().astype(int)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, k_top)], axis=1).astype(np.int32)
+    return GradientWorkload("gradient_128cube_fp64", grid, sig, sig0, it, src, rec,
+                            list(np.geomspace(0.25, 2.0, 4)), firstderiv_gauss(it, dt, FMAX_W))

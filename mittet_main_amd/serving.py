@@ -23,7 +23,7 @@ from __future__ import annotations
 
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
@@ -105,70 +105,129 @@ def shot_result(shot: Shot, req: Request) -> np.ndarray:
     return shot.receiver_dft(om, h_factor=True)
 
 
-class Server:
-    """Continuous-batching server over one persistent multi-shot batch."""
+def request_work(req: Request) -> int:
+    """Cell updates the request needs: cells (incl. CPML) x steps x shots."""
+    return (req.n_int + 2 * req.ncpml) ** 3 * (req.it - 1) * req.nshots
 
-    def __init__(self, budget_bytes: float = 200e9, chunk: int = 32, max_shots: int = 512):
+
+class Server:
+    """Continuous-batching server over one persistent multi-shot batch.
+
+    Scheduling: requests that have arrived wait in a shortest-work-first queue
+    (cell updates, ties in arrival order).  Shots are admitted one by one while
+    the cells in flight stay under `target_cells` (enough to fill every CU;
+    beyond that, more shots in flight only share the GPU and raise every
+    request's latency) and the HBM in use stays under `budget_bytes` (model
+    bytes while any of its shots is unfinished, plus each admitted shot's
+    bytes).  A request started earlier keeps priority for its remaining shots,
+    so requests complete one after another instead of all at the end.  The batch
+    advances `chunk` steps per round (one graph launch per half-step for every
+    member); finished shots leave, their memory is released, and waiting shots
+    join before the next round."""
+
+    def __init__(self, budget_bytes: float = 200e9, chunk: int = 32, max_shots: int = 512,
+                 target_cells: float = 24e6, policy: str = "swf"):
+        if policy not in ("swf", "fifo"):
+            raise ValueError("policy must be 'swf' (shortest work first) or 'fifo'")
         self.budget = float(budget_bytes)
         self.chunk = chunk
+        self.target_cells = float(target_cells)
+        self.policy = policy
         self.batch = Batch(max_shots, chunk=chunk)
         self.in_use = 0
         self.peak = 0
-        self.admissions = []                  # (rid, admit time)
+        self.admissions = []                  # (rid, time its first shot joined)
 
     def close(self):
         self.batch.close()
 
-    def run(self, requests: List[Request], realtime: bool = True) -> Dict:
-        pending = sorted(requests, key=lambda r: r.arrival)
-        active: Dict[int, dict] = {}
+    def _start(self, req: Request, now: float) -> dict:
+        grid, sig = build_model(req)
+        model = Model(grid)
+        model.media_coeff_3d(sig)
+        self.in_use += grid.ncells * MODEL_BYTES_PER_CELL
+        self.admissions.append((req.rid, now))
+        return {"req": req, "grid": grid, "model": model,
+                "signal": firstderiv_gauss(req.it, grid.dt, configs.FMAX_W),
+                "shot_bytes": shot_bytes(grid, req.it, nrec=len(req.receivers)),
+                "next": 0, "live": {}, "spectra": [None] * req.nshots, "left": req.nshots}
+
+    def run(self, requests: List[Request], realtime: bool = True,
+            on_done: Optional[Callable[[Result], None]] = None) -> Dict:
+        for req in requests:
+            grid, _ = build_model(req)
+            one = grid.ncells * MODEL_BYTES_PER_CELL + shot_bytes(grid, req.it, nrec=len(req.receivers))
+            if one > self.budget:
+                raise ValueError(f"request {req.rid} needs {one / 1e9:.1f} GB for one shot > budget")
+        incoming = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        ready: List[Request] = []
+        started: List[dict] = []              # in start order
         results: List[Result] = []
+        cells = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        while pending or active:
-            # admit everything that has arrived and fits under the budget (FIFO)
-            while pending and (not realtime or pending[0].arrival <= now()):
-                req = pending[0]
-                grid, sig = build_model(req)
-                need = request_bytes(req, grid)
-                if need > self.budget:
-                    raise ValueError(f"request {req.rid} needs {need / 1e9:.1f} GB > budget")
-                if self.in_use + need > self.budget:
-                    break                     # wait for memory to free up
-                pending.pop(0)
-                model = Model(grid)
-                model.media_coeff_3d(sig)
-                signal = firstderiv_gauss(req.it, grid.dt, configs.FMAX_W)
-                shots = []
-                for src in req.sources:
-                    sh = model.shot(req.it, signal, tuple(int(v) for v in src), 0, req.receivers)
-                    self.batch.add(sh)
-                    shots.append(sh)
-                self.in_use += need
+        while incoming or ready or started:
+            while incoming and (not realtime or incoming[0].arrival <= now()):
+                ready.append(incoming.pop(0))
+            if self.policy == "swf":
+                ready.sort(key=lambda r: (request_work(r), r.arrival, r.rid))
+            # admit shots: unfinished started requests first, then the queue head
+            while True:
+                nb = len(self.batch.members)
+                a = next((a for a in started if a["next"] < a["req"].nshots), None)
+                if a is None:
+                    if not ready:
+                        break
+                    req = ready[0]
+                    grid, _ = build_model(req)
+                    need = grid.ncells * MODEL_BYTES_PER_CELL + \
+                        shot_bytes(grid, req.it, nrec=len(req.receivers))
+                    if nb and (cells >= self.target_cells or self.in_use + need > self.budget):
+                        break
+                    ready.pop(0)
+                    a = self._start(req, now())
+                    started.append(a)
+                gcells = a["grid"].ncells
+                if nb and (cells + gcells > self.target_cells or
+                           self.in_use + a["shot_bytes"] > self.budget):
+                    break
+                if self.in_use + a["shot_bytes"] > self.budget:
+                    break                     # only the model fits now; wait for memory
+                q = a["next"]
+                sh = a["model"].shot(a["req"].it, a["signal"], tuple(int(v) for v in a["req"].sources[q]),
+                                     0, a["req"].receivers)
+                self.batch.add(sh)
+                a["live"][q] = sh
+                a["next"] += 1
+                cells += gcells
+                self.in_use += a["shot_bytes"]
                 self.peak = max(self.peak, self.in_use)
-                self.admissions.append((req.rid, now()))
-                active[req.rid] = {"req": req, "model": model, "shots": shots, "left": len(shots),
-                                   "spectra": [None] * len(shots), "bytes": need}
-            if not active:
-                if pending:                   # idle until the next arrival
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
-                continue
+            if not self.batch.members:
+                if incoming:                  # idle until the next arrival
+                    time.sleep(max(0.0, min(incoming[0].arrival - now(), 0.01)))
+                    continue
+                if not ready and not started:
+                    break
+                raise RuntimeError("no shot fits the HBM budget while the batch is empty")
             self.batch.step(self.chunk)
             self.batch.sync()
-            for rid in list(active):
-                a = active[rid]
-                for q, sh in enumerate(a["shots"]):
-                    if sh is not None and sh.step >= sh.it - 1:
+            for a in list(started):
+                for q, sh in list(a["live"].items()):
+                    if sh.step >= sh.it - 1:
                         self.batch.remove(sh)
                         a["spectra"][q] = shot_result(sh, a["req"])
                         sh.close()
-                        a["shots"][q] = None
+                        del a["live"][q]
                         a["left"] -= 1
+                        cells -= a["grid"].ncells
+                        self.in_use -= a["shot_bytes"]
                 if a["left"] == 0:
                     a["model"].close()
-                    self.in_use -= a["bytes"]
-                    results.append(Result(rid, a["spectra"], a["req"].arrival, now()))
-                    del active[rid]
+                    self.in_use -= a["grid"].ncells * MODEL_BYTES_PER_CELL
+                    results.append(Result(a["req"].rid, a["spectra"], a["req"].arrival, now()))
+                    started.remove(a)
+                    if on_done:
+                        on_done(results[-1])
         makespan = now()
         return summarize(results, requests, makespan, peak=self.peak)
 
@@ -179,16 +238,18 @@ def summarize(results: List[Result], requests: List[Request], makespan: float, p
     cells = sum((r.n_int + 2 * r.ncpml) ** 3 * (r.it - 1) * r.nshots for r in requests)
     return {"requests": len(requests), "shots": shots, "makespan_s": makespan,
             "shots_per_s": shots / makespan, "gcell_updates_per_s": cells / makespan / 1e9,
-            "p50_s": float(np.percentile(lat, 50)), "p95_s": float(np.percentile(lat, 95)),
+            "mean_s": float(lat.mean()), "p50_s": float(np.percentile(lat, 50)), "p95_s": float(np.percentile(lat, 95)),
             "p99_s": float(np.percentile(lat, 99)), "peak_hbm_gb": peak / 1e9,
             "results": {r.rid: r for r in results}}
 
 
-def run_sequential(requests: List[Request], mode: str = "fused", realtime: bool = True) -> Dict:
+def run_sequential(requests: List[Request], mode: str = "fused", realtime: bool = True,
+                   on_done: Optional[Callable[[Result], None]] = None) -> Dict:
     """The plain path: one shot at a time, in arrival order.  mode 'fused' runs
     each shot's graph-captured time loop; mode 'ops' calls the reference-named
     per-step operators (six launches per step) as the reference loop does."""
     results = []
+    peak = 0
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0
     for req in sorted(requests, key=lambda r: r.arrival):
@@ -198,6 +259,8 @@ def run_sequential(requests: List[Request], mode: str = "fused", realtime: bool 
         model = Model(grid)
         model.media_coeff_3d(sig)
         signal = firstderiv_gauss(req.it, grid.dt, configs.FMAX_W)
+        peak = max(peak, grid.ncells * MODEL_BYTES_PER_CELL +
+                   shot_bytes(grid, req.it, nrec=len(req.receivers)))
         spectra = []
         for src in req.sources:
             sh = model.shot(req.it, signal, tuple(int(v) for v in src), 0, req.receivers)
@@ -215,4 +278,6 @@ def run_sequential(requests: List[Request], mode: str = "fused", realtime: bool 
             sh.close()
         model.close()
         results.append(Result(req.rid, spectra, req.arrival, now()))
-    return summarize(results, requests, now())
+        if on_done:
+            on_done(results[-1])
+    return summarize(results, requests, now(), peak=peak)

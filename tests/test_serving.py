@@ -40,13 +40,22 @@ def test_request_bytes_counts_model_and_every_shot():
     assert sig.shape == grid.shape and sig.min() > 0
 
 
+def test_request_work_orders_queue():
+    reqs = small_stream(8)
+    w = [serving.request_work(r) for r in reqs]
+    assert all(x > 0 for x in w)
+    r = reqs[0]
+    assert w[0] == (r.n_int + 2 * r.ncpml) ** 3 * (r.it - 1) * r.nshots
+
+
 @pytest.mark.gpu
-def test_served_mix_equals_alone():
+@pytest.mark.parametrize("policy,target_cells", [("swf", 24e6), ("swf", 1.0), ("fifo", 60e3)])
+def test_served_mix_equals_alone(policy, target_cells):
     reqs = small_stream()
-    grid0, _ = serving.build_model(reqs[0])
-    # budget for roughly two requests at a time: forces queueing and joins mid-run
+    # budget for roughly two requests at a time: forces queueing and joins mid-run;
+    # target_cells 1 admits one shot at a time, 60e3 about two small grids
     budget = 2.2 * max(serving.request_bytes(r, serving.build_model(r)[0]) for r in reqs)
-    srv = serving.Server(budget_bytes=budget, chunk=8)
+    srv = serving.Server(budget_bytes=budget, chunk=8, target_cells=target_cells, policy=policy)
     served = srv.run(reqs, realtime=False)
     srv.close()
     alone = serving.run_sequential(reqs, mode="fused", realtime=False)
