@@ -23,11 +23,6 @@ namespace mt {
 
 namespace {
 
-struct Dims {
-    int ix, iy, iz, nc;
-    int n[3];
-};
-
 __device__ __forceinline__ int slabq(int a, const AdjArgs &A, int i, int j, int k)
 {
     const int nc = A.nc, ix = A.ix, iy = A.iy, iz = A.iz;
@@ -100,16 +95,63 @@ __device__ __forceinline__ double prof(const AdjArgs &A, int a, int row, int idx
     return ((const GLOBAL double *)A.prof[a])[row * n + idx];
 }
 
+// 1/ked (h = false) or 1/khd (h = true) along axis a at index idx
+__device__ __forceinline__ double rprof(const AdjArgs &A, int a, bool h, int idx)
+{
+    const int n = a == 0 ? A.ix : (a == 1 ? A.iy : A.iz);
+    return ((const GLOBAL double *)A.rp[a])[(h ? n : 0) + idx];
+}
+
+// 32 x 8 tiles of one plane per block, numbered so that each XCD owns a
+// contiguous run of tiles (the numbering of the fused forward kernels)
 __device__ __forceinline__ bool cell_of(const AdjArgs &A, int &i, int &j, int &k, int &c)
 {
-    c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= A.n)
+    const int b = blockIdx.x;
+    const int L = (b & 7) * (A.fb_pad >> 3) + (b >> 3);
+    if (L >= A.fb)
         return false;
-    i = c % A.ix;
-    const int r = c / A.ix;
-    j = r % A.iy;
-    k = r / A.iy;
+    const int tx = L % A.ftx, r = L / A.ftx;
+    const int ty = r % A.fty;
+    k = r / A.fty;
+    i = tx * ZX + (threadIdx.x & (ZX - 1));
+    j = ty * ZY + threadIdx.x / ZX;
+    if (i >= A.ix || j >= A.iy)
+        return false;
+    c = (k * A.iy + j) * A.ix + i;
     return true;
+}
+
+// Cell-local CPML-free interior [nc, n - nc): ca = da = 1, cb = dt/eps',
+// db = dt/mu0, ked = khd = dl, no psi face; only the curl terms and the
+// cb-derivative of the E update remain.
+__device__ __forceinline__ bool inner(const AdjArgs &A, int i, int j, int k)
+{
+    return i >= A.nc && i < A.inner_hi[0] && j >= A.nc && j < A.inner_hi[1] && k >= A.nc &&
+           k < A.inner_hi[2];
+}
+
+// the transposed curl stencils (offsets -2..+2) stay inside the grid
+__device__ __forceinline__ bool core(const AdjArgs &A, int i, int j, int k)
+{
+    return i >= 2 && i < A.ix - 2 && j >= 2 && j < A.iy - 2 && k >= 2 && k < A.iz - 2;
+}
+
+// along axis a, the transposed psi stencils of this cell (offsets -2..+1 for
+// psi-H, -1..+2 for psi-E) reach a CPML face cell
+__device__ __forceinline__ bool near_face(const AdjArgs &A, int a, int r)
+{
+    const int n = a == 0 ? A.ix : (a == 1 ? A.iy : A.iz);
+    return r < A.nc + 1 || r >= n - A.nc - 1;
+}
+
+__device__ __forceinline__ double dminus_at(const GLOBAL double *w, int c, int s, double c1, double c2)
+{
+    return c1 * w[c] - c1 * w[c - s] + c2 * w[c + s] - c2 * w[c - 2 * s];
+}
+
+__device__ __forceinline__ double dplus_at(const GLOBAL double *w, int c, int s, double c1, double c2)
+{
+    return c1 * w[c + s] - c1 * w[c] + c2 * w[c + 2 * s] - c2 * w[c - s];
 }
 
 }  // namespace
@@ -127,16 +169,22 @@ __global__ void k_adj_inject(AdjArgs A, const double *__restrict__ g, int nrec,
 
 // H-phase part 1 (cell-local): weights for the transposed curls, psiH adjoint,
 // H-phase gradient, lH <- da.lH.
-__global__ __launch_bounds__(256) void k_adj_h1(AdjArgs A)
+__global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
 {
     int i, j, k, c;
     if (!cell_of(A, i, j, k, c))
         return;
     const int N = A.n, sy = A.ix, sz = A.ix * A.iy;
+    const double dt = A.dt, c1 = A.c1, c2 = A.c2;
+    const double dbi = dt / A.mu0;
+    if (__all(inner(A, i, j, k))) {
+        for (int d = 0; d < 3; ++d)
+            A.t3[(size_t)d * N + c] = dbi * A.lH[(size_t)d * N + c];
+        return;
+    }
     const int idx[3] = {i, j, k};
     const long long st[3] = {1, sy, sz};
-    const double e = A.eps[c], dt = A.dt, c1 = A.c1, c2 = A.c2;
-    const double dbi = dt / A.mu0;
+    const double e = A.eps[c];
     double lh[3], ms[3], km[3], db[3];
     for (int d = 0; d < 3; ++d) {
         lh[d] = A.lH[(size_t)d * N + c];
@@ -158,7 +206,7 @@ __global__ __launch_bounds__(256) void k_adj_h1(AdjArgs A)
             const double db_de = dbi * km[d] / ((e + km[d]) * (e + km[d]));
             const double dpa = c1 * Eb[c + st[a]] - c1 * Eb[c] + c2 * Eb[c + 2 * st[a]] - c2 * Eb[c - st[a]];
             const double dpb = c1 * Ea[c + st[b]] - c1 * Ea[c] + c2 * Ea[c + 2 * st[b]] - c2 * Ea[c - st[b]];
-            const double cu = dpa / prof(A, a, PR_KHD, idx[a]) - dpb / prof(A, b, PR_KHD, idx[b]);
+            const double cu = dpa * rprof(A, a, true, idx[a]) - dpb * rprof(A, b, true, idx[b]);
             g += lh[d] * (da_de * H0[(size_t)d * N + c] - db_de * cu);
         }
     }
@@ -192,13 +240,13 @@ __global__ __launch_bounds__(256) void k_adj_h1(AdjArgs A)
 
 // weighted value at a neighbour for a transposed stencil (0 outside the grid)
 __device__ __forceinline__ double wnb(const AdjArgs &A, const double *T, int comp, int ni, int nj,
-                                      int nk, int axis, int row)
+                                      int nk, int axis, bool h)
 {
     if (ni < 0 || nj < 0 || nk < 0 || ni >= A.ix || nj >= A.iy || nk >= A.iz)
         return 0.0;
     const int c = (nk * A.iy + nj) * A.ix + ni;
     const int id[3] = {ni, nj, nk};
-    return T[(size_t)comp * A.n + c] / prof(A, axis, row, id[axis]);
+    return T[(size_t)comp * A.n + c] * rprof(A, axis, h, id[axis]);
 }
 
 __device__ __forceinline__ double wpsi(const AdjArgs &A, int a, int arr, int ni, int nj, int nk,
@@ -213,22 +261,15 @@ __device__ __forceinline__ double wpsi(const AdjArgs &A, int a, int arr, int ni,
 
 // H-phase part 2 (gathers): lE += transposed curl+ of the weights and the
 // transposed CPML-H stencils; then the source gradient.
-__global__ __launch_bounds__(256) void k_adj_h2(AdjArgs A)
+__global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
 {
     int i, j, k, c;
     if (!cell_of(A, i, j, k, c))
         return;
     const double c1 = A.c1, c2 = A.c2;
     const int N = A.n;
+    double acc[3] = {0.0, 0.0, 0.0};
     // D-_a w (c) = c1 w(c) - c1 w(c-s) + c2 w(c+s) - c2 w(c-2s)
-    auto dminus = [&](int comp, int axis, int row) {
-        int o[3] = {0, 0, 0};
-        auto at = [&](int off) {
-            o[axis] = off;
-            return wnb(A, A.t3, comp, i + o[0], j + o[1], k + o[2], axis, row);
-        };
-        return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
-    };
     auto dminus_psi = [&](int a, int arr) {
         int o[3] = {0, 0, 0};
         auto at = [&](int off) {
@@ -237,14 +278,35 @@ __global__ __launch_bounds__(256) void k_adj_h2(AdjArgs A)
         };
         return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
     };
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < 3; ++d) {
-        const int a = (d + 1) % 3, b = (d + 2) % 3;
-        // H_d -= db (D+_a E_b/khd_a - D+_b E_a/khd_b):  lE_b += D-_a(k/khd_a), lE_a -= D-_b(k/khd_b)
-        acc[b] += dminus(d, a, PR_KHD);
-        acc[a] -= dminus(d, b, PR_KHD);
+    if (A.kappa_one && __all(core(A, i, j, k))) {
+        const int st[3] = {1, A.ix, A.ix * A.iy};
+        const GLOBAL double *T = (const GLOBAL double *)A.t3;
+        for (int d = 0; d < 3; ++d) {
+            const int a = (d + 1) % 3, b = (d + 2) % 3;
+            const GLOBAL double *Td = T + (size_t)d * N;
+            acc[b] += dminus_at(Td, c, st[a], c1, c2) * A.rdl[a];
+            acc[a] -= dminus_at(Td, c, st[b], c1, c2) * A.rdl[b];
+        }
+    } else {
+        auto dminus = [&](int comp, int axis) {
+            int o[3] = {0, 0, 0};
+            auto at = [&](int off) {
+                o[axis] = off;
+                return wnb(A, A.t3, comp, i + o[0], j + o[1], k + o[2], axis, true);
+            };
+            return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
+        };
+        for (int d = 0; d < 3; ++d) {
+            const int a = (d + 1) % 3, b = (d + 2) % 3;
+            // H_d -= db (D+_a E_b/khd_a - D+_b E_a/khd_b):  lE_b += D-_a(k/khd_a), lE_a -= D-_b(k/khd_b)
+            acc[b] += dminus(d, a);
+            acc[a] -= dminus(d, b);
+        }
     }
+    const int idx[3] = {i, j, k};
     for (int a = 0; a < 3; ++a) {
+        if (!__any(near_face(A, a, idx[a])))
+            continue;
         const int p = (a + 1) % 3, q = (a + 2) % 3;
         // psiH0 <- D+_a E_q, psiH1 <- D+_a E_p; transpose of D+ is -D-
         acc[q] -= dminus_psi(a, 2);
@@ -259,15 +321,30 @@ __global__ __launch_bounds__(256) void k_adj_h2(AdjArgs A)
 }
 
 // E-phase part 1 (cell-local): weights, psiE adjoint, E-phase gradient, lE <- ca.lE.
-__global__ __launch_bounds__(256) void k_adj_e1(AdjArgs A)
+__global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
 {
     int i, j, k, c;
     if (!cell_of(A, i, j, k, c))
         return;
     const int N = A.n, sy = A.ix, sz = A.ix * A.iy;
-    const int idx[3] = {i, j, k};
     const long long st[3] = {1, sy, sz};
     const double e = A.eps[c], dt = A.dt, c1 = A.c1, c2 = A.c2;
+    const GLOBAL double *H0 = (const GLOBAL double *)A.uH0;
+    if (__all(inner(A, i, j, k))) {
+        const double cb = dt / e, cbd = -dt / (e * e);
+        double g = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const int a = (d + 1) % 3, b = (d + 2) % 3;
+            const double le = A.lE[(size_t)d * N + c];
+            A.t3[(size_t)d * N + c] = cb * le;
+            const double cu = dminus_at(H0 + (size_t)b * N, c, (int)st[a], c1, c2) * A.rdl[a] -
+                              dminus_at(H0 + (size_t)a * N, c, (int)st[b], c1, c2) * A.rdl[b];
+            g += le * cbd * cu;
+        }
+        A.grad[c] += g;
+        return;
+    }
+    const int idx[3] = {i, j, k};
     double le[3], ke[3], cb[3], cbd[3];
     for (int d = 0; d < 3; ++d) {
         le[d] = A.lE[(size_t)d * N + c];
@@ -277,7 +354,6 @@ __global__ __launch_bounds__(256) void k_adj_e1(AdjArgs A)
     }
     double g = 0.0;
     const GLOBAL double *E0 = (const GLOBAL double *)A.uE0;
-    const GLOBAL double *H0 = (const GLOBAL double *)A.uH0;
     for (int d = 0; d < 3; ++d) {
         const int a = (d + 1) % 3, b = (d + 2) % 3;
         const bool up = in_e(A, d, i, j, k);
@@ -286,7 +362,7 @@ __global__ __launch_bounds__(256) void k_adj_e1(AdjArgs A)
             const GLOBAL double *Hb = H0 + (size_t)b * N, *Ha = H0 + (size_t)a * N;
             const double dma = c1 * Hb[c] - c1 * Hb[c - st[a]] + c2 * Hb[c + st[a]] - c2 * Hb[c - 2 * st[a]];
             const double dmb = c1 * Ha[c] - c1 * Ha[c - st[b]] + c2 * Ha[c + st[b]] - c2 * Ha[c - 2 * st[b]];
-            const double cu = dma / prof(A, a, PR_KED, idx[a]) - dmb / prof(A, b, PR_KED, idx[b]);
+            const double cu = dma * rprof(A, a, false, idx[a]) - dmb * rprof(A, b, false, idx[b]);
             const double ca_de = 2.0 * ke[d] / ((e + ke[d]) * (e + ke[d]));
             g += le[d] * (ca_de * E0[(size_t)d * N + c] + cbd[d] * cu);
         }
@@ -317,22 +393,15 @@ __global__ __launch_bounds__(256) void k_adj_e1(AdjArgs A)
 
 // E-phase part 2 (gathers): lH += transposed curl- of the weights and the
 // transposed CPML-E stencils.
-__global__ __launch_bounds__(256) void k_adj_e2(AdjArgs A)
+__global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
 {
     int i, j, k, c;
     if (!cell_of(A, i, j, k, c))
         return;
     const double c1 = A.c1, c2 = A.c2;
     const int N = A.n;
+    double acc[3] = {0.0, 0.0, 0.0};
     // D+_a w (c) = c1 w(c+s) - c1 w(c) + c2 w(c+2s) - c2 w(c-s)
-    auto dplus = [&](int comp, int axis, int row) {
-        int o[3] = {0, 0, 0};
-        auto at = [&](int off) {
-            o[axis] = off;
-            return wnb(A, A.t3, comp, i + o[0], j + o[1], k + o[2], axis, row);
-        };
-        return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
-    };
     auto dplus_psi = [&](int a, int arr) {
         int o[3] = {0, 0, 0};
         auto at = [&](int off) {
@@ -341,14 +410,35 @@ __global__ __launch_bounds__(256) void k_adj_e2(AdjArgs A)
         };
         return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
     };
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < 3; ++d) {
-        const int a = (d + 1) % 3, b = (d + 2) % 3;
-        // E_d += cb (D-_a H_b/ked_a - D-_b H_a/ked_b): lH_b -= D+_a(nu/ked_a), lH_a += D+_b(nu/ked_b)
-        acc[b] -= dplus(d, a, PR_KED);
-        acc[a] += dplus(d, b, PR_KED);
+    if (A.kappa_one && __all(core(A, i, j, k))) {
+        const int st[3] = {1, A.ix, A.ix * A.iy};
+        const GLOBAL double *T = (const GLOBAL double *)A.t3;
+        for (int d = 0; d < 3; ++d) {
+            const int a = (d + 1) % 3, b = (d + 2) % 3;
+            const GLOBAL double *Td = T + (size_t)d * N;
+            acc[b] -= dplus_at(Td, c, st[a], c1, c2) * A.rdl[a];
+            acc[a] += dplus_at(Td, c, st[b], c1, c2) * A.rdl[b];
+        }
+    } else {
+        auto dplus = [&](int comp, int axis) {
+            int o[3] = {0, 0, 0};
+            auto at = [&](int off) {
+                o[axis] = off;
+                return wnb(A, A.t3, comp, i + o[0], j + o[1], k + o[2], axis, false);
+            };
+            return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
+        };
+        for (int d = 0; d < 3; ++d) {
+            const int a = (d + 1) % 3, b = (d + 2) % 3;
+            // E_d += cb (D-_a H_b/ked_a - D-_b H_a/ked_b): lH_b -= D+_a(nu/ked_a), lH_a += D+_b(nu/ked_b)
+            acc[b] -= dplus(d, a);
+            acc[a] += dplus(d, b);
+        }
     }
+    const int idx[3] = {i, j, k};
     for (int a = 0; a < 3; ++a) {
+        if (!__any(near_face(A, a, idx[a])))
+            continue;
         const int p = (a + 1) % 3, q = (a + 2) % 3;
         // psiE0 <- D-_a H_p, psiE1 <- D-_a H_q; transpose of D- is -D+
         acc[p] -= dplus_psi(a, 0);
@@ -361,15 +451,14 @@ __global__ __launch_bounds__(256) void k_adj_e2(AdjArgs A)
 hipError_t launch_adjoint_step(const AdjArgs &A, const double *g, int nrec, const int *rec,
                                hipStream_t st)
 {
-    const int blocks = (A.n + 255) / 256;
     if (nrec > 0) {
         hipLaunchKernelGGL(k_adj_inject, dim3((3 * nrec + 255) / 256), dim3(256), 0, st, A, g,
                            nrec, rec);
     }
-    hipLaunchKernelGGL(k_adj_h1, dim3(blocks), dim3(256), 0, st, A);
-    hipLaunchKernelGGL(k_adj_h2, dim3(blocks), dim3(256), 0, st, A);
-    hipLaunchKernelGGL(k_adj_e1, dim3(blocks), dim3(256), 0, st, A);
-    hipLaunchKernelGGL(k_adj_e2, dim3(blocks), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_adj_h1, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
+    hipLaunchKernelGGL(k_adj_h2, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
+    hipLaunchKernelGGL(k_adj_e1, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
+    hipLaunchKernelGGL(k_adj_e2, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
     return hipGetLastError();
 }
 

@@ -1071,6 +1071,8 @@ struct fwi3d_grad {
     double *grad = nullptr;           // N
     double *grad_src = nullptr;       // 1
     double *g = nullptr;              // [it][3][nrec] trace sensitivities
+    double *rp = nullptr;             // per axis: 1/ked, 1/khd
+    int kappa_one = 0;                // ked = khd = dl on every axis
 };
 
 static void state_layout(fwi3d_grad *g)
@@ -1137,10 +1139,28 @@ int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out)
         dalloc(&g->buf, g->state * (g->seg + 1)) != hipSuccess ||
         dalloc(&g->adj, g->state) != hipSuccess || dalloc(&g->tmp, tmp) != hipSuccess ||
         dalloc(&g->grad, N) != hipSuccess || dalloc(&g->grad_src, 1) != hipSuccess ||
-        dalloc(&g->g, std::max<size_t>(1, 3 * (size_t)s->nrec * s->it)) != hipSuccess) {
+        dalloc(&g->g, std::max<size_t>(1, 3 * (size_t)s->nrec * s->it)) != hipSuccess ||
+        dalloc(&g->rp, 2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz)) != hipSuccess) {
         fwi3d_grad_destroy(g);
         return fail(FWI3D_ENOMEM, "gradient storage (%zu states of %zu doubles)",
                     (size_t)g->nseg + g->seg + 2, g->state);
+    }
+    // reciprocal ked / khd rows: the transposed stencils multiply where the
+    // forward divides
+    std::vector<double> rp;
+    g->kappa_one = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int n = a == 0 ? s->desc.ix : (a == 1 ? s->desc.iy : s->desc.iz);
+        for (int row : {FWI3D_P_KED, FWI3D_P_KHD})
+            for (int q = 0; q < n; ++q) {
+                const double v = s->m->prof[a][(size_t)row * n + q];
+                rp.push_back(1.0 / v);
+                g->kappa_one &= v == s->desc.dl[a];
+            }
+    }
+    if (hipMemcpy(g->rp, rp.data(), sizeof(double) * rp.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        fwi3d_grad_destroy(g);
+        return fail(FWI3D_EDEVICE, "profile upload failed");
     }
     *out = g;
     return FWI3D_OK;
@@ -1159,6 +1179,7 @@ int fwi3d_grad_destroy(fwi3d_grad *g)
     dfree(g->grad);
     dfree(g->grad_src);
     dfree(g->g);
+    dfree(g->rp);
     delete g;
     return FWI3D_OK;
 }
@@ -1226,8 +1247,24 @@ int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
     A.mu0 = d.mu0;
     A.c1 = d.c1;
     A.c2 = d.c2;
+    A.ftx = tiles(d.ix, mt::ZX);
+    A.fty = tiles(d.iy, mt::ZY);
+    A.fb = A.ftx * A.fty * d.iz;
+    A.fb_pad = tiles(A.fb, 8) * 8;
+    // [nc, n - nc) is free of CPML terms when nc >= 2 (the update ranges start
+    // at index 1 or 2); with a thinner CPML no cell takes the fast paths
     for (int a = 0; a < 3; ++a) {
+        const int n = a == 0 ? d.ix : (a == 1 ? d.iy : d.iz);
+        A.inner_hi[a] = d.nc >= 2 ? n - d.nc : 0;
+    }
+    A.kappa_one = g->kappa_one;
+    const double *rp = g->rp;
+    for (int a = 0; a < 3; ++a) {
+        const int n = a == 0 ? d.ix : (a == 1 ? d.iy : d.iz);
         A.dl[a] = d.dl[a];
+        A.rdl[a] = 1.0 / d.dl[a];
+        A.rp[a] = rp;
+        rp += 2 * n;
         A.prof[a] = d.prof[a];
         A.slab[a] = d.slab[a];
         A.lpsi[a] = g->adj + g->off_psi[a];

@@ -61,9 +61,13 @@ enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
 // Arguments of one adjoint step (fdtd_adjoint.hip), passed by value.
 struct AdjArgs {
     int ix, iy, iz, nc, n;
+    int ftx, fty, fb, fb_pad;        // 32 x 8 tiles per plane, as the fused forward kernels
+    int inner_hi[3];                 // cells in [nc, inner_hi[a]) on every axis see no CPML term
+    int kappa_one;                   // ked = khd = dl everywhere
     double dt, mu0, c1, c2;
-    double dl[3];
+    double dl[3], rdl[3];
     const double *prof[3];
+    const double *rp[3];             // per axis: 1/ked (row 0) and 1/khd (row 1)
     const double *eps;
     long long slab[3];
     double *lE, *lH, *lpsi[3];       // adjoint state (forward layout)

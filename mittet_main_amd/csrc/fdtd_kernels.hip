@@ -29,9 +29,7 @@
 #ifndef MT_FLUSH_G
 #define MT_FLUSH_G 8
 #endif
-#ifndef MT_TILE_ORDER
-#define MT_TILE_ORDER 0
-#endif
+
 
 // Pointers read out of the shot descriptor are generic; casting them to the
 // global address space lets the compiler emit global_load/store instead of
@@ -956,15 +954,8 @@ __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, 
     if (L >= S.fb)
         return false;
     const int tx = L % S.ftx, r = L / S.ftx;
-#if MT_TILE_ORDER == 1
-    // x fastest, then z, then y: an XCD sweeps an 8-row band through every
-    // plane, so the z-neighbour planes it re-reads are still in its L2
-    k = r % S.iz;
-    const int ty = r / S.iz;
-#else
     const int ty = r % S.fty;
     k = r / S.fty;
-#endif
     i = tx * ZX + (threadIdx.x & (ZX - 1));
     jj = ty * ZY + threadIdx.x / ZX;
     return i < S.ix && jj < S.iy;
