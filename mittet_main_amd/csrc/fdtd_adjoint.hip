@@ -185,12 +185,15 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
     const int idx[3] = {i, j, k};
     const long long st[3] = {1, sy, sz};
     const double e = A.eps[c];
-    double lh[3], ms[3], km[3], db[3];
+    // one reciprocal per axis: r = 1/(e + Km), Km = msig dt/2; then
+    // db = dt/mu0 e r, da = (e - Km) r, dda/de = 2 Km r^2, ddb/de = dt/mu0 Km r^2
+    double lh[3], ms[3], km[3], rr[3], db[3];
     for (int d = 0; d < 3; ++d) {
         lh[d] = A.lH[(size_t)d * N + c];
         ms[d] = prof(A, d, PR_MSIG, idx[d]);
         km[d] = ms[d] * dt / 2.0;
-        db[d] = dbi * e / (e + km[d]);   // = dt/mu0/(1 + msig dt/(2e))
+        rr[d] = 1.0 / (e + km[d]);
+        db[d] = dbi * e * rr[d];
     }
     double g = 0.0;
     const GLOBAL double *E1 = (const GLOBAL double *)A.uE1;
@@ -202,8 +205,9 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
         if (up && ms[d] != 0.0) {
             // gradient: lH_d (dda/de H_d^n - ddb/de curl+_d(E^{n+1}))
             const GLOBAL double *Eb = E1 + (size_t)b * N, *Ea = E1 + (size_t)a * N;
-            const double da_de = 2.0 * km[d] / ((e + km[d]) * (e + km[d]));
-            const double db_de = dbi * km[d] / ((e + km[d]) * (e + km[d]));
+            const double r2 = rr[d] * rr[d];
+            const double da_de = 2.0 * km[d] * r2;
+            const double db_de = dbi * km[d] * r2;
             const double dpa = c1 * Eb[c + st[a]] - c1 * Eb[c] + c2 * Eb[c + 2 * st[a]] - c2 * Eb[c - st[a]];
             const double dpb = c1 * Ea[c + st[b]] - c1 * Ea[c] + c2 * Ea[c + 2 * st[b]] - c2 * Ea[c - st[b]];
             const double cu = dpa * rprof(A, a, true, idx[a]) - dpb * rprof(A, b, true, idx[b]);
@@ -223,19 +227,20 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
         const double m1 = P[3 * L + s] - db[q] * lh[q];
         P[2 * L + s] = bh * m0;
         P[3 * L + s] = bh * m1;
-        T[2 * L + s] = ch * m0 / A.dl[a];
-        T[3 * L + s] = ch * m1 / A.dl[a];
+        T[2 * L + s] = ch * m0 * A.rdl[a];
+        T[3 * L + s] = ch * m1 * A.rdl[a];
         // gradient through db in H_p += db_p psi0', H_q -= db_q psi1'
         const GLOBAL double *Q1 = (const GLOBAL double *)A.upsi1[a];
-        const double dbp = dbi * km[p] / ((e + km[p]) * (e + km[p]));
-        const double dbq = dbi * km[q] / ((e + km[q]) * (e + km[q]));
+        const double dbp = dbi * km[p] * rr[p] * rr[p];
+        const double dbq = dbi * km[q] * rr[q] * rr[q];
         g += lh[p] * dbp * Q1[2 * L + s] - lh[q] * dbq * Q1[3 * L + s];
     }
     for (int d = 0; d < 3; ++d) {
         if (in_h(A, d, i, j, k) && ms[d] != 0.0)
-            A.lH[(size_t)d * N + c] = lh[d] * (e - km[d]) / (e + km[d]);
+            A.lH[(size_t)d * N + c] = lh[d] * (e - km[d]) * rr[d];
     }
-    A.grad[c] += g;
+    if (g != 0.0)
+        A.grad[c] += g;
 }
 
 // weighted value at a neighbour for a transposed stencil (0 outside the grid)
@@ -331,7 +336,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
     const double e = A.eps[c], dt = A.dt, c1 = A.c1, c2 = A.c2;
     const GLOBAL double *H0 = (const GLOBAL double *)A.uH0;
     if (__all(inner(A, i, j, k))) {
-        const double cb = dt / e, cbd = -dt / (e * e);
+        const double r = 1.0 / e, cb = dt * r, cbd = -dt * r * r;
         double g = 0.0;
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
@@ -345,12 +350,15 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
         return;
     }
     const int idx[3] = {i, j, k};
-    double le[3], ke[3], cb[3], cbd[3];
+    // one reciprocal per axis: r = 1/(e + K), K = esig dt/2; then cb = dt r,
+    // ca = (e - K) r, dca/de = 2 K r^2, dcb/de = -dt r^2
+    double le[3], ke[3], rr[3], cb[3], cbd[3];
     for (int d = 0; d < 3; ++d) {
         le[d] = A.lE[(size_t)d * N + c];
         ke[d] = prof(A, d, PR_ESIG, idx[d]) * dt / 2.0;
-        cb[d] = dt / (e + ke[d]);
-        cbd[d] = -dt / ((e + ke[d]) * (e + ke[d]));
+        rr[d] = 1.0 / (e + ke[d]);
+        cb[d] = dt * rr[d];
+        cbd[d] = -dt * rr[d] * rr[d];
     }
     double g = 0.0;
     const GLOBAL double *E0 = (const GLOBAL double *)A.uE0;
@@ -363,8 +371,10 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
             const double dma = c1 * Hb[c] - c1 * Hb[c - st[a]] + c2 * Hb[c + st[a]] - c2 * Hb[c - 2 * st[a]];
             const double dmb = c1 * Ha[c] - c1 * Ha[c - st[b]] + c2 * Ha[c + st[b]] - c2 * Ha[c - 2 * st[b]];
             const double cu = dma * rprof(A, a, false, idx[a]) - dmb * rprof(A, b, false, idx[b]);
-            const double ca_de = 2.0 * ke[d] / ((e + ke[d]) * (e + ke[d]));
-            g += le[d] * (ca_de * E0[(size_t)d * N + c] + cbd[d] * cu);
+            double gd = cbd[d] * cu;
+            if (ke[d] != 0.0)
+                gd += 2.0 * ke[d] * rr[d] * rr[d] * E0[(size_t)d * N + c];
+            g += le[d] * gd;
         }
     }
     for (int a = 0; a < 3; ++a) {
@@ -380,14 +390,14 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
         const double m1 = P[L + s] - cb[p] * le[p];
         P[s] = be * m0;
         P[L + s] = be * m1;
-        T[s] = ce * m0 / A.dl[a];
-        T[L + s] = ce * m1 / A.dl[a];
+        T[s] = ce * m0 * A.rdl[a];
+        T[L + s] = ce * m1 * A.rdl[a];
         const GLOBAL double *Q1 = (const GLOBAL double *)A.upsi1[a];
         g += le[q] * cbd[q] * Q1[s] - le[p] * cbd[p] * Q1[L + s];
     }
     for (int d = 0; d < 3; ++d)
         if (in_e(A, d, i, j, k) && ke[d] != 0.0)
-            A.lE[(size_t)d * N + c] = le[d] * (e - ke[d]) / (e + ke[d]);
+            A.lE[(size_t)d * N + c] = le[d] * (e - ke[d]) * rr[d];
     A.grad[c] += g;
 }
 
