@@ -199,6 +199,53 @@ long long fwi3d_grad_bytes(fwi3d_grad *g);
 int fwi3d_grad_forward(fwi3d_grad *g);
 int fwi3d_grad_backward(fwi3d_grad *g, const double *dj_dtrace, double *dj_dsigma);
 
+/* ---- the reference's FWI iteration (compat mode) --------------------------- */
+
+/* One iteration of satan_i/fwi3d_cpu_inv_fwi_v1.4.c:27-109 on the GPU.  The
+ * frequency bins are the reference's w in [start_f, upper_f), omega_w =
+ * w * 2 pi/(it dt) (fwi3d_compat_omegas); every shot passed in must run on the
+ * object's model with a volume DFT at exactly those omegas.  Complex arrays are
+ * interleaved (re, im); receiver arrays are [3][nrec][upper_f], zero below
+ * start_f as in the reference.  Order of a call sequence: reset, source (each
+ * shot), green (each receiver x direction), gamma, phi; then forward runs on
+ * the returned test model, receivers + alpha (each shot), update.
+ *   create: J at the bins from signal_x (csqrt(-2w0/(i w)) factor), T = the
+ *           FFTW_BACKWARD of signal_true at the bins   (function.c:2091-2200)
+ *   reset:  init_iterate, grad = E_back = 0            (function.c:2386-2413)
+ *   source: laplaceToFreq_2 + residualE for source isrc: E_ff = (E/J) T over
+ *           the volume, dE = eobs - E_ff at the receivers, err_sum += sum|dE|
+ *                                                       (function.c:2091-2244)
+ *   green:  laplaceToFreq_back of a run with its source at receiver irec,
+ *           direction idir (backpropagation, function.c:2516-2526, 1821-1909),
+ *           streamed into sensitivity + conv_GdelE of every source
+ *                                                       (function.c:2246-2349)
+ *   gamma:  calc_gamma, grad += sum Re(E_ff . E_back)/sense over the cells
+ *           ncpml <= i,j,k < n - ncpml                  (function.c:2351-2384)
+ *   phi:    calc_phi: scaler = 0.01 max sigma/max grad over k <= iseabed,
+ *           sig2 = sigma - scaler grad                  (function.c:2565-2603)
+ *   receivers: laplaceToFreq_3 at the receivers (E_ff2) (function.c:2657-2755)
+ *   alpha:  calc_alpha: k1 += dE (E_ff2 - E_ff), k2 += (E_ff2 - E_ff)^2
+ *                                                       (function.c:2757-2788)
+ *   update: update_para2: alpha = 3|k1|/|k2|, sigma += alpha scaler grad where
+ *           k <= iseabed, clamp to [minval, maxval] (interior cells); applies
+ *           media_coeff_3d to the model, zeroes grad    (function.c:2790-2832) */
+typedef struct fwi3d_compat fwi3d_compat;
+int fwi3d_compat_create(fwi3d_model *m, int nsrc, int nrec, const int *rec_ijk, int start_f,
+                        int upper_f, int it, const double *signal_x, const double *signal_true,
+                        fwi3d_compat **out);
+int fwi3d_compat_destroy(fwi3d_compat *c);
+int fwi3d_compat_omegas(const fwi3d_compat *c, double *out);
+int fwi3d_compat_reset(fwi3d_compat *c);
+int fwi3d_compat_source(fwi3d_compat *c, int isrc, fwi3d_shot *s, const double *eobs, double *de,
+                        double *err_sum);
+int fwi3d_compat_green(fwi3d_compat *c, int irec, int idir, fwi3d_shot *g);
+int fwi3d_compat_gamma(fwi3d_compat *c, double *grad);
+int fwi3d_compat_phi(fwi3d_compat *c, int iseabed, double *scaler, double *sig2);
+int fwi3d_compat_receivers(fwi3d_compat *c, fwi3d_shot *s, double *eff);
+int fwi3d_compat_alpha(fwi3d_compat *c, int isrc, const double *eff2, double *k1, double *k2);
+int fwi3d_compat_update(fwi3d_compat *c, const double *k1, const double *k2, double minval,
+                        double maxval, int iseabed, double *alpha, double *sig);
+
 /* ---- multi-shot serving (persistent multi-shot launches) ------------------ */
 
 /* A batch advances all member shots (any models/grid sizes) with one launch per

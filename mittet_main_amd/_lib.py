@@ -81,6 +81,20 @@ _SIGS = {
     "fwi3d_grad_bytes": (C.c_longlong, [C.c_void_p]),
     "fwi3d_grad_forward": (C.c_int, [C.c_void_p]),
     "fwi3d_grad_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fwi3d_compat_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                      C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "fwi3d_compat_destroy": (C.c_int, [C.c_void_p]),
+    "fwi3d_compat_omegas": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fwi3d_compat_reset": (C.c_int, [C.c_void_p]),
+    "fwi3d_compat_source": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "fwi3d_compat_green": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "fwi3d_compat_gamma": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fwi3d_compat_phi": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "fwi3d_compat_receivers": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fwi3d_compat_alpha": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fwi3d_compat_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                      C.c_int, C.c_void_p, C.c_void_p]),
     "fwi3d_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "fwi3d_batch_destroy": (C.c_int, [C.c_void_p]),
     "fwi3d_batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),

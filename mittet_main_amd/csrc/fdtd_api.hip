@@ -1011,6 +1011,28 @@ int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal
     return FWI3D_OK;
 }
 
+}  // extern "C"
+
+// FFTW_BACKWARD (un-normalised exp(+2 pi i q n / it)) of the true waveform at
+// bins n0 <= n < n1
+static std::vector<std::complex<double>> true_spectrum(const double *strue, int it, int n0, int n1)
+{
+    std::vector<std::complex<double>> tru(std::max(0, n1 - n0));
+    for (int n = n0; n < n1; ++n) {
+        double re = 0.0, im = 0.0;
+        for (int q = 0; q < it; ++q) {
+            const long long mm = ((long long)q * n) % it;
+            const double ang = 2.0 * M_PI * (double)mm / (double)it;
+            re += strue[q] * std::cos(ang);
+            im += strue[q] * std::sin(ang);
+        }
+        tru[n - n0] = {re, im};
+    }
+    return tru;
+}
+
+extern "C" {
+
 int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue, double *e_w, double *j_w,
                           double *g_w, double *obs)
 {
@@ -1031,18 +1053,7 @@ int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue, double 
         rc = fwi3d_source_spectrum(dt, omega0, it, s->signal.data(), nbins, omg.data(), j_w);
     if (rc)
         return rc;
-    // FFTW_BACKWARD (un-normalised exp(+2 pi i j n / it)) of the true waveform, bins n < nbins
-    std::vector<std::complex<double>> tru(nbins);
-    for (int n = 0; n < nbins; ++n) {
-        double re = 0.0, im = 0.0;
-        for (int q = 0; q < it; ++q) {
-            const long long mm = ((long long)q * n) % it;
-            const double ang = 2.0 * M_PI * (double)mm / (double)it;
-            re += strue[q] * std::cos(ang);
-            im += strue[q] * std::sin(ang);
-        }
-        tru[n] = {re, im};
-    }
+    const std::vector<std::complex<double>> tru = true_spectrum(strue, it, 0, nbins);
     for (int row = 0; row < 3 * s->nrec; ++row)
         for (int n = 0; n < nbins; ++n) {
             const size_t q = (size_t)row * nbins + n;
@@ -1314,6 +1325,344 @@ int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
         djds[c] = ge[c] * de;
     if (s->src_cell >= 0)
         djds[s->src_cell] += gs;
+    return FWI3D_OK;
+}
+
+}  // extern "C"
+
+
+// ---------------------------------------------------------------- reference-style FWI
+
+struct fwi3d_compat {
+    fwi3d_model *m = nullptr;
+    hipStream_t st = nullptr;
+    int nsrc = 0, nrec = 0, f0 = 0, f1 = 0, nb = 0, it = 0;
+    std::vector<int> rec;                          // receiver cells
+    std::vector<mt::cplx> jw, tw;                  // J and T at the bins f0..f1-1
+    std::vector<std::vector<mt::cplx>> de, effl;   // per source [3][nrec][f1]: residual, E_ff at receivers
+    double scaler = 0.0;
+    int *d_rec = nullptr;
+    double *d_jw = nullptr, *d_tw = nullptr, *d_eff = nullptr, *d_eback = nullptr, *d_G = nullptr;
+    double *d_sense = nullptr, *d_grad = nullptr, *d_dconj = nullptr, *d_gather = nullptr;
+    double *d_work = nullptr, *d_blockmax = nullptr;
+};
+
+static constexpr int kPhiBlocks = 1024;
+
+static size_t compat_vol(const fwi3d_compat *c)
+{
+    const fwi3d_grid_params &p = c->m->p;
+    return (size_t)3 * c->nb * p.ix * p.iy * p.iz;   // complex values per spectra volume
+}
+
+// receiver E_ff (laplaceToFreq_2 / _3 at the receiver cells) of a shot whose
+// volume DFT ran at the object's bins: [3][nrec][f1] complex, 0 below f0
+// the shot's grid is the object's and its volume DFT ran at the object's bins
+static int compat_shot_ok(const fwi3d_compat *c, const fwi3d_shot *s)
+{
+    const fwi3d_grid_params &p = c->m->p, &q = s->m->p;
+    bool ok = p.ix == q.ix && p.iy == q.iy && p.iz == q.iz && s->nf == c->nb && s->it == c->it;
+    const double om = 2.0 * M_PI / c->it / p.dt;
+    for (int b = 0; ok && b < c->nb; ++b)
+        ok = s->omega[b] == om * (c->f0 + b);
+    return ok ? FWI3D_OK
+              : fail(FWI3D_EINVAL, "shot must share the object's grid, it and its %d DFT bins "
+                                   "(fwi3d_compat_omegas)", c->nb);
+}
+
+static int compat_receivers(fwi3d_compat *c, fwi3d_shot *s, std::vector<mt::cplx> &out)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    int rc = compat_shot_ok(c, s);
+    if (rc)
+        return rc;
+    const int nrec = c->nrec, nb = c->nb, f1 = c->f1;
+    std::vector<mt::cplx> raw((size_t)3 * nrec * nb);
+    HIP_TRY(mt::launch_compat_gather(s->d_acc, s->desc.astride, nb, nrec, c->d_rec, c->d_gather, s->st));
+    HIP_TRY(hipMemcpyAsync(raw.data(), c->d_gather, sizeof(mt::cplx) * raw.size(), hipMemcpyDeviceToHost,
+                           s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    out.assign((size_t)3 * nrec * f1, mt::cplx{0.0, 0.0});
+    for (int d = 0; d < 3; ++d)
+        for (int r = 0; r < nrec; ++r)
+            for (int b = 0; b < nb; ++b)
+                out[((size_t)d * nrec + r) * f1 + c->f0 + b] =
+                    mt::cmul(mt::cdiv(raw[((size_t)d * nrec + r) * nb + b], c->jw[b]), c->tw[b]);
+    return FWI3D_OK;
+}
+
+extern "C" {
+
+int fwi3d_compat_create(fwi3d_model *m, int nsrc, int nrec, const int *rec_ijk, int start_f,
+                        int upper_f, int it, const double *signal_x, const double *signal_true,
+                        fwi3d_compat **out)
+{
+    if (!m || !out || !rec_ijk || !signal_x || !signal_true || nsrc <= 0 || nrec <= 0 ||
+        start_f < 1 || upper_f <= start_f || it < 2 || upper_f > it)
+        return fail(FWI3D_EINVAL, "bad compat request (need nsrc, nrec > 0 and 1 <= start_f < upper_f <= it)");
+    *out = nullptr;
+    const fwi3d_grid_params &p = m->p;
+    auto *c = new fwi3d_compat();
+    c->m = m;
+    c->st = m->stream;
+    c->nsrc = nsrc;
+    c->nrec = nrec;
+    c->f0 = start_f;
+    c->f1 = upper_f;
+    c->nb = upper_f - start_f;
+    c->it = it;
+    for (int r = 0; r < nrec; ++r) {
+        const int i = rec_ijk[3 * r], j = rec_ijk[3 * r + 1], k = rec_ijk[3 * r + 2];
+        if (i < 0 || j < 0 || k < 0 || i >= p.ix || j >= p.iy || k >= p.iz) {
+            delete c;
+            return fail(FWI3D_EINVAL, "receiver %d outside the grid", r);
+        }
+        c->rec.push_back((k * p.iy + j) * p.ix + i);
+    }
+    // J at the bins (fictitious transform of signal_x with sqrt(-2 omega0/(i omega))) and T
+    const double om = 2.0 * M_PI / it / p.dt;
+    std::vector<double> omg(c->nb), jv(2 * (size_t)c->nb);
+    for (int b = 0; b < c->nb; ++b)
+        omg[b] = om * (start_f + b);
+    int rc = fwi3d_source_spectrum(p.dt, p.omega0, it, signal_x, c->nb, omg.data(), jv.data());
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    const auto tru = true_spectrum(signal_true, it, start_f, upper_f);
+    for (int b = 0; b < c->nb; ++b) {
+        c->jw.push_back({jv[2 * b], jv[2 * b + 1]});
+        c->tw.push_back({tru[b].real(), tru[b].imag()});
+    }
+    c->de.assign(nsrc, {});
+    c->effl.assign(nsrc, {});
+    const size_t N = (size_t)p.ix * p.iy * p.iz, vol = compat_vol(c);
+    if (dalloc(&c->d_rec, (size_t)nrec) != hipSuccess || dalloc(&c->d_jw, 2 * (size_t)c->nb) != hipSuccess ||
+        dalloc(&c->d_tw, 2 * (size_t)c->nb) != hipSuccess ||
+        dalloc(&c->d_eff, 2 * vol * nsrc) != hipSuccess || dalloc(&c->d_eback, 2 * vol * nsrc) != hipSuccess ||
+        dalloc(&c->d_G, 2 * vol) != hipSuccess || dalloc(&c->d_sense, N * nsrc) != hipSuccess ||
+        dalloc(&c->d_grad, N) != hipSuccess || dalloc(&c->d_dconj, 2 * (size_t)nsrc * c->nb) != hipSuccess ||
+        dalloc(&c->d_gather, 2 * (size_t)3 * nrec * c->nb) != hipSuccess || dalloc(&c->d_work, N) != hipSuccess ||
+        dalloc(&c->d_blockmax, 2 * (size_t)kPhiBlocks) != hipSuccess) {
+        fwi3d_compat_destroy(c);
+        return fail(FWI3D_ENOMEM, "compat buffers (%zu complex spectra per source)", vol);
+    }
+    if (hipMemcpy(c->d_rec, c->rec.data(), sizeof(int) * nrec, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_jw, c->jw.data(), sizeof(mt::cplx) * c->nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_tw, c->tw.data(), sizeof(mt::cplx) * c->nb, hipMemcpyHostToDevice) != hipSuccess) {
+        fwi3d_compat_destroy(c);
+        return fail(FWI3D_EDEVICE, "compat upload failed");
+    }
+    if ((rc = fwi3d_compat_reset(c))) {
+        fwi3d_compat_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_destroy(fwi3d_compat *c)
+{
+    if (!c)
+        return FWI3D_OK;
+    (void)hipStreamSynchronize(c->st);
+    dfree(c->d_rec);
+    dfree(c->d_jw);
+    dfree(c->d_tw);
+    dfree(c->d_eff);
+    dfree(c->d_eback);
+    dfree(c->d_G);
+    dfree(c->d_sense);
+    dfree(c->d_grad);
+    dfree(c->d_dconj);
+    dfree(c->d_gather);
+    dfree(c->d_work);
+    dfree(c->d_blockmax);
+    delete c;
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_omegas(const fwi3d_compat *c, double *out)
+{
+    if (!c || !out)
+        return fail(FWI3D_EINVAL, "null argument");
+    const double om = 2.0 * M_PI / c->it / c->m->p.dt;
+    for (int b = 0; b < c->nb; ++b)
+        out[b] = om * (c->f0 + b);
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_reset(fwi3d_compat *c)
+{
+    if (!c)
+        return fail(FWI3D_EINVAL, "null compat");
+    const size_t N = (size_t)c->m->p.ix * c->m->p.iy * c->m->p.iz;
+    HIP_TRY(mt::launch_fill(c->d_grad, N, 0.0, c->st));
+    HIP_TRY(mt::launch_fill(c->d_eback, 2 * compat_vol(c) * c->nsrc, 0.0, c->st));
+    HIP_TRY(mt::launch_fill(c->d_sense, N * c->nsrc, 0.0, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_source(fwi3d_compat *c, int isrc, fwi3d_shot *s, const double *eobs, double *de,
+                        double *err_sum)
+{
+    if (!c || !s || !eobs || isrc < 0 || isrc >= c->nsrc)
+        return fail(FWI3D_EINVAL, "bad compat source request");
+    int rc = check_shot_ready(s);
+    if (rc || (rc = compat_receivers(c, s, c->effl[isrc])))
+        return rc;
+    const size_t N = (size_t)s->desc.n;
+    HIP_TRY(mt::launch_compat_eff(s->d_acc, s->desc.astride, (int)N, c->nb, c->d_jw, c->d_tw,
+                                  c->d_eff + 2 * compat_vol(c) * isrc, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    // residualE: dE = Eobs - E_ff at the receivers over bins [0, upperF);
+    // err += |dEx| + |dEy| + |dEz| (receiver, then bin)
+    const int nrec = c->nrec, f1 = c->f1;
+    std::vector<mt::cplx> &d = c->de[isrc];
+    d.assign((size_t)3 * nrec * f1, mt::cplx{0.0, 0.0});
+    const mt::cplx *ob = reinterpret_cast<const mt::cplx *>(eobs);
+    for (size_t q = 0; q < d.size(); ++q)
+        d[q] = {ob[q].re - c->effl[isrc][q].re, ob[q].im - c->effl[isrc][q].im};
+    double err = err_sum ? *err_sum : 0.0;
+    for (int r = 0; r < nrec; ++r)
+        for (int w = 0; w < f1; ++w)
+            for (int comp = 0; comp < 3; ++comp) {
+                const mt::cplx v = d[((size_t)comp * nrec + r) * f1 + w];
+                err += std::hypot(v.re, v.im);
+            }
+    if (err_sum)
+        *err_sum = err;
+    if (de)
+        std::memcpy(de, d.data(), sizeof(mt::cplx) * d.size());
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_green(fwi3d_compat *c, int irec, int idir, fwi3d_shot *g)
+{
+    if (!c || !g || irec < 0 || irec >= c->nrec || idir < 0 || idir > 2)
+        return fail(FWI3D_EINVAL, "bad compat Green's function request");
+    int rc = compat_shot_ok(c, g);
+    if (rc || (rc = check_shot_ready(g)))
+        return rc;
+    for (int s = 0; s < c->nsrc; ++s)
+        if (c->de[s].empty())
+            return fail(FWI3D_ESTATE, "source %d has no residual yet", s);
+    std::vector<mt::cplx> dc((size_t)c->nsrc * c->nb);
+    for (int s = 0; s < c->nsrc; ++s)
+        for (int b = 0; b < c->nb; ++b) {
+            const mt::cplx v = c->de[s][((size_t)idir * c->nrec + irec) * c->f1 + c->f0 + b];
+            dc[(size_t)s * c->nb + b] = {v.re, -v.im};
+        }
+    HIP_TRY(hipMemcpyAsync(c->d_dconj, dc.data(), sizeof(mt::cplx) * dc.size(), hipMemcpyHostToDevice, g->st));
+    HIP_TRY(mt::launch_compat_green(g->d_acc, g->desc.astride, g->desc.n, c->nb, c->d_jw, c->nsrc, c->d_eff,
+                                    c->d_dconj, c->d_G, c->d_sense, c->d_eback, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_gamma(fwi3d_compat *c, double *grad)
+{
+    if (!c)
+        return fail(FWI3D_EINVAL, "null compat");
+    const fwi3d_grid_params &p = c->m->p;
+    HIP_TRY(mt::launch_compat_gamma(p.ix, p.iy, p.iz, p.ncpml, c->nb, c->f0, c->nsrc, c->d_eff, c->d_eback,
+                                    c->d_sense, c->d_grad, c->st));
+    if (grad)
+        HIP_TRY(hipMemcpyAsync(grad, c->d_grad, sizeof(double) * p.ix * p.iy * p.iz, hipMemcpyDeviceToHost,
+                               c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_receivers(fwi3d_compat *c, fwi3d_shot *s, double *eff)
+{
+    if (!c || !s || !eff)
+        return fail(FWI3D_EINVAL, "null argument");
+    int rc = check_shot_ready(s);
+    std::vector<mt::cplx> v;
+    if (rc || (rc = compat_receivers(c, s, v)))
+        return rc;
+    std::memcpy(eff, v.data(), sizeof(mt::cplx) * v.size());
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_phi(fwi3d_compat *c, int iseabed, double *scaler, double *sig2)
+{
+    if (!c)
+        return fail(FWI3D_EINVAL, "null compat");
+    fwi3d_model *m = c->m;
+    const fwi3d_grid_params &p = m->p;
+    const int N = p.ix * p.iy * p.iz;
+    std::vector<double> bm(2 * (size_t)kPhiBlocks);
+    double g0 = 0.0;
+    HIP_TRY(mt::launch_phi_max(m->d_sig, c->d_grad, N, p.ix * p.iy, iseabed, c->d_blockmax, kPhiBlocks, c->st));
+    HIP_TRY(hipMemcpyAsync(bm.data(), c->d_blockmax, sizeof(double) * bm.size(), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipMemcpyAsync(&g0, c->d_grad, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    // the reference starts both maxima from cell 0
+    double sigmax = m->sig[0], gradmax = g0;
+    for (int b = 0; b < kPhiBlocks; ++b) {
+        sigmax = bm[2 * b] > sigmax ? bm[2 * b] : sigmax;
+        gradmax = bm[2 * b + 1] > gradmax ? bm[2 * b + 1] : gradmax;
+    }
+    c->scaler = (double)0.01f * sigmax / gradmax;
+    HIP_TRY(mt::launch_phi_apply(m->d_sig, c->d_grad, c->scaler, N, c->d_work, c->st));
+    if (sig2)
+        HIP_TRY(hipMemcpyAsync(sig2, c->d_work, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (scaler)
+        *scaler = c->scaler;
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_alpha(fwi3d_compat *c, int isrc, const double *eff2, double *k1, double *k2)
+{
+    if (!c || !eff2 || !k1 || !k2 || isrc < 0 || isrc >= c->nsrc || c->de[isrc].empty())
+        return fail(FWI3D_EINVAL, "bad compat alpha request");
+    const mt::cplx *e2 = reinterpret_cast<const mt::cplx *>(eff2);
+    mt::cplx a = {k1[0], k1[1]}, b = {k2[0], k2[1]};
+    const int nrec = c->nrec, f1 = c->f1;
+    for (int r = 0; r < nrec; ++r)
+        for (int w = 0; w < f1; ++w)
+            for (int comp = 0; comp < 3; ++comp) {
+                const size_t q = ((size_t)comp * nrec + r) * f1 + w;
+                const mt::cplx t = {e2[q].re - c->effl[isrc][q].re, e2[q].im - c->effl[isrc][q].im};
+                const mt::cplx u = mt::cmul(c->de[isrc][q], t), v = mt::cmul(t, t);
+                a = {a.re + u.re, a.im + u.im};
+                b = {b.re + v.re, b.im + v.im};
+            }
+    k1[0] = a.re;
+    k1[1] = a.im;
+    k2[0] = b.re;
+    k2[1] = b.im;
+    return FWI3D_OK;
+}
+
+int fwi3d_compat_update(fwi3d_compat *c, const double *k1, const double *k2, double minval, double maxval,
+                        int iseabed, double *alpha, double *sig)
+{
+    if (!c || !k1 || !k2)
+        return fail(FWI3D_EINVAL, "null argument");
+    fwi3d_model *m = c->m;
+    const fwi3d_grid_params &p = m->p;
+    const size_t N = (size_t)p.ix * p.iy * p.iz;
+    const double a = (double)3.f * std::hypot(k1[0], k1[1]) / std::hypot(k2[0], k2[1]);
+    HIP_TRY(mt::launch_copy(m->d_sig, c->d_work, N, c->st));
+    HIP_TRY(mt::launch_update(c->d_work, c->d_grad, a * c->scaler, minval, maxval, p.ix, p.iy, p.iz, p.ncpml,
+                              iseabed, c->st));
+    std::vector<double> ns(N);
+    HIP_TRY(hipMemcpyAsync(ns.data(), c->d_work, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(mt::launch_fill(c->d_grad, N, 0.0, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    int rc = fwi3d_media_coeff_3d(m, ns.data());
+    if (rc)
+        return rc;
+    if (alpha)
+        *alpha = a;
+    if (sig)
+        std::memcpy(sig, ns.data(), sizeof(double) * N);
     return FWI3D_OK;
 }
 

@@ -117,4 +117,49 @@ hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t
 hipError_t launch_eps(const double *sig, double *eps, double *cbi, int n, double omega0, double dt,
                       hipStream_t st);
 
+// Complex arithmetic of the reference's C99 `double _Complex` code, usable on
+// host and device with identical rounding (no contraction): products are
+// (ac - bd) + (ad + bc)i; quotients use Smith's scaling, the algorithm of the
+// compiler runtime's complex division for finite operands.
+struct cplx {
+    double re, im;
+};
+
+__host__ __device__ inline cplx cmul(cplx a, cplx b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+__host__ __device__ inline cplx cdiv(cplx a, cplx b)
+{
+    if (__builtin_fabs(b.re) < __builtin_fabs(b.im)) {
+        const double r = b.re / b.im, den = b.re * r + b.im;
+        return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+    }
+    const double r = b.im / b.re, den = b.im * r + b.re;
+    return {(a.im * r + a.re) / den, (a.im - a.re * r) / den};
+}
+
+// Reference-style FWI gradient chain (fdtd_compat.hip).  Spectra are complex
+// [comp][bin][cell]; `acc` are a shot's running-DFT accumulators with row
+// stride `astride` cells.
+hipError_t launch_compat_eff(const double *acc, long long astride, int n, int nb,
+                             const double *jw, const double *tw, double *eff, hipStream_t st);
+hipError_t launch_compat_green(const double *acc, long long astride, int n, int nb,
+                               const double *jw, int nsrc, const double *eff,
+                               const double *dconj, double *G, double *sense, double *eback,
+                               hipStream_t st);
+hipError_t launch_compat_gamma(int ix, int iy, int iz, int nc, int nb, int nlow, int nsrc,
+                               const double *eff, const double *eback, const double *sense,
+                               double *grad, hipStream_t st);
+hipError_t launch_compat_gather(const double *acc, long long astride, int nb, int nrec,
+                                const int *rec, double *out, hipStream_t st);
+hipError_t launch_phi_max(const double *sig, const double *grad, int n, int nxy, int iseabed,
+                          double *blockmax, int nblocks, hipStream_t st);
+hipError_t launch_phi_apply(const double *sig, const double *grad, double scaler, int n,
+                            double *sig2, hipStream_t st);
+hipError_t launch_update(double *sig, const double *grad, double step, double minval,
+                         double maxval, int ix, int iy, int iz, int nc, int iseabed,
+                         hipStream_t st);
+
 }  // namespace mt

@@ -104,6 +104,14 @@ def lib():
         L.or_fft_backward_real.argtypes = [i, dp, dp]
         L.or_laplace_to_freq.argtypes = [i, d, d, i, dp, dp, dp, dp, dp, dp, dp]
         L.or_num_threads.restype = i
+        L.or_laplace_volume.argtypes = [i, i, i, dp, dp, dp, dp]
+        L.or_residual_e.argtypes = [i, i, dp, dp, dp, dp]
+        L.or_sensitivity.argtypes = [i, i, i, dp, dp, dp]
+        L.or_conv_gdele.argtypes = [i, i, i, dp, dp, dp]
+        L.or_calc_gamma.argtypes = [i, i, i, i, i, dp, dp, dp, dp]
+        L.or_calc_phi.argtypes = [i, i, i, i, dp, dp, dp, dp]
+        L.or_calc_alpha.argtypes = [i, i, dp, dp, dp, dp, dp]
+        L.or_update_para2.argtypes = [i, i, i, i, i, dp, dp, d, d, d, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -318,3 +326,96 @@ class RefCpml:
 
     def hpml(self):
         self.L.ref_hpml()
+
+
+# ---- the reference's FWI gradient chain (compat mode) -------------------------
+# Thin wrappers over the literal C loops; complex arrays are complex128.
+
+def _cd(a):
+    return _dp(np.ascontiguousarray(a, dtype=np.complex128).view(np.float64))
+
+
+def laplace_volume(acc_bins, jw, tw, upperf, startf):
+    """G = E/J (tw None) or E_ff = (E/J) T over the volume: [3, upperf, N], zero
+    below startf (satan_i/fwi3d_cpu_function_v1.4.c:2166-2189, 1883-1894)."""
+    acc = np.ascontiguousarray(acc_bins, dtype=np.complex128)
+    n = acc.shape[-1]
+    out = np.zeros((3, upperf, n), dtype=np.complex128)
+    jwc = np.ascontiguousarray(jw, dtype=np.complex128)
+    twc = None if tw is None else np.ascontiguousarray(tw, dtype=np.complex128)
+    lib().or_laplace_volume(n, upperf, startf, _cd(acc), _cd(jwc),
+                            None if twc is None else _cd(twc), _cd(out))
+    return out
+
+
+def residual_e(eobs, effrec):
+    """dE = Eobs - E_ff at the receivers and sum |dE| (function.c:2202-2244)."""
+    ob = np.ascontiguousarray(eobs, dtype=np.complex128)
+    ef = np.ascontiguousarray(effrec, dtype=np.complex128)
+    de = np.zeros_like(ob)
+    err = np.zeros(1)
+    lib().or_residual_e(ob.shape[1], ob.shape[2], _cd(ob), _cd(ef), _cd(de), _dp(err))
+    return de, float(err[0])
+
+
+def sensitivity(eff, ebb):
+    """eff [3, upperf, N]; ebb [3, nrec, 3, upperf, N] (function.c:2246-2290)."""
+    eff = np.ascontiguousarray(eff, dtype=np.complex128)
+    ebb = np.ascontiguousarray(ebb, dtype=np.complex128)
+    n, upperf, nrec = eff.shape[-1], eff.shape[1], ebb.shape[1]
+    sense = np.zeros(n)
+    lib().or_sensitivity(n, nrec, upperf, _cd(eff), _cd(ebb), _dp(sense))
+    return sense
+
+
+def conv_gdele(ebb, de):
+    """E_back [3, upperf, N] (function.c:2292-2349)."""
+    ebb = np.ascontiguousarray(ebb, dtype=np.complex128)
+    de = np.ascontiguousarray(de, dtype=np.complex128)
+    nrec, upperf, n = ebb.shape[1], ebb.shape[3], ebb.shape[4]
+    out = np.zeros((3, upperf, n), dtype=np.complex128)
+    lib().or_conv_gdele(n, nrec, upperf, _cd(ebb), _cd(de), _cd(out))
+    return out
+
+
+def calc_gamma(shape, nc, eff, eback, sense, grad=None):
+    """grad += ... on the interior cells (function.c:2351-2384); shape (iz, iy, ix)."""
+    iz, iy, ix = shape
+    eff = np.ascontiguousarray(eff, dtype=np.complex128)
+    g = np.zeros(ix * iy * iz) if grad is None else np.ascontiguousarray(grad, dtype=np.float64)
+    lib().or_calc_gamma(ix, iy, iz, nc, eff.shape[1], _cd(eff), _cd(eback),
+                        _dp(np.ascontiguousarray(sense)), _dp(g))
+    return g
+
+
+def calc_phi(shape, iseabed, sig, grad):
+    """(sig2, scaler) (function.c:2565-2603)."""
+    iz, iy, ix = shape
+    sg = np.ascontiguousarray(sig, dtype=np.float64).ravel()
+    gr = np.ascontiguousarray(grad, dtype=np.float64).ravel()
+    sig2 = np.zeros_like(sg)
+    sc = np.zeros(1)
+    lib().or_calc_phi(ix, iy, iz, iseabed, _dp(sg), _dp(gr), _dp(sig2), _dp(sc))
+    return sig2, float(sc[0])
+
+
+def calc_alpha(de, eff2, effl, k1=0j, k2=0j):
+    """(k1, k2) accumulated over one source (function.c:2757-2788)."""
+    a = np.array([k1.real, k1.imag])
+    b = np.array([k2.real, k2.imag])
+    de = np.ascontiguousarray(de, dtype=np.complex128)
+    lib().or_calc_alpha(de.shape[1], de.shape[2], _cd(de), _cd(eff2), _cd(effl), _dp(a), _dp(b))
+    return complex(a[0], a[1]), complex(b[0], b[1])
+
+
+def update_para2(shape, nc, iseabed, k1, k2, scaler, minval, maxval, sig, grad):
+    """(alpha, new sig); grad is zeroed as the reference does (function.c:2790-2832)."""
+    iz, iy, ix = shape
+    sg = np.array(sig, dtype=np.float64).ravel()
+    gr = np.array(grad, dtype=np.float64).ravel()
+    a = np.array([k1.real, k1.imag])
+    b = np.array([k2.real, k2.imag])
+    al = np.zeros(1)
+    lib().or_update_para2(ix, iy, iz, nc, iseabed, _dp(a), _dp(b), scaler, minval, maxval,
+                          _dp(sg), _dp(gr), _dp(al))
+    return float(al[0]), sg

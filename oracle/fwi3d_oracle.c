@@ -554,3 +554,181 @@ void or_laplace_to_freq(int it, double dt, double omega0, int nbins,
     }
     free(tru);
 }
+
+
+/* ------------------------------------------------------------------------
+ * The reference's FWI gradient chain (compat mode).
+ */
+
+/* laplaceToFreq_2 / laplaceToFreq_back per cell (fwi3d_cpu_function_v1.4.c:
+ * 2166-2189, 1883-1894): G = E/J, and E_ff = G * out_True when tw is given.
+ * acc_bins holds the volume DFT at bins startF..upperF-1: [3][upperF-startF][N];
+ * out spans [3][upperF][N] and stays zero below startF. */
+void or_laplace_volume(int n, int upperf, int startf, const double *acc_bins,
+                       const double *jw, const double *tw, double *out)
+{
+    const int nb = upperf - startf;
+    const double complex *A = (const double complex *)acc_bins;
+    const double complex *J = (const double complex *)jw;
+    const double complex *T = (const double complex *)tw;
+    double complex *O = (double complex *)out;
+    memset(out, 0, sizeof(double complex) * 3 * (size_t)upperf * n);
+    for (int d = 0; d < 3; ++d)
+        for (int b = 0; b < nb; ++b)
+            for (int c = 0; c < n; ++c) {
+                const double complex g = A[((size_t)d * nb + b) * n + c] / J[b];
+                O[((size_t)d * upperf + startf + b) * n + c] = T ? g * T[b] : g;
+            }
+}
+
+/* residualE (fwi3d_cpu_function_v1.4.c:2202-2244): dE = Eobs - E_ff at the
+ * receivers; err += |dEx| + |dEy| + |dEz| over receivers, then bins */
+void or_residual_e(int nrec, int upperf, const double *eobs, const double *effrec,
+                   double *de, double *err_sum)
+{
+    const double complex *ob = (const double complex *)eobs;
+    const double complex *ef = (const double complex *)effrec;
+    double complex *dd = (double complex *)de;
+    for (int irec = 0; irec < nrec; ++irec)
+        for (int w = 0; w < upperf; ++w)
+            for (int d = 0; d < 3; ++d) {
+                const size_t q = ((size_t)d * nrec + irec) * upperf + w;
+                dd[q] = ob[q] - ef[q];
+                *err_sum += cabs(dd[q]);
+            }
+}
+
+/* sensitivity (fwi3d_cpu_function_v1.4.c:2246-2290):
+ * sense = sum_{irec, idir, w} |Ex_ff Ex_bb| + |Ey_ff Ey_bb| + |Ez_ff Ez_bb| */
+void or_sensitivity(int n, int nrec, int upperf, const double *eff, const double *ebb,
+                    double *sense)
+{
+    const double complex *F = (const double complex *)eff;
+    const double complex *B = (const double complex *)ebb;
+    const size_t vol = (size_t)upperf * n, gvol = (size_t)nrec * 3 * vol;
+#pragma omp parallel for
+    for (int c = 0; c < n; ++c) {
+        double s = 0.0;
+        for (int irec = 0; irec < nrec; ++irec)
+            for (int idir = 0; idir < 3; ++idir)
+                for (int w = 0; w < upperf; ++w) {
+                    const size_t ijks = (size_t)w * n + c;
+                    const size_t ijkr = ((size_t)irec * 3 + idir) * vol + (size_t)w * n + c;
+                    for (int d = 0; d < 3; ++d)
+                        s += cabs(F[d * vol + ijks] * B[d * gvol + ijkr]);
+                }
+        sense[c] = s;
+    }
+}
+
+/* conv_GdelE (fwi3d_cpu_function_v1.4.c:2292-2349):
+ * E_back,d(w) = sum_{irec, idir} G^{irec,idir}_d(w) conj(dE_idir(irec, w)) */
+void or_conv_gdele(int n, int nrec, int upperf, const double *ebb, const double *de,
+                   double *eback)
+{
+    const double complex *B = (const double complex *)ebb;
+    const double complex *D = (const double complex *)de;
+    double complex *E = (double complex *)eback;
+    const size_t vol = (size_t)upperf * n, gvol = (size_t)nrec * 3 * vol;
+    memset(eback, 0, sizeof(double complex) * 3 * vol);
+#pragma omp parallel for
+    for (int c = 0; c < n; ++c)
+        for (int irec = 0; irec < nrec; ++irec)
+            for (int idir = 0; idir < 3; ++idir)
+                for (int w = 0; w < upperf; ++w) {
+                    const size_t ijkw = (size_t)w * n + c;
+                    const size_t ijkr = ((size_t)irec * 3 + idir) * vol + (size_t)w * n + c;
+                    const double complex cd = conj(D[((size_t)idir * nrec + irec) * upperf + w]);
+                    for (int d = 0; d < 3; ++d)
+                        E[d * vol + ijkw] += B[d * gvol + ijkr] * cd;
+                }
+}
+
+/* calc_gamma (fwi3d_cpu_function_v1.4.c:2351-2384): on ncpml <= i,j,k < n-ncpml,
+ * grad += sum_w Re(Ex_ff Ex_back + Ey_ff Ey_back + Ez_ff Ez_back)/sense */
+void or_calc_gamma(int ix, int iy, int iz, int nc, int upperf, const double *eff,
+                   const double *eback, const double *sense, double *grad)
+{
+    const double complex *F = (const double complex *)eff;
+    const double complex *E = (const double complex *)eback;
+    const int n = ix * iy * iz;
+    const size_t vol = (size_t)upperf * n;
+#pragma omp parallel for
+    for (int k = nc; k < iz - nc; ++k)
+        for (int j = nc; j < iy - nc; ++j)
+            for (int i = nc; i < ix - nc; ++i) {
+                const int ijk = k * ix * iy + j * ix + i;
+                for (int w = 0; w < upperf; ++w) {
+                    const size_t q = (size_t)w * n + ijk;
+                    grad[ijk] += creal(F[q] * E[q] + F[vol + q] * E[vol + q] +
+                                       F[2 * vol + q] * E[2 * vol + q]) / sense[ijk];
+                }
+            }
+}
+
+/* calc_phi (fwi3d_cpu_function_v1.4.c:2565-2603) */
+void or_calc_phi(int ix, int iy, int iz, int iseabed, const double *sig, const double *grad,
+                 double *sig2, double *scaler)
+{
+    double sigmax2 = sig[0], gradmax = grad[0];
+    for (int k = 0; k < iz; ++k)
+        for (int j = 0; j < iy; ++j)
+            for (int i = 0; i < ix; ++i) {
+                const int ijk = k * ix * iy + j * ix + i;
+                if (k <= iseabed) {
+                    if (sig[ijk] > sigmax2)
+                        sigmax2 = sig[ijk];
+                    if (grad[ijk] > gradmax)
+                        gradmax = grad[ijk];
+                }
+            }
+    *scaler = 0.01f * sigmax2 / gradmax;
+    for (int c = 0; c < ix * iy * iz; ++c)
+        sig2[c] = sig[c] - *scaler * grad[c];
+}
+
+/* calc_alpha (fwi3d_cpu_function_v1.4.c:2757-2788), receiver arrays */
+void or_calc_alpha(int nrec, int upperf, const double *de, const double *eff2,
+                   const double *effl, double *k1, double *k2)
+{
+    const double complex *D = (const double complex *)de;
+    const double complex *F2 = (const double complex *)eff2;
+    const double complex *FL = (const double complex *)effl;
+    double complex a = k1[0] + I * k1[1], b = k2[0] + I * k2[1];
+    for (int irec = 0; irec < nrec; ++irec)
+        for (int w = 0; w < upperf; ++w) {
+            double complex t[3];
+            for (int d = 0; d < 3; ++d) {
+                const size_t q = ((size_t)d * nrec + irec) * upperf + w;
+                t[d] = F2[q] - FL[q];
+            }
+            for (int d = 0; d < 3; ++d)
+                a += D[((size_t)d * nrec + irec) * upperf + w] * t[d];
+            for (int d = 0; d < 3; ++d)
+                b += t[d] * t[d];
+        }
+    k1[0] = creal(a);
+    k1[1] = cimag(a);
+    k2[0] = creal(b);
+    k2[1] = cimag(b);
+}
+
+/* update_para2 (fwi3d_cpu_function_v1.4.c:2790-2832) */
+void or_update_para2(int ix, int iy, int iz, int nc, int iseabed, const double *k1,
+                     const double *k2, double scaler, double minval, double maxval,
+                     double *sig, double *grad, double *alpha)
+{
+    *alpha = 3.f * cabs(k1[0] + I * k1[1]) / cabs(k2[0] + I * k2[1]);
+    for (int k = nc; k < iz - nc; ++k)
+        for (int j = nc; j < iy - nc; ++j)
+            for (int i = nc; i < ix - nc; ++i) {
+                const int ijk = k * ix * iy + j * ix + i;
+                if (k <= iseabed)
+                    sig[ijk] += *alpha * scaler * grad[ijk];
+                if (sig[ijk] < minval)
+                    sig[ijk] = minval;
+                if (sig[ijk] > maxval)
+                    sig[ijk] = maxval;
+            }
+    memset(grad, 0, sizeof(double) * (size_t)ix * iy * iz);
+}

@@ -87,6 +87,29 @@ void or_laplace_to_freq(int it, double dt, double omega0, int nbins,
                         const double *strue,
                         double *e_w, double *j_w, double *g_w, double *obs);
 
+/* ---- the reference's FWI gradient chain (compat mode), literal loops ----
+ * Spectra arrays are complex128 over all bins w in [0, upperF) (zero below
+ * startF, as the reference leaves them): volume [3][upperF][N], Green's
+ * functions [3 comps][nrec][3 dirs][upperF][N], receiver arrays
+ * [3][nrec][upperF]. */
+void or_laplace_volume(int n, int upperf, int startf, const double *acc_bins,
+                       const double *jw, const double *tw, double *out);
+void or_residual_e(int nrec, int upperf, const double *eobs, const double *effrec,
+                   double *de, double *err_sum);
+void or_sensitivity(int n, int nrec, int upperf, const double *eff, const double *ebb,
+                    double *sense);
+void or_conv_gdele(int n, int nrec, int upperf, const double *ebb, const double *de,
+                   double *eback);
+void or_calc_gamma(int ix, int iy, int iz, int nc, int upperf, const double *eff,
+                   const double *eback, const double *sense, double *grad);
+void or_calc_phi(int ix, int iy, int iz, int iseabed, const double *sig, const double *grad,
+                 double *sig2, double *scaler);
+void or_calc_alpha(int nrec, int upperf, const double *de, const double *eff2,
+                   const double *effl, double *k1, double *k2);
+void or_update_para2(int ix, int iy, int iz, int nc, int iseabed, const double *k1,
+                     const double *k2, double scaler, double minval, double maxval,
+                     double *sig, double *grad, double *alpha);
+
 #ifdef __cplusplus
 }
 #endif

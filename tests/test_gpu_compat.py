@@ -1,0 +1,106 @@
+"""The reference's FWI iteration (compat mode) on the GPU against the oracle's
+literal restatement of the same chain (laplaceToFreq_2/_back/_3, residualE,
+sensitivity, conv_GdelE, calc_gamma, calc_phi, calc_alpha, update_para2;
+satan_i/fwi3d_cpu_function_v1.4.c:1821-2384, 2565-2832).
+
+The forward runs and their volume DFTs are bitwise equal on both sides
+(test_gpu_parity.py); what differs is rounding in the source transform J
+(the library factors the csqrt term out of the sum) and in the complex
+moduli, so the comparison is at 1e-9 relative L2, the tolerance stated for
+diffusive-domain frequency fields."""
+import numpy as np
+import pytest
+
+import oracle
+from mittet_main_amd import Grid
+from mittet_main_amd.configs import dt_for
+from mittet_main_amd.fwi3d import firstderiv_gauss
+from mittet_main_amd.fwi_compat import CompatFWI
+
+pytestmark = pytest.mark.gpu
+
+OM0 = 2 * np.pi
+TOL = 1e-9
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b))
+
+
+def oracle_acc(grid, sig, src, comp, signal, it, omegas):
+    S = oracle.OracleSim(grid.ix, grid.iy, grid.iz, grid.ncpml, grid.dx, grid.dy, grid.dz,
+                         grid.dt, grid.omega0, grid.cmax)
+    S.set_sigma(sig)
+    S.init_cpml()
+    S.media_coeff_3d()
+    w = oracle.dft_weights(it, grid.dt, grid.omega0, omegas)
+    _, acc = S.forward(comp, signal, tuple(int(c) + 1 for c in src), it - 1, it=it, weights=w)
+    return acc
+
+
+def test_compat_iteration_matches_oracle():
+    ix, iy, iz, nc, it = 22, 20, 18, 4, 90
+    start_f, upper_f, iseabed = 1, 4, 9
+    rng = np.random.default_rng(12)
+    sig_true = np.exp(rng.uniform(np.log(0.2), np.log(2.0), (iz, iy, ix)))
+    sig0 = np.full((iz, iy, ix), 0.6)
+    dt, cmax = dt_for(50.0, 45.0, 40.0, sig_min=0.2)
+    g = Grid(ix, iy, iz, nc, 50.0, 45.0, 40.0, dt, OM0, cmax)
+    signal_x = firstderiv_gauss(it, dt, 6.0)
+    signal_true = firstderiv_gauss(it, dt, 2.0)
+    sources = np.array([[11, 10, 6], [8, 12, 7]])
+    receivers = np.array([[7, 7, 5], [14, 12, 5]])
+    F = CompatFWI(g, sources, receivers, it, signal_x, signal_true, start_f, upper_f,
+                  minval=0.05, maxval=5.0, iseabed=iseabed)
+    true_model = F.test_model
+    true_model.media_coeff_3d(sig_true)
+    eobs = [F.forward_receivers(true_model, s) for s in sources]
+    res = F.iterate(sig0, eobs)
+
+    # ---- oracle: the same iteration with the reference's loops
+    om = F.omegas
+    lap = oracle.laplace_to_freq(np.zeros(it), signal_x, signal_true, dt, OM0, upper_f)
+    jw = lap["J"][start_f:]
+    tw = oracle.fft_backward_real(signal_true)[start_f:upper_f]
+    shape, N = sig0.shape, sig0.size
+    rflat = [(r[2] * iy + r[1]) * ix + r[0] for r in receivers]
+    ebb = np.zeros((3, len(receivers), 3, upper_f, N), dtype=complex)
+    for r, rec in enumerate(receivers):
+        for d in range(3):
+            acc = oracle_acc(g, sig0, rec, d, signal_x, it, om)
+            ebb[:, r, d] = oracle.laplace_volume(acc, jw, None, upper_f, start_f)
+    grad = np.zeros(N)
+    err = 0.0
+    effl, des = [], []
+    for s, src in enumerate(sources):
+        acc = oracle_acc(g, sig0, src, 2, signal_x, it, om)
+        eff = oracle.laplace_volume(acc, jw, tw, upper_f, start_f)
+        effrec = eff[:, :, rflat].transpose(0, 2, 1)
+        de, e = oracle.residual_e(eobs[s], effrec)
+        err += e
+        sense = oracle.sensitivity(eff, ebb)
+        eback = oracle.conv_gdele(ebb, de)
+        grad = oracle.calc_gamma(shape, nc, eff, eback, sense, grad)
+        effl.append(effrec)
+        des.append(de)
+    sig2, scaler = oracle.calc_phi(shape, iseabed, sig0, grad)
+    k1 = k2 = 0j
+    for s, src in enumerate(sources):
+        acc = oracle_acc(g, sig2.reshape(shape), src, 2, signal_x, it, om)
+        eff2 = oracle.laplace_volume(acc, jw, tw, upper_f, start_f)[:, :, rflat].transpose(0, 2, 1)
+        k1, k2 = oracle.calc_alpha(des[s], eff2, effl[s], k1, k2)
+    alpha, sig_new = oracle.update_para2(shape, nc, iseabed, k1, k2, scaler, 0.05, 5.0, sig0, grad)
+
+    assert abs(res.err_sum - err) <= TOL * err
+    for s in range(len(sources)):
+        assert rel(res.residuals[s], des[s]) <= TOL
+    assert np.any(res.grad) and rel(res.grad.ravel(), grad) <= TOL
+    assert abs(res.scaler - scaler) <= TOL * abs(scaler)
+    assert rel(res.sigma_test.ravel(), sig2) <= TOL
+    assert abs(res.alpha - alpha) <= TOL * alpha
+    assert rel(res.sigma.ravel(), sig_new) <= TOL
+    # the model now holds the updated conductivity: same data as a fresh upload
+    after = F.forward_receivers(F.model, sources[0])
+    F.test_model.media_coeff_3d(res.sigma)
+    np.testing.assert_array_equal(after, F.forward_receivers(F.test_model, sources[0]))
+    F.close()
