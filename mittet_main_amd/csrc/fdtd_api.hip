@@ -181,6 +181,8 @@ struct fwi3d_shot {
     bool desc_dirty = true;
     int kind = 0;          // fused kernel family (see launch_eh)
     int nblocks = 0;       // fused blocks (padded to a multiple of 8)
+    int nblocks_l = 0;     // LDS-kernel blocks of family 3 (padded to a multiple of 8)
+    int *d_shell = nullptr;  // family 3: tiles of the per-cell kernels
     int nblocks_op = 0;    // per-cell blocks of the stand-alone operators
     int ncellblocks = 0;
     int step = 0;          // steps done (host mirror of *d_step)
@@ -195,8 +197,8 @@ struct fwi3d_batch {
     bool own_stream = false;
     std::vector<fwi3d_shot *> shots;
     ShotDev *d_desc = nullptr;
-    int *d_pre = nullptr, *d_pre_cells = nullptr;
-    int nblocks = 0, ncellblocks = 0;
+    int *d_pre = nullptr, *d_pre_l = nullptr, *d_pre_cells = nullptr;
+    int nblocks = 0, nblocks_l = 0, ncellblocks = 0;
     int kind = 0;         // fused kernel family of the members (all equal)
     bool dirty = true;
     hipGraphExec_t graph = nullptr;
@@ -245,32 +247,44 @@ int check_shot_ready(fwi3d_shot *s)
     return FWI3D_OK;
 }
 
-// launches for local steps j in [0, L) followed by the DFT flush and step advance
-// fused kernel family: 0 per-cell, 1 per-cell x-pairs, 2 z-marching
-int launch_eh(int kind, const ShotDev *d, int ns, const int *pre, int nblocks, int j, bool h,
-              hipStream_t st)
+// A fused kernel family and its blocks: 0 per-cell, 1 per-cell x-pairs,
+// 2 z-marching, 3 LDS z-marching interior (pre_l / nblocks_l) plus per-cell
+// kernels over the shell tiles (pre / nblocks).  `pre` are exclusive prefixes
+// of per-shot block counts.
+struct Fam {
+    int kind;
+    const int *pre;
+    int nblocks;
+    const int *pre_l;
+    int nblocks_l;
+};
+
+int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t st)
 {
     hipError_t e;
-    if (kind == 2)
-        e = h ? mt::launch_hfield_z(d, ns, pre, nblocks, j, 1, st)
-              : mt::launch_efield_z(d, ns, pre, nblocks, j, 1, st);
-    else if (kind == 1)
-        e = h ? mt::launch_hfield_p(d, ns, pre, nblocks, j, 1, st)
-              : mt::launch_efield_p(d, ns, pre, nblocks, j, 1, st);
+    if (f.kind == 2)
+        e = h ? mt::launch_hfield_z(d, ns, f.pre, f.nblocks, j, 1, st)
+              : mt::launch_efield_z(d, ns, f.pre, f.nblocks, j, 1, st);
+    else if (f.kind == 1)
+        e = h ? mt::launch_hfield_p(d, ns, f.pre, f.nblocks, j, 1, st)
+              : mt::launch_efield_p(d, ns, f.pre, f.nblocks, j, 1, st);
     else
-        e = h ? mt::launch_hfield_f(d, ns, pre, nblocks, j, 1, st)
-              : mt::launch_efield_f(d, ns, pre, nblocks, j, 1, st);
+        e = h ? mt::launch_hfield_f(d, ns, f.pre, f.nblocks, j, 1, st)
+              : mt::launch_efield_f(d, ns, f.pre, f.nblocks, j, 1, st);
     HIP_TRY(e);
+    if (f.kind == 3)
+        HIP_TRY(h ? mt::launch_hfield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st)
+                  : mt::launch_efield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st));
     return FWI3D_OK;
 }
 
-int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const int *pre_cells,
-                 int ncellblocks, int L, bool any_dft, bool any_rec, int kind, hipStream_t st)
+// launches for local steps j in [0, L) followed by the DFT flush and step advance
+int launch_chunk(const ShotDev *d, int ns, const Fam &f, const int *pre_cells, int ncellblocks,
+                 int L, bool any_dft, bool any_rec, hipStream_t st)
 {
     int rc;
     for (int j = 0; j < L; ++j) {
-        if ((rc = launch_eh(kind, d, ns, pre, nblocks, j, false, st)) ||
-            (rc = launch_eh(kind, d, ns, pre, nblocks, j, true, st)))
+        if ((rc = launch_eh(f, d, ns, j, false, st)) || (rc = launch_eh(f, d, ns, j, true, st)))
             return rc;
         if (any_rec)
             HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
@@ -281,14 +295,13 @@ int launch_chunk(const ShotDev *d, int ns, const int *pre, int nblocks, const in
     return FWI3D_OK;
 }
 
-int capture_chunk(hipGraphExec_t *out, const ShotDev *d, int ns, const int *pre, int nblocks,
+int capture_chunk(hipGraphExec_t *out, const ShotDev *d, int ns, const Fam &f,
                   const int *pre_cells, int ncellblocks, int L, bool any_dft, bool any_rec,
-                  int kind, hipStream_t st)
+                  hipStream_t st)
 {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = launch_chunk(d, ns, pre, nblocks, pre_cells, ncellblocks, L, any_dft, any_rec, kind,
-                          st);
+    int rc = launch_chunk(d, ns, f, pre_cells, ncellblocks, L, any_dft, any_rec, st);
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc)
         return rc;
@@ -601,6 +614,48 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.step = s->d_step;
     s->nblocks_op = d.nbx * d.nby * p.iz;
     s->kind = sp->zchunk > 0 ? 2 : d.pair;
+    // family 3: LDS z-marching kernels over the CPML-free interior (rows start
+    // 128-byte aligned, at ncpml rounded up to 16 cells), per-cell kernels over
+    // the tiles that hold any other cell
+    d.shell = nullptr;
+    const int xlo = (p.ncpml + 15) / 16 * 16;
+    if (s->kind == 0 && p.ncpml >= 2 && xlo < p.ix - p.ncpml && p.ncpml < p.iy - p.ncpml &&
+        p.ncpml < p.iz - p.ncpml && !env_pad("MITTET_FDTD_NOLDS", 0)) {
+        const int lo[3] = {xlo, p.ncpml, p.ncpml};
+        const int hi[3] = {p.ix - p.ncpml, p.iy - p.ncpml, p.iz - p.ncpml};
+        for (int a = 0; a < 3; ++a) {
+            d.lo[a] = lo[a];
+            d.hi[a] = hi[a];
+        }
+        d.ltx = tiles(hi[0] - lo[0], mt::LX);
+        d.lty = tiles(hi[1] - lo[1], mt::LY);
+        d.lkz = tiles(hi[2] - lo[2], mt::LKB);
+        d.lb = d.ltx * d.lty * d.lkz;
+        d.lb_pad = tiles(d.lb, 8) * 8;
+        std::vector<int> shell;
+        for (int k = 0; k < p.iz; ++k)
+            for (int ty = 0; ty < d.fty; ++ty)
+                for (int tx = 0; tx < d.ftx; ++tx) {
+                    const int x0 = tx * mt::ZX, x1 = std::min(x0 + mt::ZX, p.ix);
+                    const int y0 = ty * mt::ZY, y1 = std::min(y0 + mt::ZY, p.iy);
+                    const bool inside = k >= lo[2] && k < hi[2] && x0 >= lo[0] && x1 <= hi[0] &&
+                                        y0 >= lo[1] && y1 <= hi[1];
+                    if (!inside)
+                        shell.push_back((k * d.fty + ty) * d.ftx + tx);
+                }
+        d.fb = (int)shell.size();
+        d.fb_pad = tiles(d.fb, 8) * 8;
+        shell.resize(d.fb_pad, -1);
+        if (dalloc(&s->d_shell, shell.size()) != hipSuccess ||
+            hipMemcpy(s->d_shell, shell.data(), sizeof(int) * shell.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            fwi3d_shot_destroy(s);
+            return fail(FWI3D_ENOMEM, "shell tile list");
+        }
+        d.shell = s->d_shell;
+        s->kind = 3;
+        s->nblocks_l = d.lb_pad;
+    }
     s->nblocks = s->kind == 2 ? d.zb_pad : d.fb_pad;
     s->ncellblocks = tiles((int)N, 256);
     if ((rc = upload_desc(s)) || (rc = fwi3d_set_zero_eh(s))) {
@@ -632,6 +687,7 @@ int fwi3d_shot_destroy(fwi3d_shot *s)
     dfree(s->d_rec);
     dfree(s->d_step);
     dfree(s->d_pre);
+    dfree(s->d_shell);
     dfree(s->d_desc);
     if (s->m)
         s->m->nshots--;
@@ -712,6 +768,11 @@ int fwi3d_hpml_abcs4(fwi3d_shot *s) { return single_op(s, 3, 1, 0); }
 int fwi3d_read_source_3d(fwi3d_shot *s, int step) { return single_op(s, 4, 1, step); }
 int fwi3d_output(fwi3d_shot *s, int step) { return single_op(s, 5, 1, step); }
 
+static Fam shot_fam(const fwi3d_shot *s)
+{
+    return {s->kind, s->d_pre, s->nblocks, s->d_pre, s->nblocks_l};
+}
+
 int fwi3d_step_n(fwi3d_shot *s, int nsteps)
 {
     int rc = check_shot_ready(s);
@@ -727,14 +788,13 @@ int fwi3d_step_n(fwi3d_shot *s, int nsteps)
     while (left > 0) {
         const int L = std::min(chunk, left);
         if (L == chunk) {
-            if (!s->graph &&
-                (rc = capture_chunk(&s->graph, s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre,
-                                    s->ncellblocks, L, dft, rec, s->kind, s->st)))
+            if (!s->graph && (rc = capture_chunk(&s->graph, s->d_desc, 1, shot_fam(s), s->d_pre,
+                                                 s->ncellblocks, L, dft, rec, s->st)))
                 return rc;
             HIP_TRY(hipGraphLaunch(s->graph, s->st));
         } else {
-            if ((rc = launch_chunk(s->d_desc, 1, s->d_pre, s->nblocks, s->d_pre, s->ncellblocks, L,
-                                   dft, rec, s->kind, s->st)))
+            if ((rc = launch_chunk(s->d_desc, 1, shot_fam(s), s->d_pre, s->ncellblocks, L, dft, rec,
+                                   s->st)))
                 return rc;
         }
         s->step += L;
@@ -780,13 +840,13 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
         const int L = std::min(chunk, left);
         for (int j = 0; j < L && !rc; ++j) {
             rc = timed(0, [&] {
-                return launch_eh(s->kind, s->d_desc, 1, s->d_pre, s->nblocks, j, false, s->st)
-                           ? hipErrorUnknown : hipSuccess;
+                return launch_eh(shot_fam(s), s->d_desc, 1, j, false, s->st) ? hipErrorUnknown
+                                                                             : hipSuccess;
             });
             if (!rc)
                 rc = timed(1, [&] {
-                    return launch_eh(s->kind, s->d_desc, 1, s->d_pre, s->nblocks, j, true, s->st)
-                               ? hipErrorUnknown : hipSuccess;
+                    return launch_eh(shot_fam(s), s->d_desc, 1, j, true, s->st) ? hipErrorUnknown
+                                                                                : hipSuccess;
                 });
             if (!rc && s->nrec)
                 rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
@@ -1691,6 +1751,7 @@ int fwi3d_batch_create(int max_shots, int chunk, void *stream, fwi3d_batch **out
     }
     if (dalloc(&b->d_desc, (size_t)max_shots) != hipSuccess ||
         dalloc(&b->d_pre, (size_t)max_shots + 1) != hipSuccess ||
+        dalloc(&b->d_pre_l, (size_t)max_shots + 1) != hipSuccess ||
         dalloc(&b->d_pre_cells, (size_t)max_shots + 1) != hipSuccess) {
         fwi3d_batch_destroy(b);
         return fail(FWI3D_ENOMEM, "batch descriptors");
@@ -1711,6 +1772,7 @@ int fwi3d_batch_destroy(fwi3d_batch *b)
         (void)hipGraphExecDestroy(b->graph);
     dfree(b->d_desc);
     dfree(b->d_pre);
+    dfree(b->d_pre_l);
     dfree(b->d_pre_cells);
     if (b->own_stream && b->st)
         (void)hipStreamDestroy(b->st);
@@ -1728,10 +1790,13 @@ int fwi3d_batch_add(fwi3d_batch *b, fwi3d_shot *s)
         return fail(FWI3D_EBUDGET, "batch holds at most %d shots", b->max_shots);
     if (s->ring != 1 && s->ring != b->chunk)
         return fail(FWI3D_EINVAL, "shot ring %d must equal the batch chunk %d", s->ring, b->chunk);
-    if (!b->shots.empty() && s->kind != b->kind)
+    // families 0 and 3 share launches (a family-0 member has no LDS blocks and
+    // no shell list); pairs and z-marching need members of their own family
+    const bool mix03 = (s->kind == 0 || s->kind == 3) && (b->kind == 0 || b->kind == 3);
+    if (!b->shots.empty() && s->kind != b->kind && !mix03)
         return fail(FWI3D_EINVAL, "batch members must use the same fused kernel family "
-                                  "(even/odd ix, zchunk)");
-    b->kind = s->kind;
+                                  "(x-pairs, zchunk)");
+    b->kind = b->shots.empty() ? s->kind : std::max(b->kind, s->kind);
     int rc = ensure_amp(s);
     if (rc)
         return rc;
@@ -1761,19 +1826,23 @@ static int batch_upload(fwi3d_batch *b)
 {
     const int ns = (int)b->shots.size();
     std::vector<ShotDev> descs(ns);
-    std::vector<int> pre(ns + 1, 0), prec(ns + 1, 0);
+    std::vector<int> pre(ns + 1, 0), prel(ns + 1, 0), prec(ns + 1, 0);
     for (int q = 0; q < ns; ++q) {
         descs[q] = b->shots[q]->desc;
         pre[q + 1] = pre[q] + b->shots[q]->nblocks;
+        prel[q + 1] = prel[q] + b->shots[q]->nblocks_l;
         prec[q + 1] = prec[q] + b->shots[q]->ncellblocks;
     }
     b->nblocks = pre[ns];
+    b->nblocks_l = prel[ns];
     b->ncellblocks = prec[ns];
     if (ns) {
         HIP_TRY(hipMemcpyAsync(b->d_desc, descs.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice,
                                b->st));
         HIP_TRY(hipMemcpyAsync(b->d_pre, pre.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice,
                                b->st));
+        HIP_TRY(hipMemcpyAsync(b->d_pre_l, prel.data(), sizeof(int) * (ns + 1),
+                               hipMemcpyHostToDevice, b->st));
         HIP_TRY(hipMemcpyAsync(b->d_pre_cells, prec.data(), sizeof(int) * (ns + 1),
                                hipMemcpyHostToDevice, b->st));
         HIP_TRY(hipStreamSynchronize(b->st));
@@ -1784,6 +1853,11 @@ static int batch_upload(fwi3d_batch *b)
     }
     b->dirty = false;
     return FWI3D_OK;
+}
+
+static Fam batch_fam(const fwi3d_batch *b)
+{
+    return {b->kind, b->d_pre, b->nblocks, b->d_pre_l, b->nblocks_l};
 }
 
 int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
@@ -1806,13 +1880,13 @@ int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
     while (left > 0) {
         const int L = std::min(b->chunk, left);
         if (L == b->chunk) {
-            if (!b->graph && (rc = capture_chunk(&b->graph, b->d_desc, ns, b->d_pre, b->nblocks,
+            if (!b->graph && (rc = capture_chunk(&b->graph, b->d_desc, ns, batch_fam(b),
                                                  b->d_pre_cells, b->ncellblocks, L, dft, rec,
-                                                 b->kind, b->st)))
+                                                 b->st)))
                 return rc;
             HIP_TRY(hipGraphLaunch(b->graph, b->st));
-        } else if ((rc = launch_chunk(b->d_desc, ns, b->d_pre, b->nblocks, b->d_pre_cells,
-                                      b->ncellblocks, L, dft, rec, b->kind, b->st))) {
+        } else if ((rc = launch_chunk(b->d_desc, ns, batch_fam(b), b->d_pre_cells,
+                                      b->ncellblocks, L, dft, rec, b->st))) {
             return rc;
         }
         left -= L;

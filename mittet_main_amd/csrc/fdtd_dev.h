@@ -18,6 +18,8 @@ constexpr int TY = 4;    // rows per block
 constexpr int RING_MAX = 32;
 constexpr int ZX = 32;   // z-marching tile: 32 x 8 columns, one thread per column
 constexpr int ZY = 8;
+// LDS z-marching kernels of the CPML-free interior: 64 x 4 tiles, 8-plane chunks
+constexpr int LX = 64, LY = 4, LKB = 8;
 
 // Everything a kernel needs about one shot.  Lives in device memory; blocks
 // find their shot through an exclusive prefix of per-shot block counts.
@@ -37,6 +39,11 @@ struct ShotDev {
     int fb, fb_pad;           // per-cell fused blocks (one plane each), padded to 8
     int ftx, fty;             // per-cell fused tiles along x and y
     int pair;                 // 1: fused kernels update x-pairs of cells (even ix)
+    const int *shell;         // tiles L of the per-cell kernels when the LDS kernels cover the
+                              // interior (fb entries, -1 padded); null: every tile
+    int lo[3], hi[3];         // LDS-kernel region [lo, hi) per axis (inside the CPML-free interior)
+    int ltx, lty, lkz;        // LDS-kernel tiles along x, y and z-chunks
+    int lb, lb_pad;           // LDS-kernel blocks, padded to a multiple of 8
     double *e;                // E ring: slot q, comp d at e + q*sstride + d*n
     long long sstride;        // doubles between ring slots (3n + padding)
     long long astride;        // complex elements between accumulator rows (n + padding)
@@ -101,6 +108,10 @@ hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks
 hipError_t launch_efield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_hfield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
+hipError_t launch_efield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           int sign, hipStream_t st);
+hipError_t launch_hfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
                             hipStream_t st);

@@ -950,15 +950,22 @@ hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks
 
 __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, int &k)
 {
-    const int L = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
-    if (L >= S.fb)
+    const int q = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
+    if (q >= S.fb)
+        return false;
+    const int L = S.shell ? gc(S.shell)[q] : q;
+    if (L < 0)
         return false;
     const int tx = L % S.ftx, r = L / S.ftx;
     const int ty = r % S.fty;
     k = r / S.fty;
     i = tx * ZX + (threadIdx.x & (ZX - 1));
     jj = ty * ZY + threadIdx.x / ZX;
-    return i < S.ix && jj < S.iy;
+    if (i >= S.ix || jj >= S.iy)
+        return false;
+    // cells of the LDS kernels' region are theirs
+    return !(S.shell && i >= S.lo[0] && i < S.hi[0] && jj >= S.lo[1] && jj < S.hi[1] &&
+             k >= S.lo[2] && k < S.hi[2]);
 }
 
 __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__ D, int ns,
@@ -1091,6 +1098,313 @@ hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks
     if (nblocks <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_hfield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
+}
+
+
+// ------------------------------------------------------------------------
+// LDS z-marching kernels of the CPML-free interior.  A block owns a 64 x 4
+// column of the shot's LDS region and marches LKB planes along z: the x and y
+// stencil neighbours of a plane come from LDS tiles that the block fills
+// once per plane, the z neighbours from a register queue along each lane's
+// own column, so every plane is read from HBM once per column (plus halo)
+// instead of once per stencil point.  The tiles of the next plane are
+// fetched into registers while the current plane is computed; one barrier per
+// plane (the LDS ring holds the planes a slow lane may still read).  The
+// arithmetic is the per-cell kernels' fast path, term for term.
+
+namespace {
+
+constexpr int LNT = LX * LY;
+
+// tile of LX x LY cells with a halo of XL/XR cells before/after along x and
+// YL/YR along y
+template <int XL, int XR, int YL, int YR> struct LTile {
+    static constexpr int W = LX + XL + XR, H = LY + YL + YR, N = W * H;
+    static constexpr int R = (N + LNT - 1) / LNT;   // elements per lane
+};
+
+template <class T, int XL, int YL>
+__device__ __forceinline__ void tile_load(const GLOBAL double *f, size_t pl, int sx, int x0,
+                                          int y0, int xmax, int ymax, double (&r)[T::R])
+{
+#pragma unroll
+    for (int q = 0; q < T::R; ++q) {
+        const int e = threadIdx.x + q * LNT;
+        double v = 0.0;
+        if (e < T::N) {
+            const int gy = y0 - YL + e / T::W, gx = x0 - XL + e % T::W;
+            if (gx <= xmax && gy <= ymax)
+                v = f[pl + (size_t)gy * sx + gx];
+        }
+        r[q] = v;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void tile_store(double *lds, const double (&r)[T::R])
+{
+#pragma unroll
+    for (int q = 0; q < T::R; ++q) {
+        const int e = threadIdx.x + q * LNT;
+        if (e < T::N)
+            lds[e] = r[q];
+    }
+}
+
+// block -> (shot region tile, z-chunk); false for padding blocks
+__device__ __forceinline__ bool lcol(const ShotDev &S, int b, int &x0, int &y0, int &k0, int &k1)
+{
+    const int L = (b & 7) * (S.lb_pad >> 3) + (b >> 3);
+    if (L >= S.lb)
+        return false;
+    const int tx = L % S.ltx, r = L / S.ltx;
+    const int ty = r % S.lty, kz = r / S.lty;
+    x0 = S.lo[0] + tx * LX;
+    y0 = S.lo[1] + ty * LY;
+    k0 = S.lo[2] + kz * LKB;
+    k1 = min(k0 + LKB, S.hi[2]);
+    return true;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D, int ns,
+                                                  const int *__restrict__ pre, int j, int sign)
+{
+    // Hz: x -2..+1, y -2..+1; Hy: x -2..+1; Hx: y -2..+1
+    using TZ = LTile<2, 1, 2, 1>;
+    using TY = LTile<2, 1, 0, 0>;
+    using TX = LTile<0, 0, 2, 1>;
+    __shared__ double sz[3][TZ::N], sy[3][TY::N], sx[3][TX::N];
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    const int n = *S.step + j;
+    if (n >= S.it - 1)
+        return;
+    int x0, y0, k0, k1;
+    if (!lcol(S, blockIdx.x - pre[s], x0, y0, k0, k1))
+        return;
+    const int lx = threadIdx.x % LX, ly = threadIdx.x / LX;
+    const int i = x0 + lx, jj = y0 + ly;
+    const bool valid = i < S.hi[0] && jj < S.hi[1];
+    // idle lanes of a partial tile read their row's last cell (stays in bounds)
+    const int ci = min(i, S.hi[0] - 1), cj = min(jj, S.hi[1] - 1);
+    const int ix = S.ix, iy = S.iy;
+    const size_t N = (size_t)S.n, pl = (size_t)ix * iy;
+    const int xmax = min(x0 + LX, S.hi[0]), ymax = min(y0 + LY, S.hi[1]);
+    const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
+    const GLOBAL double *Ein = gc(S.e + (size_t)si * S.sstride);
+    GLOBAL double *Eout = gm(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *Hx = gc(S.h), *Hy = gc(S.h + N), *Hz = gc(S.h + 2 * N);
+    const GLOBAL double *CB = gc(S.cbi);
+    const double kedx = gc(S.prof[0])[PR_KED * ix + ci];
+    const double kedy = gc(S.prof[1])[PR_KED * iy + cj];
+    const double c1 = S.c1, c2 = S.c2;
+    const size_t col = (size_t)cj * ix + ci;
+    double yq2 = Hy[(size_t)(k0 - 2) * pl + col], yq1 = Hy[(size_t)(k0 - 1) * pl + col];
+    double xq2 = Hx[(size_t)(k0 - 2) * pl + col], xq1 = Hx[(size_t)(k0 - 1) * pl + col];
+    double rz[TZ::R], ry[TY::R], rx[TX::R];
+    int b0 = 0, b1 = 1, b2 = 2;   // LDS buffers of planes k, k+1 and the one being filled
+    tile_load<TZ, 2, 2>(Hz, (size_t)k0 * pl, ix, x0, y0, xmax, ymax, rz);
+    tile_load<TY, 2, 0>(Hy, (size_t)k0 * pl, ix, x0, y0, xmax, ymax, ry);
+    tile_load<TX, 0, 2>(Hx, (size_t)k0 * pl, ix, x0, y0, xmax, ymax, rx);
+    tile_store<TZ>(sz[b0], rz);
+    tile_store<TY>(sy[b0], ry);
+    tile_store<TX>(sx[b0], rx);
+    tile_load<TZ, 2, 2>(Hz, (size_t)(k0 + 1) * pl, ix, x0, y0, xmax, ymax, rz);
+    tile_load<TY, 2, 0>(Hy, (size_t)(k0 + 1) * pl, ix, x0, y0, xmax, ymax, ry);
+    tile_load<TX, 0, 2>(Hx, (size_t)(k0 + 1) * pl, ix, x0, y0, xmax, ymax, rx);
+    size_t c = (size_t)k0 * pl + col;
+    double ex0 = Ein[c], ey0 = Ein[N + c], ez0 = Ein[2 * N + c], cb = CB[c];
+    for (int k = k0; k < k1; ++k) {
+        tile_store<TZ>(sz[b1], rz);
+        tile_store<TY>(sy[b1], ry);
+        tile_store<TX>(sx[b1], rx);
+        __syncthreads();
+        const bool more = k + 1 < k1;
+        const size_t cn = c + pl;
+        double nex = 0.0, ney = 0.0, nez = 0.0, ncb = 0.0;
+        if (more) {
+            tile_load<TZ, 2, 2>(Hz, (size_t)(k + 2) * pl, ix, x0, y0, xmax, ymax, rz);
+            tile_load<TY, 2, 0>(Hy, (size_t)(k + 2) * pl, ix, x0, y0, xmax, ymax, ry);
+            tile_load<TX, 0, 2>(Hx, (size_t)(k + 2) * pl, ix, x0, y0, xmax, ymax, rx);
+            nex = Ein[cn];
+            ney = Ein[N + cn];
+            nez = Ein[2 * N + cn];
+            ncb = CB[cn];
+        }
+        const double kedz = gc(S.prof[2])[PR_KED * S.iz + k];
+        const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
+        const int zc = (ly + 2) * TZ::W + lx + 2;
+        const double z0 = Z[zc], zym1 = Z[zc - TZ::W], zyp1 = Z[zc + TZ::W], zym2 = Z[zc - 2 * TZ::W];
+        const double zxm1 = Z[zc - 1], zxp1 = Z[zc + 1], zxm2 = Z[zc - 2];
+        const int yc = ly * TY::W + lx + 2;
+        const double y0v = Y[yc], yxm1 = Y[yc - 1], yxp1 = Y[yc + 1], yxm2 = Y[yc - 2];
+        const double yzp1 = sy[b1][yc];
+        const int xc = (ly + 2) * TX::W + lx;
+        const double x0v = X[xc], xym1 = X[xc - TX::W], xyp1 = X[xc + TX::W], xym2 = X[xc - 2 * TX::W];
+        const double xzp1 = sx[b1][xc];
+        const double hzy = c1 * z0 - c1 * zym1 + c2 * zyp1 - c2 * zym2;
+        const double hyz = c1 * y0v - c1 * yq1 + c2 * yzp1 - c2 * yq2;
+        const double hxz = c1 * x0v - c1 * xq1 + c2 * xzp1 - c2 * xq2;
+        const double hzx = c1 * z0 - c1 * zxm1 + c2 * zxp1 - c2 * zxm2;
+        const double hyx = c1 * y0v - c1 * yxm1 + c2 * yxp1 - c2 * yxm2;
+        const double hxy = c1 * x0v - c1 * xym1 + c2 * xyp1 - c2 * xym2;
+        const double cux = hzy / kedy - hyz / kedz;
+        const double cuy = hxz / kedz - hzx / kedx;
+        const double cuz = hyx / kedx - hxy / kedy;
+        if (valid) {
+            double ex = sign > 0 ? ex0 + cb * cux : ex0 - cb * cux;
+            double ey = sign > 0 ? ey0 + cb * cuy : ey0 - cb * cuy;
+            double ez = sign > 0 ? ez0 + cb * cuz : ez0 - cb * cuz;
+            if ((long long)c == S.src_cell) {
+                const double a = gc(S.amp)[n];
+                if (S.src_comp == 0)
+                    ex -= a;
+                else if (S.src_comp == 1)
+                    ey -= a;
+                else
+                    ez -= a;
+            }
+            Eout[c] = ex;
+            Eout[N + c] = ey;
+            Eout[2 * N + c] = ez;
+        }
+        yq2 = yq1;
+        yq1 = y0v;
+        xq2 = xq1;
+        xq1 = x0v;
+        ex0 = nex;
+        ey0 = ney;
+        ez0 = nez;
+        cb = ncb;
+        c = cn;
+        const int t = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = t;
+    }
+}
+
+__global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D, int ns,
+                                                  const int *__restrict__ pre, int j, int sign)
+{
+    // Ez: x -1..+2, y -1..+2; Ey: x -1..+2; Ex: y -1..+2
+    using TZ = LTile<1, 2, 1, 2>;
+    using TY = LTile<1, 2, 0, 0>;
+    using TX = LTile<0, 0, 1, 2>;
+    __shared__ double sz[4][TZ::N], sy[4][TY::N], sx[4][TX::N];
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    if (*S.step + j >= S.it - 1)
+        return;
+    int x0, y0, k0, k1;
+    if (!lcol(S, blockIdx.x - pre[s], x0, y0, k0, k1))
+        return;
+    const int lx = threadIdx.x % LX, ly = threadIdx.x / LX;
+    const int i = x0 + lx, jj = y0 + ly;
+    const bool valid = i < S.hi[0] && jj < S.hi[1];
+    const int ci = min(i, S.hi[0] - 1), cj = min(jj, S.hi[1] - 1);
+    const int ix = S.ix, iy = S.iy;
+    const size_t N = (size_t)S.n, pl = (size_t)ix * iy;
+    const int xmax = min(x0 + LX, S.hi[0]) + 1, ymax = min(y0 + LY, S.hi[1]) + 1;
+    const int so = (S.cur + j + 1) % S.ring;
+    const GLOBAL double *Ex = gc(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *Ey = Ex + N, *Ez = Ex + 2 * N;
+    GLOBAL double *Hx = gm(S.h), *Hy = gm(S.h + N), *Hz = gm(S.h + 2 * N);
+    const double khdx = gc(S.prof[0])[PR_KHD * ix + ci];
+    const double khdy = gc(S.prof[1])[PR_KHD * iy + cj];
+    const double c1 = S.c1, c2 = S.c2;
+    const double db = S.dt / S.mu0 / 1.0;
+    const size_t col = (size_t)cj * ix + ci;
+    double yq1 = Ey[(size_t)(k0 - 1) * pl + col], xq1 = Ex[(size_t)(k0 - 1) * pl + col];
+    double rz[TZ::R], ry[TY::R], rx[TX::R];
+    int b0 = 0, b1 = 1, b2 = 2, b3 = 3;   // planes k, k+1, k+2 (being filled), free
+    for (int p = 0; p < 2; ++p) {
+        const int bp = p == 0 ? b0 : b1;
+        tile_load<TZ, 1, 1>(Ez, (size_t)(k0 + p) * pl, ix, x0, y0, xmax, ymax, rz);
+        tile_load<TY, 1, 0>(Ey, (size_t)(k0 + p) * pl, ix, x0, y0, xmax, ymax, ry);
+        tile_load<TX, 0, 1>(Ex, (size_t)(k0 + p) * pl, ix, x0, y0, xmax, ymax, rx);
+        tile_store<TZ>(sz[bp], rz);
+        tile_store<TY>(sy[bp], ry);
+        tile_store<TX>(sx[bp], rx);
+    }
+    tile_load<TZ, 1, 1>(Ez, (size_t)(k0 + 2) * pl, ix, x0, y0, xmax, ymax, rz);
+    tile_load<TY, 1, 0>(Ey, (size_t)(k0 + 2) * pl, ix, x0, y0, xmax, ymax, ry);
+    tile_load<TX, 0, 1>(Ex, (size_t)(k0 + 2) * pl, ix, x0, y0, xmax, ymax, rx);
+    size_t c = (size_t)k0 * pl + col;
+    double hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
+    for (int k = k0; k < k1; ++k) {
+        tile_store<TZ>(sz[b2], rz);
+        tile_store<TY>(sy[b2], ry);
+        tile_store<TX>(sx[b2], rx);
+        __syncthreads();
+        const bool more = k + 1 < k1;
+        const size_t cn = c + pl;
+        double nhx = 0.0, nhy = 0.0, nhz = 0.0;
+        if (more) {
+            tile_load<TZ, 1, 1>(Ez, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, rz);
+            tile_load<TY, 1, 0>(Ey, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, ry);
+            tile_load<TX, 0, 1>(Ex, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, rx);
+            nhx = Hx[cn];
+            nhy = Hy[cn];
+            nhz = Hz[cn];
+        }
+        const double khdz = gc(S.prof[2])[PR_KHD * S.iz + k];
+        const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
+        const int zc = (ly + 1) * TZ::W + lx + 1;
+        const double z0 = Z[zc], zyp1 = Z[zc + TZ::W], zyp2 = Z[zc + 2 * TZ::W], zym1 = Z[zc - TZ::W];
+        const double zxp1 = Z[zc + 1], zxp2 = Z[zc + 2], zxm1 = Z[zc - 1];
+        const int yc = ly * TY::W + lx + 1;
+        const double y0v = Y[yc], yxp1 = Y[yc + 1], yxp2 = Y[yc + 2], yxm1 = Y[yc - 1];
+        const double yzp1 = sy[b1][yc], yzp2 = sy[b2][yc];
+        const int xc = (ly + 1) * TX::W + lx;
+        const double x0v = X[xc], xyp1 = X[xc + TX::W], xyp2 = X[xc + 2 * TX::W], xym1 = X[xc - TX::W];
+        const double xzp1 = sx[b1][xc], xzp2 = sx[b2][xc];
+        const double ezy = c1 * zyp1 - c1 * z0 + c2 * zyp2 - c2 * zym1;
+        const double eyz = c1 * yzp1 - c1 * y0v + c2 * yzp2 - c2 * yq1;
+        const double exz = c1 * xzp1 - c1 * x0v + c2 * xzp2 - c2 * xq1;
+        const double ezx = c1 * zxp1 - c1 * z0 + c2 * zxp2 - c2 * zxm1;
+        const double eyx = c1 * yxp1 - c1 * y0v + c2 * yxp2 - c2 * yxm1;
+        const double exy = c1 * xyp1 - c1 * x0v + c2 * xyp2 - c2 * xym1;
+        const double cux = ezy / khdy - eyz / khdz;
+        const double cuy = exz / khdz - ezx / khdx;
+        const double cuz = eyx / khdx - exy / khdy;
+        if (valid) {
+            Hx[c] = sign > 0 ? hx0 - db * cux : hx0 + db * cux;
+            Hy[c] = sign > 0 ? hy0 - db * cuy : hy0 + db * cuy;
+            Hz[c] = sign > 0 ? hz0 - db * cuz : hz0 + db * cuz;
+        }
+        yq1 = y0v;
+        xq1 = x0v;
+        hx0 = nhx;
+        hy0 = nhy;
+        hz0 = nhz;
+        c = cn;
+        const int t = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = t;
+    }
+}
+
+hipError_t launch_efield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_efield_l, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_hfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
+                           hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_hfield_l, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
     return hipGetLastError();
 }
 
