@@ -182,7 +182,7 @@ struct fwi3d_shot {
     int kind = 0;          // fused kernel family (see launch_eh)
     int nblocks = 0;       // fused blocks (padded to a multiple of 8)
     int nblocks_l = 0;     // LDS-kernel blocks of family 3 (padded to a multiple of 8)
-    int *d_shell = nullptr;  // family 3: tiles of the per-cell kernels
+    int4 *d_shell = nullptr;  // family 3: tiles of the per-cell kernels
     int nblocks_op = 0;    // per-cell blocks of the stand-alone operators
     int ncellblocks = 0;
     int step = 0;          // steps done (host mirror of *d_step)
@@ -632,22 +632,34 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
         d.lkz = tiles(hi[2] - lo[2], mt::LKB);
         d.lb = d.ltx * d.lty * d.lkz;
         d.lb_pad = tiles(d.lb, 8) * 8;
-        std::vector<int> shell;
-        for (int k = 0; k < p.iz; ++k)
-            for (int ty = 0; ty < d.fty; ++ty)
-                for (int tx = 0; tx < d.ftx; ++tx) {
-                    const int x0 = tx * mt::ZX, x1 = std::min(x0 + mt::ZX, p.ix);
-                    const int y0 = ty * mt::ZY, y1 = std::min(y0 + mt::ZY, p.iy);
-                    const bool inside = k >= lo[2] && k < hi[2] && x0 >= lo[0] && x1 <= hi[0] &&
-                                        y0 >= lo[1] && y1 <= hi[1];
-                    if (!inside)
-                        shell.push_back((k * d.fty + ty) * d.ftx + tx);
-                }
+        // planes outside the region and rows of 32 x 8 tiles not inside it in y:
+        // every tile; rows inside it in y: 16 x 16 tiles over the two x-edge
+        // segments [0, lo_x) and [hi_x, ix) only
+        std::vector<int4> shell;
+        const int ya = tiles(lo[1], mt::ZY), yb = hi[1] / mt::ZY;   // tile rows inside [lo_y, hi_y)
+        for (int k = 0; k < p.iz; ++k) {
+            const bool inz = k >= lo[2] && k < hi[2];
+            for (int ty = 0; ty < d.fty; ++ty) {
+                if (inz && ty >= ya && ty < yb)
+                    continue;
+                for (int tx = 0; tx < d.ftx; ++tx)
+                    shell.push_back(int4{tx * mt::ZX, ty * mt::ZY, k, 0});
+            }
+            if (!inz || ya >= yb)
+                continue;
+            const int yend = yb * mt::ZY;
+            for (int y0 = ya * mt::ZY; y0 < yend; y0 += 16) {
+                for (int x0 = 0; x0 < lo[0]; x0 += 16)
+                    shell.push_back(int4{x0, y0, k, yend});
+                for (int x0 = hi[0]; x0 < p.ix; x0 += 16)
+                    shell.push_back(int4{x0, y0, k, yend});
+            }
+        }
         d.fb = (int)shell.size();
         d.fb_pad = tiles(d.fb, 8) * 8;
-        shell.resize(d.fb_pad, -1);
+        shell.resize(d.fb_pad, int4{0, 0, -1, 0});
         if (dalloc(&s->d_shell, shell.size()) != hipSuccess ||
-            hipMemcpy(s->d_shell, shell.data(), sizeof(int) * shell.size(), hipMemcpyHostToDevice) !=
+            hipMemcpy(s->d_shell, shell.data(), sizeof(int4) * shell.size(), hipMemcpyHostToDevice) !=
                 hipSuccess) {
             fwi3d_shot_destroy(s);
             return fail(FWI3D_ENOMEM, "shell tile list");

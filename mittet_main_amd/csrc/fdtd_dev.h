@@ -39,8 +39,10 @@ struct ShotDev {
     int fb, fb_pad;           // per-cell fused blocks (one plane each), padded to 8
     int ftx, fty;             // per-cell fused tiles along x and y
     int pair;                 // 1: fused kernels update x-pairs of cells (even ix)
-    const int *shell;         // tiles L of the per-cell kernels when the LDS kernels cover the
-                              // interior (fb entries, -1 padded); null: every tile
+    const int4 *shell;        // tiles of the per-cell kernels when the LDS kernels cover the
+                              // interior: {x0, y0, k, yend} (yend 0: a 32 x 8 tile; else a
+                              // 16 x 16 tile with rows below yend), fb entries then k = -1
+                              // padding; null: every 32 x 8 tile
     int lo[3], hi[3];         // LDS-kernel region [lo, hi) per axis (inside the CPML-free interior)
     int ltx, lty, lkz;        // LDS-kernel tiles along x, y and z-chunks
     int lb, lb_pad;           // LDS-kernel blocks, padded to a multiple of 8

@@ -953,19 +953,31 @@ __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, 
     const int q = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
     if (q >= S.fb)
         return false;
-    const int L = S.shell ? gc(S.shell)[q] : q;
-    if (L < 0)
-        return false;
-    const int tx = L % S.ftx, r = L / S.ftx;
-    const int ty = r % S.fty;
-    k = r / S.fty;
-    i = tx * ZX + (threadIdx.x & (ZX - 1));
-    jj = ty * ZY + threadIdx.x / ZX;
-    if (i >= S.ix || jj >= S.iy)
+    if (!S.shell) {
+        const int tx = q % S.ftx, r = q / S.ftx;
+        const int ty = r % S.fty;
+        k = r / S.fty;
+        i = tx * ZX + (threadIdx.x & (ZX - 1));
+        jj = ty * ZY + threadIdx.x / ZX;
+        return i < S.ix && jj < S.iy;
+    }
+    const GLOBAL int *t = (const GLOBAL int *)(S.shell + q);   // {x0, y0, k, yend}
+    const int x0 = t[0], y0 = t[1], yend = t[3];
+    k = t[2];
+    if (yend) {   // 16 x 16 tile of an x-edge segment, rows below yend
+        i = x0 + (threadIdx.x & 15);
+        jj = y0 + (threadIdx.x >> 4);
+        if (jj >= yend)
+            return false;
+    } else {
+        i = x0 + (threadIdx.x & (ZX - 1));
+        jj = y0 + threadIdx.x / ZX;
+    }
+    if (k < 0 || i >= S.ix || jj >= S.iy)
         return false;
     // cells of the LDS kernels' region are theirs
-    return !(S.shell && i >= S.lo[0] && i < S.hi[0] && jj >= S.lo[1] && jj < S.hi[1] &&
-             k >= S.lo[2] && k < S.hi[2]);
+    return !(i >= S.lo[0] && i < S.hi[0] && jj >= S.lo[1] && jj < S.hi[1] && k >= S.lo[2] &&
+             k < S.hi[2]);
 }
 
 __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__ D, int ns,
