@@ -205,7 +205,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--workload", default="roofline_256", choices=["roofline_256", "plain_64"])
-    ap.add_argument("--ring", type=int, default=32)
+    ap.add_argument("--ring", type=int, default=64)
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,

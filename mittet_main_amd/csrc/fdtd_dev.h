@@ -15,7 +15,7 @@ enum { PSI_EP = 0, PSI_EM, PSI_HP, PSI_HM, PSI_N };
 
 constexpr int TX = 64;   // threads along x (one wavefront row, 512 B coalesced)
 constexpr int TY = 4;    // rows per block
-constexpr int RING_MAX = 32;
+constexpr int RING_MAX = 64;
 constexpr int ZX = 32;   // z-marching tile: 32 x 8 columns, one thread per column
 constexpr int ZY = 8;
 // LDS z-marching kernels of the CPML-free interior: 64 x 4 tiles, 8-plane chunks
