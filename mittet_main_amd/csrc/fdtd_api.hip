@@ -907,7 +907,7 @@ int fwi3d_diag_copy(long long bytes, int reps, double *ms_out)
         rc = fail(FWI3D_EDEVICE, "fill");
     for (int r = 0; r < reps && !rc; ++r) {
         float t = 0.f;
-        if (hipEventRecord(e0, st) != hipSuccess || mt::launch_copy(a, b, count, st) != hipSuccess ||
+        if (hipEventRecord(e0, st) != hipSuccess || mt::launch_diag_copy(a, b, count, st) != hipSuccess ||
             hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&t, e0, e1) != hipSuccess)
             rc = fail(FWI3D_EDEVICE, "copy launch");
@@ -1156,6 +1156,8 @@ struct fwi3d_grad {
     double *g = nullptr;              // [it][3][nrec] trace sensitivities
     double *rp = nullptr;             // per axis: 1/ked, 1/khd
     int kappa_one = 0;                // ked = khd = dl on every axis
+    hipGraphExec_t replay[2] = {nullptr, nullptr};   // segment replays: full length, last
+    int replay_len[2] = {0, 0};
 };
 
 static void state_layout(fwi3d_grad *g)
@@ -1171,20 +1173,29 @@ static void state_layout(fwi3d_grad *g)
     g->state = o;
 }
 
-// copy the shot's live state to / from a stored state
+// the shot's live state (E slot 0 -- ring 1 -- H, the psi slabs) in the
+// stored-state order
+static mt::StateSegs state_segs(const fwi3d_grad *g)
+{
+    const fwi3d_shot *s = g->s;
+    const size_t N = (size_t)s->desc.n;
+    mt::StateSegs v{};
+    v.live[0] = s->d_e;
+    v.count[0] = 3 * N;
+    v.live[1] = s->d_h;
+    v.count[1] = 3 * N;
+    for (int a = 0; a < 3; ++a) {
+        v.live[2 + a] = s->d_psi[a];
+        v.count[2 + a] = mt::PSI_N * (size_t)s->desc.slab[a];
+    }
+    v.nseg = 5;
+    return v;
+}
+
+// copy the shot's live state to / from a stored state (one launch)
 static int state_copy(fwi3d_grad *g, double *dst_state, bool save)
 {
-    fwi3d_shot *s = g->s;
-    const size_t N = (size_t)s->desc.n;
-    double *E = s->d_e;  // ring == 1: slot 0
-    auto cp = [&](double *live, double *stored, size_t count) -> hipError_t {
-        return save ? hipMemcpyAsync(stored, live, sizeof(double) * count, hipMemcpyDeviceToDevice, s->st)
-                    : hipMemcpyAsync(live, stored, sizeof(double) * count, hipMemcpyDeviceToDevice, s->st);
-    };
-    HIP_TRY(cp(E, dst_state, 3 * N));
-    HIP_TRY(cp(s->d_h, dst_state + g->off_h, 3 * N));
-    for (int a = 0; a < 3; ++a)
-        HIP_TRY(cp(s->d_psi[a], dst_state + g->off_psi[a], mt::PSI_N * (size_t)s->desc.slab[a]));
+    HIP_TRY(mt::launch_state_copy(state_segs(g), dst_state, save, g->s->st));
     return FWI3D_OK;
 }
 
@@ -1193,6 +1204,50 @@ static int set_step(fwi3d_shot *s, int n)
     HIP_TRY(hipMemcpyAsync(s->d_step, &n, sizeof(int), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     s->step = n;
+    return FWI3D_OK;
+}
+
+// L forward steps from the shot's current step, storing the state after step
+// t into buffer slot t + 1; captured once per segment length as a HIP graph
+static int replay_segment(fwi3d_grad *g, int L)
+{
+    fwi3d_shot *s = g->s;
+    const int slot = L == g->seg ? 0 : 1;
+    if (g->replay[slot] && g->replay_len[slot] != L) {
+        (void)hipGraphExecDestroy(g->replay[slot]);
+        g->replay[slot] = nullptr;
+    }
+    if (!g->replay[slot]) {
+        const Fam f = shot_fam(s);
+        const mt::StateSegs segs = state_segs(g);
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+        int rc = FWI3D_OK;
+        for (int j = 0; j < L && !rc; ++j) {
+            if ((rc = launch_eh(f, s->d_desc, 1, j, false, s->st)) ||
+                (rc = launch_eh(f, s->d_desc, 1, j, true, s->st)))
+                break;
+            if (s->nrec && mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st) != hipSuccess)
+                rc = fail(FWI3D_EDEVICE, "receiver launch");
+            else if (mt::launch_state_copy(segs, g->buf + g->state * (j + 1), true, s->st) !=
+                     hipSuccess)
+                rc = fail(FWI3D_EDEVICE, "state copy launch");
+        }
+        if (!rc && mt::launch_advance(s->d_desc, 1, L, s->st) != hipSuccess)
+            rc = fail(FWI3D_EDEVICE, "advance launch");
+        hipError_t e = hipStreamEndCapture(s->st, &graph);
+        if (rc)
+            return rc;
+        if (e != hipSuccess)
+            return fail(FWI3D_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&g->replay[slot], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess)
+            return fail(FWI3D_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        g->replay_len[slot] = L;
+    }
+    HIP_TRY(hipGraphLaunch(g->replay[slot], s->st));
+    s->step += L;
     return FWI3D_OK;
 }
 
@@ -1263,6 +1318,9 @@ int fwi3d_grad_destroy(fwi3d_grad *g)
     dfree(g->grad_src);
     dfree(g->g);
     dfree(g->rp);
+    for (auto &r : g->replay)
+        if (r)
+            (void)hipGraphExecDestroy(r);
     delete g;
     return FWI3D_OK;
 }
@@ -1372,9 +1430,8 @@ int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
         if ((rc = state_copy(g, g->ckpt + g->state * q, false)) || (rc = set_step(s, n0)) ||
             (rc = state_copy(g, g->buf, true)))
             return rc;
-        for (int t = 0; t < L; ++t)
-            if ((rc = fwi3d_step_n(s, 1)) || (rc = state_copy(g, g->buf + g->state * (t + 1), true)))
-                return rc;
+        if ((rc = replay_segment(g, L)))
+            return rc;
         for (int t = L - 1; t >= 0; --t) {
             const int n = n0 + t;
             const double *u0 = g->buf + g->state * t, *u1 = g->buf + g->state * (t + 1);

@@ -127,6 +127,17 @@ hipError_t launch_bin_dft(const double *traces, int nrows, int it, double dt, do
                           int nbins, double *out, hipStream_t st);
 hipError_t launch_fill(double *p, size_t count, double v, hipStream_t st);
 hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t st);
+// one launch that gathers up to 5 segments into a contiguous buffer (save) or
+// scatters the buffer back (restore): segment q is `count[q]` doubles at
+// `live[q]`, stored from offset sum(count[< q])
+struct StateSegs {
+    double *live[5];
+    size_t count[5];
+    int nseg;
+};
+hipError_t launch_state_copy(const StateSegs &segs, double *stored, bool save, hipStream_t st);
+// plain 16-byte copy kernel of fwi3d_diag_copy (even count, 16-byte-aligned buffers)
+hipError_t launch_diag_copy(const double *src, double *dst, size_t count, hipStream_t st);
 hipError_t launch_eps(const double *sig, double *eps, double *cbi, int n, double omega0, double dt,
                       hipStream_t st);
 

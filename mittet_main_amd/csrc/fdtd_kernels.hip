@@ -576,6 +576,14 @@ __global__ void k_diag_copy(const double2 *__restrict__ src, double2 *__restrict
         dst[q] = src[q];
 }
 
+// element copy: any count, any 8-byte-aligned addresses
+__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = src[q];
+}
+
 // eps' = sigma/2/omega0 (satan_i/fwi3d_cpu_function_v1.4.c:335) and dt/eps', the
 // E coefficient cb wherever the CPML conductivity is zero (function.c:415-417
 // with esig = 0 reduces to dt/eps'/1.0).
@@ -1717,6 +1725,49 @@ hipError_t launch_fill(double *p, size_t count, double v, hipStream_t st)
 }
 
 hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t st)
+{
+    if (count == 0)
+        return hipSuccess;
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 262144)
+        blocks = 262144;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, count);
+    return hipGetLastError();
+}
+
+__global__ void k_state_copy(StateSegs segs, double *__restrict__ stored, size_t total, int save)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total;
+         q += (size_t)gridDim.x * blockDim.x) {
+        size_t off = q;
+        int k = 0;
+        while (k + 1 < segs.nseg && off >= segs.count[k]) {
+            off -= segs.count[k];
+            ++k;
+        }
+        if (save)
+            stored[q] = segs.live[k][off];
+        else
+            segs.live[k][off] = stored[q];
+    }
+}
+
+hipError_t launch_state_copy(const StateSegs &segs, double *stored, bool save, hipStream_t st)
+{
+    size_t total = 0;
+    for (int k = 0; k < segs.nseg; ++k)
+        total += segs.count[k];
+    if (total == 0)
+        return hipSuccess;
+    size_t blocks = (total + 255) / 256;
+    if (blocks > 262144)
+        blocks = 262144;
+    hipLaunchKernelGGL(k_state_copy, dim3((unsigned)blocks), dim3(256), 0, st, segs, stored, total,
+                       save ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_copy(const double *src, double *dst, size_t count, hipStream_t st)
 {
     const size_t n2 = count / 2;
     size_t blocks = (n2 + 255) / 256;

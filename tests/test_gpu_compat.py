@@ -38,8 +38,12 @@ def oracle_acc(grid, sig, src, comp, signal, it, omegas):
     return acc
 
 
-def test_compat_iteration_matches_oracle():
-    ix, iy, iz, nc, it = 22, 20, 18, 4, 90
+@pytest.mark.parametrize("dims", [(22, 20, 18), (21, 19, 17)])
+def test_compat_iteration_matches_oracle(dims):
+    """(21, 19, 17) has an odd cell count: every cell, down to the last CPML
+    corner, must come back from the update."""
+    ix, iy, iz = dims
+    nc, it = 4, 90
     start_f, upper_f, iseabed = 1, 4, 9
     rng = np.random.default_rng(12)
     sig_true = np.exp(rng.uniform(np.log(0.2), np.log(2.0), (iz, iy, ix)))
@@ -50,6 +54,7 @@ def test_compat_iteration_matches_oracle():
     signal_true = firstderiv_gauss(it, dt, 2.0)
     sources = np.array([[11, 10, 6], [8, 12, 7]])
     receivers = np.array([[7, 7, 5], [14, 12, 5]])
+    assert sources.max() < min(dims) and receivers.max() < min(dims)
     F = CompatFWI(g, sources, receivers, it, signal_x, signal_true, start_f, upper_f,
                   minval=0.05, maxval=5.0, iseabed=iseabed)
     true_model = F.test_model
@@ -99,6 +104,7 @@ def test_compat_iteration_matches_oracle():
     assert rel(res.sigma_test.ravel(), sig2) <= TOL
     assert abs(res.alpha - alpha) <= TOL * alpha
     assert rel(res.sigma.ravel(), sig_new) <= TOL
+    assert res.sigma.ravel()[-1] == sig0.ravel()[-1]     # CPML corner: untouched
     # the model now holds the updated conductivity: same data as a fresh upload
     after = F.forward_receivers(F.model, sources[0])
     F.test_model.media_coeff_3d(res.sigma)
