@@ -38,33 +38,12 @@ def oracle_acc(grid, sig, src, comp, signal, it, omegas):
     return acc
 
 
-@pytest.mark.parametrize("dims", [(22, 20, 18), (21, 19, 17)])
-def test_compat_iteration_matches_oracle(dims):
-    """(21, 19, 17) has an odd cell count: every cell, down to the last CPML
-    corner, must come back from the update."""
-    ix, iy, iz = dims
-    nc, it = 4, 90
-    start_f, upper_f, iseabed = 1, 4, 9
-    rng = np.random.default_rng(12)
-    sig_true = np.exp(rng.uniform(np.log(0.2), np.log(2.0), (iz, iy, ix)))
-    sig0 = np.full((iz, iy, ix), 0.6)
-    dt, cmax = dt_for(50.0, 45.0, 40.0, sig_min=0.2)
-    g = Grid(ix, iy, iz, nc, 50.0, 45.0, 40.0, dt, OM0, cmax)
-    signal_x = firstderiv_gauss(it, dt, 6.0)
-    signal_true = firstderiv_gauss(it, dt, 2.0)
-    sources = np.array([[11, 10, 6], [8, 12, 7]])
-    receivers = np.array([[7, 7, 5], [14, 12, 5]])
-    assert sources.max() < min(dims) and receivers.max() < min(dims)
-    F = CompatFWI(g, sources, receivers, it, signal_x, signal_true, start_f, upper_f,
-                  minval=0.05, maxval=5.0, iseabed=iseabed)
-    true_model = F.test_model
-    true_model.media_coeff_3d(sig_true)
-    eobs = [F.forward_receivers(true_model, s) for s in sources]
-    res = F.iterate(sig0, eobs)
-
-    # ---- oracle: the same iteration with the reference's loops
-    om = F.omegas
-    lap = oracle.laplace_to_freq(np.zeros(it), signal_x, signal_true, dt, OM0, upper_f)
+def oracle_iteration(g, sig0, sources, receivers, it, signal_x, signal_true, start_f, upper_f,
+                     iseabed, minval, maxval, eobs, om):
+    """One pass of fwi3d_cpu_inv_fwi_v1.4.c:27-109 with the oracle's loops."""
+    iz, iy, ix = sig0.shape
+    nc = g.ncpml
+    lap = oracle.laplace_to_freq(np.zeros(it), signal_x, signal_true, g.dt, g.omega0, upper_f)
     jw = lap["J"][start_f:]
     tw = oracle.fft_backward_real(signal_true)[start_f:upper_f]
     shape, N = sig0.shape, sig0.size
@@ -94,17 +73,53 @@ def test_compat_iteration_matches_oracle(dims):
         acc = oracle_acc(g, sig2.reshape(shape), src, 2, signal_x, it, om)
         eff2 = oracle.laplace_volume(acc, jw, tw, upper_f, start_f)[:, :, rflat].transpose(0, 2, 1)
         k1, k2 = oracle.calc_alpha(des[s], eff2, effl[s], k1, k2)
-    alpha, sig_new = oracle.update_para2(shape, nc, iseabed, k1, k2, scaler, 0.05, 5.0, sig0, grad)
+    alpha, sig_new = oracle.update_para2(shape, nc, iseabed, k1, k2, scaler, minval, maxval,
+                                         sig0, grad)
+    return dict(err=err, des=des, grad=grad, scaler=scaler, sig2=sig2, alpha=alpha,
+                sigma=sig_new)
 
-    assert abs(res.err_sum - err) <= TOL * err
-    for s in range(len(sources)):
-        assert rel(res.residuals[s], des[s]) <= TOL
-    assert np.any(res.grad) and rel(res.grad.ravel(), grad) <= TOL
-    assert abs(res.scaler - scaler) <= TOL * abs(scaler)
-    assert rel(res.sigma_test.ravel(), sig2) <= TOL
-    assert abs(res.alpha - alpha) <= TOL * alpha
-    assert rel(res.sigma.ravel(), sig_new) <= TOL
+
+def check_iteration(res, want, sig0):
+    """Library iteration vs the oracle's, at the frequency-field tolerance."""
+    assert abs(res.err_sum - want["err"]) <= TOL * want["err"]
+    for s, de in enumerate(want["des"]):
+        assert rel(res.residuals[s], de) <= TOL
+    assert np.any(res.grad) and rel(res.grad.ravel(), want["grad"]) <= TOL
+    assert abs(res.scaler - want["scaler"]) <= TOL * abs(want["scaler"])
+    assert rel(res.sigma_test.ravel(), want["sig2"]) <= TOL
+    assert abs(res.alpha - want["alpha"]) <= TOL * want["alpha"]
+    assert rel(res.sigma.ravel(), want["sigma"]) <= TOL
     assert res.sigma.ravel()[-1] == sig0.ravel()[-1]     # CPML corner: untouched
+
+
+@pytest.mark.parametrize("dims", [(22, 20, 18), (21, 19, 17)])
+def test_compat_iteration_matches_oracle(dims):
+    """(21, 19, 17) has an odd cell count: every cell, down to the last CPML
+    corner, must come back from the update."""
+    ix, iy, iz = dims
+    nc, it = 4, 90
+    start_f, upper_f, iseabed = 1, 4, 9
+    rng = np.random.default_rng(12)
+    sig_true = np.exp(rng.uniform(np.log(0.2), np.log(2.0), (iz, iy, ix)))
+    sig0 = np.full((iz, iy, ix), 0.6)
+    dt, cmax = dt_for(50.0, 45.0, 40.0, sig_min=0.2)
+    g = Grid(ix, iy, iz, nc, 50.0, 45.0, 40.0, dt, OM0, cmax)
+    signal_x = firstderiv_gauss(it, dt, 6.0)
+    signal_true = firstderiv_gauss(it, dt, 2.0)
+    sources = np.array([[11, 10, 6], [8, 12, 7]])
+    receivers = np.array([[7, 7, 5], [14, 12, 5]])
+    assert sources.max() < min(dims) and receivers.max() < min(dims)
+    F = CompatFWI(g, sources, receivers, it, signal_x, signal_true, start_f, upper_f,
+                  minval=0.05, maxval=5.0, iseabed=iseabed)
+    true_model = F.test_model
+    true_model.media_coeff_3d(sig_true)
+    eobs = [F.forward_receivers(true_model, s) for s in sources]
+    res = F.iterate(sig0, eobs)
+
+    # ---- oracle: the same iteration with the reference's loops
+    want = oracle_iteration(g, sig0, sources, receivers, it, signal_x, signal_true, start_f,
+                            upper_f, iseabed, 0.05, 5.0, eobs, F.omegas)
+    check_iteration(res, want, sig0)
     # the model now holds the updated conductivity: same data as a fresh upload
     after = F.forward_receivers(F.model, sources[0])
     F.test_model.media_coeff_3d(res.sigma)
