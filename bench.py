@@ -202,12 +202,13 @@ def main():
     ap.add_argument("--grad-nt", type=int, default=0, help="time steps (default 30 x 128)")
     ap.add_argument("--segment", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=2048)
+    ap.add_argument("--warmup", type=int, default=128)
     ap.add_argument("--workload", default="roofline_256", choices=["roofline_256", "plain_64"])
-    ap.add_argument("--ring", type=int, default=64)
+    ap.add_argument("--ring", type=int, default=128)
     ap.add_argument("--zchunk", type=int, default=0)
-    ap.add_argument("--profile-steps", type=int, default=64)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="steps timed per kernel after the timed region (default: one ring)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall time of the CPU-oracle sample (whole steps, at least 2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -237,6 +238,8 @@ def main():
     wl = configs.roofline_256() if args.workload == "roofline_256" else configs.plain_64()
     g = wl.grid
     nf = len(wl.omegas)
+    if args.profile_steps <= 0:
+        args.profile_steps = args.ring
     need = args.warmup + args.steps + args.profile_steps
     if need > wl.steps:
         sys.exit(f"--warmup + --steps + --profile-steps = {need} exceeds the {wl.steps} steps "
@@ -259,9 +262,11 @@ def main():
     prof = shot.profile_steps(args.profile_steps)
     log(f"[bench] per-kernel ms: {prof}")
 
-    kb = kernel_bytes(g, nf, args.ring)
+    # the profiled flush covers min(ring, profile-steps) slots
+    flushed = min(args.ring, args.profile_steps)
+    kb = kernel_bytes(g, nf, flushed)
     per_step = {"efield": prof["efield"], "hfield": prof["hfield"],
-                "flush": prof["flush"] / args.ring}
+                "flush": prof["flush"] / flushed}
     dom = max(per_step, key=per_step.get)
     dom_ms = prof[dom]
     achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
