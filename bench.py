@@ -156,6 +156,53 @@ def serve_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_fwi_mode(args):
+    """Served FWI gradients with checkpointing (BASELINE cfg5, bounded):
+    384^3 interior + 16-cell CPML (416^3 cells), a seeded Poisson stream of
+    forward (70 %) and gradient (30 %) requests with 1-2 x-dipole shots each,
+    8 frequencies, 200 GB HBM budget; Nt = --fwi-it (default 1000) instead of
+    a full-length run, so the stream finishes in minutes.  At this size the
+    sqrt(Nt) checkpoint plan (about 66 states of 4 GB) does not fit the
+    budget, so each gradient recomputes recursively under it.  Observed data of
+    the gradient requests are computed before the stream starts.  Compared
+    with the same requests run one after another."""
+    from mittet_main_amd import serving
+    reqs = serving.fwi_stream(args.seed, args.requests, n_int=args.fwi_interior,
+                              it=args.fwi_it, max_shots=args.max_shots)
+    log(f"[fwi] {len(reqs)} requests on {args.fwi_interior + 32}^3 cells, it={args.fwi_it}: "
+        f"{[(r.kind, r.nshots) for r in reqs]}")
+    t0 = time.perf_counter()
+    for r in reqs:
+        if r.kind == "gradient":
+            r.d_obs = serving.observed_data(r)
+    log(f"[fwi] observed data for the gradient requests: {time.perf_counter() - t0:.1f} s")
+    done = lambda r: log(f"[fwi]   request {r.rid} done, latency {r.latency:.2f} s"
+                         + (f", levels {r.levels}" if r.grad is not None else ""))
+    srv = serving.Server(budget_bytes=200e9, chunk=32)
+    served = srv.run(reqs, realtime=True, on_done=done)
+    srv.close()
+    strip = lambda d: {k: (round(v, 4) if isinstance(v, float) else v)
+                       for k, v in d.items() if k != "results"}
+    gshots = sum(r.nshots for r in reqs if r.kind == "gradient")
+    out = {"mode": "serve-fwi", "metric": "gradients/s, p99 latency, peak HBM (cfg5)",
+           "workload": f"fwi_stream_{args.fwi_interior}cube_fp64", "seed": args.seed,
+           "cells": (args.fwi_interior + 32) ** 3, "nt": args.fwi_it, "budget_gb": 200,
+           "requests": len(reqs), "gradient_shots": gshots,
+           "gradients_per_s": round(gshots / served["makespan_s"], 5),
+           "levels": {rid: r.levels for rid, r in served["results"].items() if r.grad is not None},
+           "served": strip(served)}
+    log(f"[fwi] served: makespan {served['makespan_s']:.1f} s, peak {served['peak_hbm_gb']:.1f} GB")
+    if not args.no_sequential:
+        seq = serving.run_sequential(reqs, mode="fused", realtime=True, on_done=done)
+        out["sequential"] = strip(seq)
+        out["makespan_vs_sequential"] = round(served["makespan_s"] / seq["makespan_s"], 3)
+        same = all((np.array_equal(r.grad, served["results"][rid].grad) if r.grad is not None else
+                    all(np.array_equal(a, b) for a, b in zip(r.spectra, served["results"][rid].spectra)))
+                   for rid, r in seq["results"].items())
+        out["served_equals_alone_bitwise"] = bool(same)
+    print(json.dumps(out, default=float), flush=True)
+
+
 def gradient_mode(args):
     """Adjoint gradient: 128^3 interior + 12-cell CPML, 8 x-dipole shots, 64
     receivers, L2 misfit on complex receiver E at 4 frequencies, Nt = 30 x 128;
@@ -194,13 +241,16 @@ def gradient_mode(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", default="forward", choices=["forward", "serve", "gradient"])
+    ap.add_argument("--mode", default="forward", choices=["forward", "serve", "gradient", "serve-fwi"])
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--requests", type=int, default=4)
     ap.add_argument("--max-shots", type=int, default=16)
     ap.add_argument("--ops-requests", type=int, default=1)
     ap.add_argument("--grad-nt", type=int, default=0, help="time steps (default 30 x 128)")
     ap.add_argument("--segment", type=int, default=0)
+    ap.add_argument("--fwi-it", type=int, default=1000, help="serve-fwi: time samples per shot")
+    ap.add_argument("--fwi-interior", type=int, default=384)
+    ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2048)
     ap.add_argument("--warmup", type=int, default=128)
@@ -230,6 +280,8 @@ def main():
         sys.exit(2)
     if args.mode == "serve":
         return serve_mode(args)
+    if args.mode == "serve-fwi":
+        return serve_fwi_mode(args)
     if args.mode == "gradient":
         return gradient_mode(args)
 

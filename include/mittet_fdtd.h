@@ -194,6 +194,21 @@ int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue,
  *       each segment forward from its checkpoint.  out: dJ/dsigma per cell. */
 typedef struct fwi3d_grad fwi3d_grad;
 int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out);
+/* Memory-bounded variant, also without a reference counterpart (the reference
+ * keeps every time step of every Green's run in host memory, function.c:1652-1724).
+ * Recursive recomputation over `levels` depths (1..8): the forward pass stores
+ * ~k top checkpoints, each depth below re-runs one segment to store the starts
+ * of its k pieces, the last depth replays k steps keeping every state;
+ * k = ceil(Nt^(1/(levels+1))), so storage is ~(levels+1) k states for
+ * levels + 1 forward sweeps.  levels = 1 is fwi3d_grad_create(s, 0, out); the
+ * gradient is bitwise the same for every level count.
+ * fwi3d_grad_plan_bytes: device bytes such a gradient would allocate
+ * (-1 on bad arguments), for admission control before creating it. */
+int fwi3d_grad_create_levels(fwi3d_shot *s, int levels, fwi3d_grad **out);
+/* The plan for nt steps (host only): seglen[levels] segment lengths per depth,
+ * the top segment count and the number of stored states (incl. the adjoint). */
+int fwi3d_grad_plan(int nt, int levels, int *seglen, int *nseg, long long *states);
+long long fwi3d_grad_plan_bytes(fwi3d_shot *s, int levels);
 int fwi3d_grad_destroy(fwi3d_grad *g);
 long long fwi3d_grad_bytes(fwi3d_grad *g);
 int fwi3d_grad_forward(fwi3d_grad *g);

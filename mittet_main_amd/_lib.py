@@ -77,6 +77,10 @@ _SIGS = {
     "fwi3d_laplace_to_freq": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p]),
     "fwi3d_grad_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "fwi3d_grad_create_levels": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "fwi3d_grad_plan_bytes": (C.c_longlong, [C.c_void_p, C.c_int]),
+    "fwi3d_grad_plan": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_longlong)]),
     "fwi3d_grad_destroy": (C.c_int, [C.c_void_p]),
     "fwi3d_grad_bytes": (C.c_longlong, [C.c_void_p]),
     "fwi3d_grad_forward": (C.c_int, [C.c_void_p]),

@@ -51,6 +51,50 @@ class ShotSpec:
         self.src_comp = int(src_comp)
 
 
+def grad_plan(nt: int, levels: int) -> Tuple[List[int], int, int]:
+    """Checkpoint plan for nt steps: segment length per depth, top segments,
+    stored states (fwi3d_grad_plan)."""
+    sl = (C.c_int * levels)()
+    ns, st = C.c_int(0), C.c_longlong(0)
+    check(_lib.load().fwi3d_grad_plan(int(nt), int(levels), sl, C.byref(ns), C.byref(st)))
+    return list(sl), ns.value, st.value
+
+
+def levels_for_budget(shot, budget_bytes: float, max_levels: int = 8) -> int:
+    """Fewest recursion levels whose gradient storage fits budget_bytes."""
+    L = _lib.load()
+    for lv in range(1, max_levels + 1):
+        b = int(L.fwi3d_grad_plan_bytes(shot._h, lv))
+        if 0 <= b <= budget_bytes:
+            return lv
+    raise ValueError(f"no checkpoint plan of <= {max_levels} levels fits "
+                     f"{budget_bytes / 1e9:.1f} GB")
+
+
+def shot_value_and_grad(shot, obs: np.ndarray, w: np.ndarray, levels: int = 1,
+                        segment: int = 0) -> Tuple[float, np.ndarray, int]:
+    """One shot's misfit term, dJ/dsigma (flat) and the checkpoint bytes it
+    held, on the model the shot runs on.  levels = 1 checkpoints every
+    `segment` steps (0: sqrt(Nt)); levels > 1 recomputes recursively."""
+    L = _lib.load()
+    h = C.c_void_p()
+    if levels == 1:
+        check(L.fwi3d_grad_create(shot._h, int(segment), C.byref(h)))
+    else:
+        check(L.fwi3d_grad_create_levels(shot._h, int(levels), C.byref(h)))
+    try:
+        nbytes = int(L.fwi3d_grad_bytes(h))
+        check(L.fwi3d_grad_forward(h))
+        J, g = misfit_terms(shot.traces()[:3], obs, w)
+        dj = np.zeros(shot.model.grid.ncells)
+        gg = np.ascontiguousarray(g)
+        check(L.fwi3d_grad_backward(h, gg.ctypes.data_as(C.c_void_p),
+                                    dj.ctypes.data_as(C.c_void_p)))
+    finally:
+        check(L.fwi3d_grad_destroy(h))
+    return J, dj, nbytes
+
+
 class GradientProblem:
     """Misfit J(sigma) over several shots sharing a grid, receivers and frequencies.
 
@@ -60,13 +104,22 @@ class GradientProblem:
     time: it attaches checkpoint storage to the shot (one state every `segment`
     steps plus one replayed segment), runs the checkpointed forward pass and the
     adjoint sweep, and frees the storage, so the HBM held at once is one shot's
-    checkpoints whatever the number of shots."""
+    checkpoints whatever the number of shots.
+
+    `levels` > 1 selects recursive recomputation (fwi3d_grad_create_levels):
+    about (levels + 1) Nt^(1/(levels+1)) stored states for levels + 1 forward
+    sweeps.  With `budget_bytes` the fewest levels whose storage fits are
+    chosen per evaluation (ValueError if none does).  The gradient is bitwise
+    the same for every choice."""
 
     def __init__(self, grid: Grid, shots: Sequence[ShotSpec], receivers, it: int, omegas,
-                 d_obs: Optional[List[np.ndarray]] = None, segment: int = 0):
+                 d_obs: Optional[List[np.ndarray]] = None, segment: int = 0, levels: int = 1,
+                 budget_bytes: Optional[float] = None):
         self.grid = grid
         self.it = int(it)
         self.segment = int(segment)
+        self.levels = int(levels)
+        self.budget_bytes = budget_bytes
         self.omegas = np.asarray(omegas, dtype=np.float64)
         self.receivers = np.ascontiguousarray(np.asarray(receivers, dtype=np.int32).reshape(-1, 3))
         self.model = Model(grid)
@@ -105,22 +158,14 @@ class GradientProblem:
         """J and dJ/dlog(sigma) (same shape as log_sigma)."""
         sigma = np.exp(np.asarray(log_sigma, dtype=np.float64))
         self.model.media_coeff_3d(sigma)
-        L = _lib.load()
         total, terms = np.zeros(sigma.size), []
         for sh, obs in zip(self.shots, self.d_obs):
-            h = C.c_void_p()
-            check(L.fwi3d_grad_create(sh._h, self.segment, C.byref(h)))
-            try:
-                self.peak_storage = max(self.peak_storage, int(L.fwi3d_grad_bytes(h)))
-                check(L.fwi3d_grad_forward(h))
-                J, g = misfit_terms(sh.traces()[:3], obs, self.w)
-                terms.append(J)
-                dj = np.zeros(sigma.size)
-                gg = np.ascontiguousarray(g)
-                check(L.fwi3d_grad_backward(h, gg.ctypes.data_as(C.c_void_p),
-                                            dj.ctypes.data_as(C.c_void_p)))
-            finally:
-                check(L.fwi3d_grad_destroy(h))
+            levels = self.levels if self.budget_bytes is None else \
+                levels_for_budget(sh, self.budget_bytes)
+            J, dj, nbytes = shot_value_and_grad(sh, obs, self.w, levels, self.segment)
+            self.last_levels = levels
+            self.peak_storage = max(self.peak_storage, nbytes)
+            terms.append(J)
             total += dj
         return math.fsum(terms), (total * sigma.reshape(-1)).reshape(np.shape(log_sigma))
 

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mittet_fdtd.h"
@@ -42,21 +43,50 @@ int fail(int code, const char *fmt, ...)
                         __FILE__, __LINE__);                                            \
     } while (0)
 
+// The HIP runtime refuses an allocation or a synchronous copy while a stream
+// captures a graph, in any thread.  Objects may be driven from several host
+// threads (a gradient request beside the forward batch), so capture sections,
+// allocations, frees and synchronous copies take this lock; kernel launches and
+// graph launches do not.
+std::recursive_mutex g_rt_mu;
+
 template <typename T>
 hipError_t dalloc(T **p, size_t count)
 {
     *p = nullptr;
     if (count == 0)
         return hipSuccess;
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
     return hipMalloc((void **)p, count * sizeof(T));
 }
 
 template <typename T>
 void dfree(T *&p)
 {
-    if (p)
+    if (p) {
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
         (void)hipFree((void *)p);
+    }
     p = nullptr;
+}
+
+hipError_t new_stream(hipStream_t *st)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
+    return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+}
+
+void del_stream(hipStream_t st)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
+    (void)hipStreamDestroy(st);
+}
+
+// synchronous host-to-device copy
+hipError_t hcopy(void *dst, const void *src, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
 }
 
 int axis_len(const fwi3d_grid_params &p, int a) { return a == 0 ? p.ix : (a == 1 ? p.iy : p.iz); }
@@ -300,6 +330,7 @@ int capture_chunk(hipGraphExec_t *out, const ShotDev *d, int ns, const Fam &f,
                   hipStream_t st)
 {
     hipGraph_t g = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = launch_chunk(d, ns, f, pre_cells, ncellblocks, L, any_dft, any_rec, st);
     hipError_t e = hipStreamEndCapture(st, &g);
@@ -356,7 +387,7 @@ int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **o
     if (stream) {
         m->stream = (hipStream_t)stream;
     } else {
-        hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+        hipError_t e = new_stream(&m->stream);
         if (e != hipSuccess) {
             delete m;
             return fail(FWI3D_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
@@ -369,7 +400,7 @@ int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **o
         const int n = axis_len(*p, a);
         pack_device_profile(m->prof[a], n, dp);
         if (dalloc(&m->d_prof[a], dp.size()) != hipSuccess ||
-            hipMemcpy(m->d_prof[a], dp.data(), sizeof(double) * dp.size(), hipMemcpyHostToDevice) !=
+            hcopy(m->d_prof[a], dp.data(), sizeof(double) * dp.size()) !=
                 hipSuccess) {
             fwi3d_model_destroy(m);
             return fail(FWI3D_ENOMEM, "profile upload failed");
@@ -397,7 +428,7 @@ int fwi3d_model_destroy(fwi3d_model *m)
     dfree(m->d_eps);
     dfree(m->d_cbi);
     if (m->own_stream && m->stream)
-        (void)hipStreamDestroy(m->stream);
+        del_stream(m->stream);
     delete m;
     return FWI3D_OK;
 }
@@ -532,10 +563,10 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
 
     int rc = FWI3D_OK;
     const int zero2[2] = {0, 0};
-    if (hipMemcpy(s->d_pre, zero2, sizeof zero2, hipMemcpyHostToDevice) != hipSuccess)
+    if (hcopy(s->d_pre, zero2, sizeof zero2) != hipSuccess)
         rc = fail(FWI3D_EDEVICE, "prefix upload failed");
     if (!rc && sp->nrec &&
-        hipMemcpy(s->d_rec, s->rec.data(), sizeof(int) * sp->nrec, hipMemcpyHostToDevice) != hipSuccess)
+        hcopy(s->d_rec, s->rec.data(), sizeof(int) * sp->nrec) != hipSuccess)
         rc = fail(FWI3D_EDEVICE, "receiver upload failed");
     if (!rc && s->nf) {
         // w[n][f] = dt*exp(-a n dt)*(cos, sin)(a n dt), a = sqrt(omega0*omega_f)
@@ -551,7 +582,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
                 w[2 * ((size_t)n * s->nf + f) + 1] = e * sn;
             }
         }
-        if (hipMemcpy(s->d_w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+        if (hcopy(s->d_w, w.data(), sizeof(double) * w.size()) != hipSuccess)
             rc = fail(FWI3D_EDEVICE, "weight upload failed");
     }
     if (rc) {
@@ -658,11 +689,12 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
         d.fb = (int)shell.size();
         d.fb_pad = tiles(d.fb, 8) * 8;
         shell.resize(d.fb_pad, int4{0, 0, -1, 0});
-        if (dalloc(&s->d_shell, shell.size()) != hipSuccess ||
-            hipMemcpy(s->d_shell, shell.data(), sizeof(int4) * shell.size(), hipMemcpyHostToDevice) !=
-                hipSuccess) {
+        hipError_t e = dalloc(&s->d_shell, shell.size());
+        if (e == hipSuccess)
+            e = hcopy(s->d_shell, shell.data(), sizeof(int4) * shell.size());
+        if (e != hipSuccess) {
             fwi3d_shot_destroy(s);
-            return fail(FWI3D_ENOMEM, "shell tile list");
+            return fail(FWI3D_ENOMEM, "shell tile list: %s", hipGetErrorString(e));
         }
         d.shell = s->d_shell;
         s->kind = 3;
@@ -1067,7 +1099,7 @@ int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal
     if (dalloc(&d_sig, (size_t)it) != hipSuccess)
         return fail(FWI3D_ENOMEM, "signal buffer");
     int rc = FWI3D_OK;
-    if (hipMemcpy(d_sig, signal, sizeof(double) * it, hipMemcpyHostToDevice) != hipSuccess)
+    if (hcopy(d_sig, signal, sizeof(double) * it) != hipSuccess)
         rc = fail(FWI3D_EDEVICE, "signal upload");
     if (!rc)
         rc = device_rows_dft(nullptr, d_sig, 1, it, dt, omega0, nfreq, omegas, out);
@@ -1142,13 +1174,62 @@ int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue, double 
 
 // ---------------------------------------------------------------- gradients
 
+// Checkpoint plan of a gradient (recursive recomputation).  The forward pass
+// stores the state at the start of every top segment (length seglen[0]).  The
+// backward sweep takes the segments in reverse; at depth l < levels - 1 it
+// re-runs a segment once from its stored start, storing the starts of its
+// pieces (length seglen[l + 1]) in that depth's slots, and recurses into the
+// pieces in reverse; at depth levels - 1 it replays the segment keeping every
+// state and runs the adjoint steps over it.  levels = 1 is the classic
+// sqrt(Nt) scheme; every extra level costs one more forward sweep and divides
+// the stored states by about Nt^(1/(levels(levels+1))).
+struct GradPlan {
+    int levels = 1;
+    int k = 1;                        // pieces per segment below the top
+    std::vector<int> seglen;          // [levels]: segment length at each depth
+    int nseg = 0;                     // top segments
+    size_t states = 0;                // stored states: top + inner depths + leaf buffer + adjoint
+};
+
+static GradPlan grad_plan(int nt, int levels, int segment)
+{
+    GradPlan P;
+    P.levels = std::max(1, levels);
+    if (P.levels == 1) {
+        const int seg = segment > 0 ? std::min(segment, nt)
+                                    : std::max(1, (int)std::ceil(std::sqrt((double)nt)));
+        P.k = seg;
+        P.seglen = {seg};
+    } else {
+        // smallest k with k^(levels + 1) >= nt
+        int k = std::max(1, (int)std::floor(std::pow((double)nt, 1.0 / (P.levels + 1))));
+        while (std::pow((double)k, P.levels + 1) < (double)nt)
+            ++k;
+        P.k = k;
+        P.seglen.resize(P.levels);
+        for (int l = P.levels - 1; l >= 0; --l) {
+            const double len = std::pow((double)k, P.levels - l);
+            P.seglen[l] = (int)std::min<double>(len, nt);
+        }
+    }
+    P.nseg = (nt + P.seglen[0] - 1) / P.seglen[0];
+    size_t inner = 0;
+    for (int l = 1; l < P.levels; ++l)
+        inner += (size_t)((P.seglen[l - 1] + P.seglen[l] - 1) / P.seglen[l]);
+    P.states = (size_t)P.nseg + inner + (size_t)P.seglen[P.levels - 1] + 1 + 1;
+    return P;
+}
+
 struct fwi3d_grad {
     fwi3d_shot *s = nullptr;
-    int seg = 0, nseg = 0, nt = 0;
+    GradPlan plan;
+    int seg = 0, nseg = 0, nt = 0;    // top segment length and count
+    int leaf = 0;                     // steps replayed with every state kept
     size_t state = 0;                 // doubles per stored state
     size_t off_h = 0, off_psi[3] = {0, 0, 0};
     double *ckpt = nullptr;           // nseg states
-    double *buf = nullptr;            // seg + 1 states
+    std::vector<double *> inner;      // depth l >= 1: slots for the starts of a segment's pieces
+    double *buf = nullptr;            // leaf + 1 states
     double *adj = nullptr;            // adjoint state (one state)
     double *tmp = nullptr;            // 3N + psi slabs
     double *grad = nullptr;           // N
@@ -1212,7 +1293,7 @@ static int set_step(fwi3d_shot *s, int n)
 static int replay_segment(fwi3d_grad *g, int L)
 {
     fwi3d_shot *s = g->s;
-    const int slot = L == g->seg ? 0 : 1;
+    const int slot = L == g->leaf ? 0 : 1;
     if (g->replay[slot] && g->replay_len[slot] != L) {
         (void)hipGraphExecDestroy(g->replay[slot]);
         g->replay[slot] = nullptr;
@@ -1221,6 +1302,7 @@ static int replay_segment(fwi3d_grad *g, int L)
         const Fam f = shot_fam(s);
         const mt::StateSegs segs = state_segs(g);
         hipGraph_t graph = nullptr;
+        std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
         HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
         int rc = FWI3D_OK;
         for (int j = 0; j < L && !rc; ++j) {
@@ -1253,7 +1335,26 @@ static int replay_segment(fwi3d_grad *g, int L)
 
 extern "C" {
 
-int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out)
+static size_t state_doubles(const fwi3d_shot *s)
+{
+    size_t o = 6 * (size_t)s->desc.n;
+    for (int a = 0; a < 3; ++a)
+        o += mt::PSI_N * (size_t)s->desc.slab[a];
+    return o;
+}
+
+// device bytes a gradient with this plan allocates (states, scratch, gradient,
+// trace sensitivities, profile reciprocals)
+static long long grad_plan_bytes(const fwi3d_shot *s, const GradPlan &P)
+{
+    const size_t N = (size_t)s->desc.n, st = state_doubles(s);
+    const size_t tmp = st - 3 * N;
+    const size_t other = tmp + N + 1 + std::max<size_t>(1, 3 * (size_t)s->nrec * s->it) +
+                         2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz);
+    return 8LL * (long long)(P.states * st + other);
+}
+
+static int grad_new(fwi3d_shot *s, int levels, int segment, fwi3d_grad **out)
 {
     if (!s || !out)
         return fail(FWI3D_EINVAL, "null argument");
@@ -1262,26 +1363,37 @@ int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out)
         return fail(FWI3D_EINVAL, "gradient shots must not carry a volume DFT");
     if (s->batch)
         return fail(FWI3D_ESTATE, "shot is in a batch");
+    if (levels < 1 || levels > 8)
+        return fail(FWI3D_EINVAL, "levels must be in [1, 8] (got %d)", levels);
     const int nt = s->it - 1;
     fwi3d_grad *g = new fwi3d_grad();
     g->s = s;
     g->nt = nt;
-    g->seg = segment > 0 ? std::min(segment, nt) : std::max(1, (int)std::ceil(std::sqrt((double)nt)));
-    g->nseg = (nt + g->seg - 1) / g->seg;
+    g->plan = grad_plan(nt, levels, segment);
+    g->seg = g->plan.seglen[0];
+    g->nseg = g->plan.nseg;
+    g->leaf = g->plan.seglen[levels - 1];
     state_layout(g);
     const size_t N = (size_t)s->desc.n;
     size_t tmp = 3 * N;
     for (int a = 0; a < 3; ++a)
         tmp += mt::PSI_N * (size_t)s->desc.slab[a];
-    if (dalloc(&g->ckpt, g->state * g->nseg) != hipSuccess ||
-        dalloc(&g->buf, g->state * (g->seg + 1)) != hipSuccess ||
+    bool ok = dalloc(&g->ckpt, g->state * g->nseg) == hipSuccess;
+    for (int l = 1; l < levels && ok; ++l) {
+        const size_t pieces =
+            (size_t)((g->plan.seglen[l - 1] + g->plan.seglen[l] - 1) / g->plan.seglen[l]);
+        double *p = nullptr;
+        ok = dalloc(&p, g->state * pieces) == hipSuccess;
+        g->inner.push_back(p);
+    }
+    if (!ok || dalloc(&g->buf, g->state * (g->leaf + 1)) != hipSuccess ||
         dalloc(&g->adj, g->state) != hipSuccess || dalloc(&g->tmp, tmp) != hipSuccess ||
         dalloc(&g->grad, N) != hipSuccess || dalloc(&g->grad_src, 1) != hipSuccess ||
         dalloc(&g->g, std::max<size_t>(1, 3 * (size_t)s->nrec * s->it)) != hipSuccess ||
         dalloc(&g->rp, 2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz)) != hipSuccess) {
         fwi3d_grad_destroy(g);
         return fail(FWI3D_ENOMEM, "gradient storage (%zu states of %zu doubles)",
-                    (size_t)g->nseg + g->seg + 2, g->state);
+                    g->plan.states, g->state);
     }
     // reciprocal ked / khd rows: the transposed stencils multiply where the
     // forward divides
@@ -1296,12 +1408,41 @@ int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out)
                 g->kappa_one &= v == s->desc.dl[a];
             }
     }
-    if (hipMemcpy(g->rp, rp.data(), sizeof(double) * rp.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hcopy(g->rp, rp.data(), sizeof(double) * rp.size()) != hipSuccess) {
         fwi3d_grad_destroy(g);
         return fail(FWI3D_EDEVICE, "profile upload failed");
     }
     *out = g;
     return FWI3D_OK;
+}
+
+int fwi3d_grad_create(fwi3d_shot *s, int segment, fwi3d_grad **out)
+{
+    return grad_new(s, 1, segment, out);
+}
+
+int fwi3d_grad_create_levels(fwi3d_shot *s, int levels, fwi3d_grad **out)
+{
+    return grad_new(s, levels, 0, out);
+}
+
+int fwi3d_grad_plan(int nt, int levels, int *seglen, int *nseg, long long *states)
+{
+    if (nt < 1 || levels < 1 || levels > 8 || !seglen || !nseg || !states)
+        return fail(FWI3D_EINVAL, "bad gradient plan request");
+    const GradPlan P = grad_plan(nt, levels, 0);
+    for (int l = 0; l < levels; ++l)
+        seglen[l] = P.seglen[l];
+    *nseg = P.nseg;
+    *states = (long long)P.states;
+    return FWI3D_OK;
+}
+
+long long fwi3d_grad_plan_bytes(fwi3d_shot *s, int levels)
+{
+    if (!s || levels < 1 || levels > 8 || s->it < 2)
+        return -1;
+    return grad_plan_bytes(s, grad_plan(s->it - 1, levels, 0));
 }
 
 int fwi3d_grad_destroy(fwi3d_grad *g)
@@ -1311,6 +1452,8 @@ int fwi3d_grad_destroy(fwi3d_grad *g)
     if (g->s)
         (void)hipStreamSynchronize(g->s->st);
     dfree(g->ckpt);
+    for (double *p : g->inner)
+        dfree(p);
     dfree(g->buf);
     dfree(g->adj);
     dfree(g->tmp);
@@ -1329,7 +1472,7 @@ long long fwi3d_grad_bytes(fwi3d_grad *g)
 {
     if (!g)
         return -1;
-    return 8LL * (long long)g->state * (g->nseg + g->seg + 2);
+    return grad_plan_bytes(g->s, g->plan);
 }
 
 int fwi3d_grad_forward(fwi3d_grad *g)
@@ -1346,6 +1489,50 @@ int fwi3d_grad_forward(fwi3d_grad *g)
     if (!rc)
         rc = fwi3d_shot_sync(s);
     return rc;
+}
+
+// Adjoint steps over [n0, n0 + L) in reverse, at recursion depth `depth` of
+// the checkpoint plan; the stored state `start` is the forward state at n0.
+static int adj_range(fwi3d_grad *g, int depth, int n0, int L, double *start, mt::AdjArgs &A,
+                     const std::vector<double> &amp)
+{
+    fwi3d_shot *s = g->s;
+    const int nrec = s->nrec;
+    int rc;
+    if ((rc = state_copy(g, start, false)) || (rc = set_step(s, n0)))
+        return rc;
+    if (depth == g->plan.levels - 1) {
+        // replay the segment keeping every state, then step the adjoint back
+        if ((rc = state_copy(g, g->buf, true)) || (rc = replay_segment(g, L)))
+            return rc;
+        for (int t = L - 1; t >= 0; --t) {
+            const int n = n0 + t;
+            const double *u0 = g->buf + g->state * t, *u1 = g->buf + g->state * (t + 1);
+            A.uE0 = u0;
+            A.uH0 = u0 + g->off_h;
+            A.uE1 = u1;
+            for (int a = 0; a < 3; ++a)
+                A.upsi1[a] = u1 + g->off_psi[a];
+            A.amp_n = amp[n];
+            HIP_TRY(mt::launch_adjoint_step(A, g->g + (size_t)n * 3 * nrec, nrec, s->d_rec, s->st));
+        }
+        return FWI3D_OK;
+    }
+    // one forward pass over the segment storing the start of each piece
+    const int P = g->plan.seglen[depth + 1];
+    const int np = (L + P - 1) / P;
+    double *slots = g->inner[depth];
+    for (int p = 0; p < np; ++p) {
+        if ((rc = state_copy(g, slots + g->state * p, true)))
+            return rc;
+        if (p + 1 < np && (rc = fwi3d_step_n(s, P)))
+            return rc;
+    }
+    for (int p = np - 1; p >= 0; --p)
+        if ((rc = adj_range(g, depth + 1, n0 + p * P, std::min(P, L - p * P),
+                            slots + g->state * p, A, amp)))
+            return rc;
+    return FWI3D_OK;
 }
 
 int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
@@ -1426,23 +1613,8 @@ int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
     A.sig_src = sig_src;
     for (int q = g->nseg - 1; q >= 0; --q) {
         const int n0 = q * g->seg, L = std::min(g->seg, g->nt - n0);
-        // replay the segment from its checkpoint, keeping every state
-        if ((rc = state_copy(g, g->ckpt + g->state * q, false)) || (rc = set_step(s, n0)) ||
-            (rc = state_copy(g, g->buf, true)))
+        if ((rc = adj_range(g, 0, n0, L, g->ckpt + g->state * q, A, amp)))
             return rc;
-        if ((rc = replay_segment(g, L)))
-            return rc;
-        for (int t = L - 1; t >= 0; --t) {
-            const int n = n0 + t;
-            const double *u0 = g->buf + g->state * t, *u1 = g->buf + g->state * (t + 1);
-            A.uE0 = u0;
-            A.uH0 = u0 + g->off_h;
-            A.uE1 = u1;
-            for (int a = 0; a < 3; ++a)
-                A.upsi1[a] = u1 + g->off_psi[a];
-            A.amp_n = amp[n];
-            HIP_TRY(mt::launch_adjoint_step(A, g->g + (size_t)n * 3 * nrec, nrec, s->d_rec, s->st));
-        }
     }
     std::vector<double> ge(N);
     double gs = 0.0;
@@ -1577,9 +1749,9 @@ int fwi3d_compat_create(fwi3d_model *m, int nsrc, int nrec, const int *rec_ijk, 
         fwi3d_compat_destroy(c);
         return fail(FWI3D_ENOMEM, "compat buffers (%zu complex spectra per source)", vol);
     }
-    if (hipMemcpy(c->d_rec, c->rec.data(), sizeof(int) * nrec, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_jw, c->jw.data(), sizeof(mt::cplx) * c->nb, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_tw, c->tw.data(), sizeof(mt::cplx) * c->nb, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hcopy(c->d_rec, c->rec.data(), sizeof(int) * nrec) != hipSuccess ||
+        hcopy(c->d_jw, c->jw.data(), sizeof(mt::cplx) * c->nb) != hipSuccess ||
+        hcopy(c->d_tw, c->tw.data(), sizeof(mt::cplx) * c->nb) != hipSuccess) {
         fwi3d_compat_destroy(c);
         return fail(FWI3D_EDEVICE, "compat upload failed");
     }
@@ -1812,7 +1984,7 @@ int fwi3d_batch_create(int max_shots, int chunk, void *stream, fwi3d_batch **out
     b->chunk = chunk;
     if (stream) {
         b->st = (hipStream_t)stream;
-    } else if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+    } else if (new_stream(&b->st) != hipSuccess) {
         delete b;
         return fail(FWI3D_EDEVICE, "hipStreamCreate failed");
     } else {
@@ -1844,7 +2016,7 @@ int fwi3d_batch_destroy(fwi3d_batch *b)
     dfree(b->d_pre_l);
     dfree(b->d_pre_cells);
     if (b->own_stream && b->st)
-        (void)hipStreamDestroy(b->st);
+        del_stream(b->st);
     delete b;
     return FWI3D_OK;
 }

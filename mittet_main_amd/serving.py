@@ -1,4 +1,5 @@
-"""Serving: continuous batching of forward-modelling shots.
+"""Serving: continuous batching of forward-modelling shots, and gradient
+requests served beside them.
 
 A request carries a conductivity model (seeded layered + box generator), a
 grid size, 1-16 shots (x-dipoles at random positions) and asks for receiver
@@ -16,18 +17,31 @@ spectra at a few frequencies.  The server
 Each shot's result is bit-identical to the same shot run alone (per-shot
 descriptors, no shared accumulators, no data-dependent reduction order).
 
+A gradient request (kind "gradient") brings observed receiver spectra for
+its shots and asks for the misfit and dJ/dlog(sigma) at its starting model
+(the smoothed background of its seed).  Gradient requests run one at a time in
+a worker thread, on their own model's HIP stream, while the main loop keeps
+advancing the forward batch, so their kernels overlap on the device.  A
+gradient request is admitted when its model, one shot and the checkpoint
+storage fit in the budget left; the storage plan is the fewest recursion
+levels (fwi3d_grad_create_levels) that fit, so a large grid recomputes more
+instead of waiting.  The gradient does not depend on the plan (bitwise).
+
 Latency = completion time - arrival time (wall clock, the request stream is
 replayed in real time); throughput = shots / makespan.
 """
 from __future__ import annotations
 
+import math
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from . import configs
+from . import _lib, configs
+from .adjoint import dft_weights, receiver_spectra, shot_value_and_grad
 from .fwi3d import Batch, Grid, Model, Shot, firstderiv_gauss, shot_bytes
 
 MODEL_BYTES_PER_CELL = 8 * 3      # sigma, eps', dt/eps'
@@ -44,6 +58,8 @@ class Request:
     sources: np.ndarray             # (nshots, 3) 0-based cells
     receivers: np.ndarray           # (nrec, 3)
     freqs_hz: List[float]
+    kind: str = "forward"           # "forward" or "gradient"
+    d_obs: Optional[List[np.ndarray]] = None   # gradient: per shot (3, nrec, nf) complex
 
     @property
     def nshots(self) -> int:
@@ -53,9 +69,12 @@ class Request:
 @dataclass
 class Result:
     rid: int
-    spectra: List[np.ndarray]       # per shot (6, nrec, nf) complex
+    spectra: List[np.ndarray]       # forward: per shot (6, nrec, nf) complex
     arrival: float
     done: float
+    misfit: Optional[float] = None  # gradient requests
+    grad: Optional[np.ndarray] = None   # dJ/dlog(sigma), [iz, iy, ix]
+    levels: int = 0                 # checkpoint recursion levels used
 
     @property
     def latency(self) -> float:
@@ -87,12 +106,122 @@ def request_stream(seed: int, n_requests: int, rate: float = 20.0,
     return out
 
 
+def fwi_stream(seed: int, n_requests: int, n_int: int = 384, ncpml: int = 16,
+               rate: float = 20.0, grad_frac: float = 0.3, max_shots: int = 2, nf: int = 8,
+               it: int = 1000, nrec_side: int = 8) -> List[Request]:
+    """Mixed stream of forward and gradient requests on one grid size:
+    Poisson arrivals (mean `rate`/s), each request a gradient with probability
+    `grad_frac`, 1..max_shots x-dipole shots at random interior positions,
+    receivers on an nrec_side^2 grid on the top interface, `nf` frequencies
+    log-spaced in 0.1-4 Hz.  Gradient requests still need observed data
+    (observed_data) before they are served."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    freqs = list(np.geomspace(0.1, 4.0, nf))
+    lo, hi = ncpml + 2, ncpml + n_int - 3
+    g = np.linspace(lo, hi, nrec_side).round().astype(int)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    k_top = ncpml + n_int // 3
+    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, k_top)], axis=1).astype(np.int32)
+    for rid in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        kind = "gradient" if rng.random() < grad_frac else "forward"
+        ns = int(rng.integers(1, max_shots + 1))
+        src = rng.integers(lo, hi + 1, size=(ns, 3)).astype(np.int32)
+        out.append(Request(rid, t, n_int, ncpml, it, int(rng.integers(1 << 30)), src, rec.copy(),
+                           freqs, kind=kind))
+    return out
+
+
 def build_model(req: Request):
     sig, _ = configs.layered_box_sigma(req.n_int, req.ncpml, req.seed)
     n = req.n_int + 2 * req.ncpml
     dt, cmax = configs.dt_for(50.0, 50.0, 50.0)
     grid = Grid(n, n, n, req.ncpml, 50.0, 50.0, 50.0, dt, configs.OMEGA0, cmax)
     return grid, sig
+
+
+def start_model(req: Request) -> np.ndarray:
+    """Starting model of a gradient request: the smoothed three-layer
+    background of its seed."""
+    return configs.smoothed_background(req.n_int, req.ncpml, req.seed)
+
+
+def observed_data(req: Request) -> List[np.ndarray]:
+    """The receiver E spectra a gradient request brings: its shots run on the
+    true model of its seed (layered + boxes)."""
+    grid, sig = build_model(req)
+    model = Model(grid)
+    try:
+        model.media_coeff_3d(sig)
+        w = dft_weights(req.it, grid.dt, grid.omega0, 2 * np.pi * np.asarray(req.freqs_hz))
+        signal = firstderiv_gauss(req.it, grid.dt, configs.FMAX_W)
+        out = []
+        for src in req.sources:
+            sh = model.shot(req.it, signal, tuple(int(v) for v in src), 0, req.receivers)
+            sh.run()
+            out.append(receiver_spectra(sh.traces()[:3], w))
+            sh.close()
+        return out
+    finally:
+        model.close()
+
+
+def gradient_bytes(req: Request, budget: float):
+    """(levels, bytes) of the cheapest checkpoint plan whose model + shot +
+    storage fits `budget`, or None."""
+    grid, _ = build_model(req)
+    fixed = grid.ncells * MODEL_BYTES_PER_CELL + shot_bytes(grid, req.it, nrec=len(req.receivers))
+    nt = req.it - 1
+    for lv in range(1, 9):
+        need = fixed + plan_bytes(grid, nt, len(req.receivers), lv)
+        if need <= budget:
+            return lv, need
+    return None
+
+
+def plan_bytes(grid: Grid, nt: int, nrec: int, levels: int) -> int:
+    """Device bytes of a gradient's checkpoint storage (host-side mirror of
+    fwi3d_grad_plan_bytes: states of 6 fields + shell psi, scratch, gradient)."""
+    import ctypes as C
+    sl = (C.c_int * levels)()
+    ns, st = C.c_int(0), C.c_longlong(0)
+    _lib.check(_lib.load().fwi3d_grad_plan(nt, levels, sl, C.byref(ns), C.byref(st)))
+    n, nc = grid.ncells, grid.ncpml
+    psi = 4 * 2 * nc * (grid.iy * grid.iz + grid.ix * grid.iz + grid.ix * grid.iy)
+    state = 6 * n + psi
+    other = (state - 3 * n) + n + 1 + max(1, 3 * nrec * (nt + 1)) + 2 * (grid.ix + grid.iy + grid.iz)
+    return 8 * (st.value * state + other)
+
+
+def run_gradient(req: Request, budget: float) -> Result:
+    """A gradient request, one shot at a time on its own model: misfit and
+    dJ/dlog(sigma) at the starting model against req.d_obs."""
+    fit = gradient_bytes(req, budget)
+    if fit is None:
+        raise ValueError(f"request {req.rid}: no checkpoint plan fits {budget / 1e9:.1f} GB")
+    levels = fit[0]
+    grid, _ = build_model(req)
+    sig0 = start_model(req)
+    model = Model(grid)
+    try:
+        model.media_coeff_3d(sig0)
+        w = dft_weights(req.it, grid.dt, grid.omega0, 2 * np.pi * np.asarray(req.freqs_hz))
+        signal = firstderiv_gauss(req.it, grid.dt, configs.FMAX_W)
+        total, terms = np.zeros(grid.ncells), []
+        for src, obs in zip(req.sources, req.d_obs):
+            sh = model.shot(req.it, signal, tuple(int(v) for v in src), 0, req.receivers)
+            try:
+                J, dj, _ = shot_value_and_grad(sh, obs, w, levels)
+            finally:
+                sh.close()
+            terms.append(J)
+            total += dj
+    finally:
+        model.close()
+    g = (total * sig0.reshape(-1)).reshape(sig0.shape)
+    return Result(req.rid, [], req.arrival, 0.0, misfit=math.fsum(terms), grad=g, levels=levels)
 
 
 def request_bytes(req: Request, grid: Grid) -> int:
@@ -106,8 +235,11 @@ def shot_result(shot: Shot, req: Request) -> np.ndarray:
 
 
 def request_work(req: Request) -> int:
-    """Cell updates the request needs: cells (incl. CPML) x steps x shots."""
-    return (req.n_int + 2 * req.ncpml) ** 3 * (req.it - 1) * req.nshots
+    """Cell updates the request needs: cells (incl. CPML) x steps x shots
+    (a gradient shot counts as the forward plus its recomputation and adjoint,
+    about 6 forward runs)."""
+    w = (req.n_int + 2 * req.ncpml) ** 3 * (req.it - 1) * req.nshots
+    return 6 * w if req.kind == "gradient" else w
 
 
 class Server:
@@ -152,25 +284,76 @@ class Server:
                 "shot_bytes": shot_bytes(grid, req.it, nrec=len(req.receivers)),
                 "next": 0, "live": {}, "spectra": [None] * req.nshots, "left": req.nshots}
 
+    def _start_gradient(self, req: Request, now: float) -> Optional[dict]:
+        fit = gradient_bytes(req, self.budget - self.in_use)
+        if fit is None:
+            return None
+        need = fit[1]
+        self.in_use += need
+        self.peak = max(self.peak, self.in_use)
+        self.admissions.append((req.rid, now))
+        job = {"req": req, "need": need, "result": None, "error": None,
+               "budget": self.budget - self.in_use + need}
+
+        def work():
+            try:
+                job["result"] = run_gradient(req, job["budget"])
+            except BaseException as e:          # re-raised on the main thread
+                job["error"] = e
+
+        job["thread"] = threading.Thread(target=work, name=f"gradient-{req.rid}", daemon=True)
+        job["thread"].start()
+        return job
+
     def run(self, requests: List[Request], realtime: bool = True,
             on_done: Optional[Callable[[Result], None]] = None) -> Dict:
         for req in requests:
             grid, _ = build_model(req)
+            if req.kind == "gradient":
+                if req.d_obs is None or len(req.d_obs) != req.nshots:
+                    raise ValueError(f"gradient request {req.rid} needs observed data per shot")
+                if gradient_bytes(req, self.budget) is None:
+                    raise ValueError(f"gradient request {req.rid}: no checkpoint plan fits the budget")
+                continue
             one = grid.ncells * MODEL_BYTES_PER_CELL + shot_bytes(grid, req.it, nrec=len(req.receivers))
             if one > self.budget:
                 raise ValueError(f"request {req.rid} needs {one / 1e9:.1f} GB for one shot > budget")
         incoming = sorted(requests, key=lambda r: (r.arrival, r.rid))
         ready: List[Request] = []
+        gready: List[Request] = []            # gradient requests waiting
+        job: Optional[dict] = None            # the gradient request in flight
         started: List[dict] = []              # in start order
         results: List[Result] = []
         cells = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        while incoming or ready or started:
+
+        def finish(res: Result):
+            results.append(res)
+            if on_done:
+                on_done(res)
+
+        while incoming or ready or started or gready or job:
             while incoming and (not realtime or incoming[0].arrival <= now()):
-                ready.append(incoming.pop(0))
+                r = incoming.pop(0)
+                (gready if r.kind == "gradient" else ready).append(r)
             if self.policy == "swf":
                 ready.sort(key=lambda r: (request_work(r), r.arrival, r.rid))
+                gready.sort(key=lambda r: (request_work(r), r.arrival, r.rid))
+            # a finished gradient request leaves; the next one starts if it fits
+            if job and not job["thread"].is_alive():
+                job["thread"].join()
+                self.in_use -= job["need"]
+                if job["error"] is not None:
+                    raise job["error"]
+                res = job["result"]
+                res.done = now()
+                finish(res)
+                job = None
+            if job is None and gready:
+                job = self._start_gradient(gready[0], now())
+                if job is not None:
+                    gready.pop(0)
             # admit shots: unfinished started requests first, then the queue head
             while True:
                 nb = len(self.batch.members)
@@ -184,6 +367,8 @@ class Server:
                         shot_bytes(grid, req.it, nrec=len(req.receivers))
                     if nb and (cells >= self.target_cells or self.in_use + need > self.budget):
                         break
+                    if self.in_use + need > self.budget:
+                        break                 # a gradient holds the memory; wait for it
                     ready.pop(0)
                     a = self._start(req, now())
                     started.append(a)
@@ -203,12 +388,16 @@ class Server:
                 self.in_use += a["shot_bytes"]
                 self.peak = max(self.peak, self.in_use)
             if not self.batch.members:
-                if incoming:                  # idle until the next arrival
-                    time.sleep(max(0.0, min(incoming[0].arrival - now(), 0.01)))
+                if job is not None or incoming:
+                    # wait for the gradient in flight or the next arrival
+                    if job is not None:
+                        job["thread"].join(timeout=0.005)
+                    else:
+                        time.sleep(max(0.0, min(incoming[0].arrival - now(), 0.01)))
                     continue
-                if not ready and not started:
+                if not ready and not started and not gready:
                     break
-                raise RuntimeError("no shot fits the HBM budget while the batch is empty")
+                raise RuntimeError("no request fits the HBM budget while nothing runs")
             self.batch.step(self.chunk)
             self.batch.sync()
             for a in list(started):
@@ -224,30 +413,44 @@ class Server:
                 if a["left"] == 0:
                     a["model"].close()
                     self.in_use -= a["grid"].ncells * MODEL_BYTES_PER_CELL
-                    results.append(Result(a["req"].rid, a["spectra"], a["req"].arrival, now()))
                     started.remove(a)
-                    if on_done:
-                        on_done(results[-1])
+                    finish(Result(a["req"].rid, a["spectra"], a["req"].arrival, now()))
         makespan = now()
         return summarize(results, requests, makespan, peak=self.peak)
 
 
+def _percentiles(lat) -> Dict:
+    lat = np.asarray(lat) if len(lat) else np.zeros(1)
+    return {"mean_s": float(lat.mean()), "p50_s": float(np.percentile(lat, 50)),
+            "p95_s": float(np.percentile(lat, 95)), "p99_s": float(np.percentile(lat, 99))}
+
+
 def summarize(results: List[Result], requests: List[Request], makespan: float, peak=0) -> Dict:
-    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+    """Throughput and latency over all requests; with gradient requests in the
+    stream also per kind (forward shots, gradient shots)."""
+    kind = {r.rid: r.kind for r in requests}
     shots = sum(r.nshots for r in requests)
     cells = sum((r.n_int + 2 * r.ncpml) ** 3 * (r.it - 1) * r.nshots for r in requests)
-    return {"requests": len(requests), "shots": shots, "makespan_s": makespan,
-            "shots_per_s": shots / makespan, "gcell_updates_per_s": cells / makespan / 1e9,
-            "mean_s": float(lat.mean()), "p50_s": float(np.percentile(lat, 50)), "p95_s": float(np.percentile(lat, 95)),
-            "p99_s": float(np.percentile(lat, 99)), "peak_hbm_gb": peak / 1e9,
-            "results": {r.rid: r for r in results}}
+    out = {"requests": len(requests), "shots": shots, "makespan_s": makespan,
+           "shots_per_s": shots / makespan, "gcell_updates_per_s": cells / makespan / 1e9,
+           **_percentiles([r.latency for r in results]), "peak_hbm_gb": peak / 1e9,
+           "results": {r.rid: r for r in results}}
+    if any(k == "gradient" for k in kind.values()):
+        for k in ("forward", "gradient"):
+            sel = [r for r in results if kind[r.rid] == k]
+            n = sum(q.nshots for q in requests if q.kind == k)
+            out[k] = {"requests": len(sel), "shots": n, "shots_per_s": n / makespan,
+                      **_percentiles([r.latency for r in sel])}
+    return out
 
 
 def run_sequential(requests: List[Request], mode: str = "fused", realtime: bool = True,
-                   on_done: Optional[Callable[[Result], None]] = None) -> Dict:
+                   on_done: Optional[Callable[[Result], None]] = None,
+                   budget_bytes: float = 200e9) -> Dict:
     """The plain path: one shot at a time, in arrival order.  mode 'fused' runs
     each shot's graph-captured time loop; mode 'ops' calls the reference-named
-    per-step operators (six launches per step) as the reference loop does."""
+    per-step operators (six launches per step) as the reference loop does.
+    Gradient requests run alone with the whole budget (run_gradient)."""
     results = []
     peak = 0
     t0 = time.perf_counter()
@@ -255,6 +458,14 @@ def run_sequential(requests: List[Request], mode: str = "fused", realtime: bool 
     for req in sorted(requests, key=lambda r: r.arrival):
         if realtime and req.arrival > now():
             time.sleep(req.arrival - now())
+        if req.kind == "gradient":
+            peak = max(peak, gradient_bytes(req, budget_bytes)[1])
+            res = run_gradient(req, budget_bytes)
+            res.done = now()
+            results.append(res)
+            if on_done:
+                on_done(res)
+            continue
         grid, sig = build_model(req)
         model = Model(grid)
         model.media_coeff_3d(sig)

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 OM0 = 2 * np.pi
 
 
-def problem(seed=0, ix=26, iy=24, iz=22, nc=5, it=200, segment=0):
+def problem(seed=0, ix=26, iy=24, iz=22, nc=5, it=200, segment=0, levels=1, budget=None):
     rng = np.random.default_rng(seed)
     sig_true = np.exp(rng.uniform(np.log(0.1), np.log(2.0), (iz, iy, ix)))
     sig0 = np.full((iz, iy, ix), float(np.exp(np.mean(np.log(sig_true)))))
@@ -30,7 +30,8 @@ def problem(seed=0, ix=26, iy=24, iz=22, nc=5, it=200, segment=0):
     rec = np.stack([rng.integers(nc, ix - nc, 8), rng.integers(nc, iy - nc, 8),
                     rng.integers(nc, iz - nc, 8)], axis=1)
     omegas = 2 * np.pi * np.array([1.0, 2.5, 4.0])
-    P = GradientProblem(g, shots, rec, it, omegas, segment=segment)
+    P = GradientProblem(g, shots, rec, it, omegas, segment=segment, levels=levels,
+                        budget_bytes=budget)
     P.d_obs = P.forward_data(sig_true)
     return P, sig0, rng
 
@@ -70,6 +71,47 @@ def test_segment_length_does_not_change_gradient():
     np.testing.assert_array_equal(g1, g2)
     P1.close()
     P2.close()
+
+
+@pytest.mark.parametrize("levels", [2, 3, 4])
+def test_recursive_checkpoints_give_the_same_gradient(levels):
+    """Recursive recomputation replays the same forward states, so misfit and
+    gradient are bitwise those of the sqrt(Nt) scheme, with less storage.
+    it = 200 (199 steps) leaves partial segments at every depth."""
+    P1, sig0, _ = problem(seed=4)
+    P2, _, _ = problem(seed=4, levels=levels)
+    print(f"recursive checkpoints: levels {levels}", flush=True)
+    J1, g1 = P1.value_and_grad(np.log(sig0))
+    J2, g2 = P2.value_and_grad(np.log(sig0))
+    assert J1 == J2
+    np.testing.assert_array_equal(g1, g2)
+    assert P2.storage_bytes() < P1.storage_bytes()
+    P1.close()
+    P2.close()
+
+
+def test_budget_selects_the_fewest_levels_that_fit():
+    from mittet_main_amd.adjoint import levels_for_budget
+    import mittet_main_amd._lib as lib
+
+    P, sig0, _ = problem(seed=6)
+    L = lib.load()
+    sh = P.shots[0]
+    b = [int(L.fwi3d_grad_plan_bytes(sh._h, lv)) for lv in (1, 2, 3)]
+    assert b[0] > b[1] > b[2] > 0
+    assert levels_for_budget(sh, b[0]) == 1
+    assert levels_for_budget(sh, b[0] - 1) == 2
+    assert levels_for_budget(sh, b[2]) == 3
+    with pytest.raises(ValueError):
+        levels_for_budget(sh, 1e3)
+    P.budget_bytes = b[1]
+    J2, g2 = P.value_and_grad(np.log(sig0))
+    assert P.last_levels == 2 and P.storage_bytes() == b[1]
+    P.budget_bytes = None
+    J1, g1 = P.value_and_grad(np.log(sig0))
+    assert J1 == J2
+    np.testing.assert_array_equal(g1, g2)
+    P.close()
 
 
 def test_autograd_function_returns_adjoint_gradient():
