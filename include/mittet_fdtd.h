@@ -180,6 +180,27 @@ int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal
 int fwi3d_laplace_to_freq(fwi3d_shot *s, int nbins, const double *strue,
                           double *e_w, double *j_w, double *g_w, double *obs);
 
+/* ---- diffusive-time back-transform ----------------------------------------- */
+
+/* out[r][k] = scale * sum_n in[r][n] * exp(sign * 2 pi i k n / nd) for complex
+ * rows (interleaved re, im), rows <= 65535, sign +1 or -1, on the GPU (host
+ * buffers in and out).  With sign -1 and scale 2/(nd dt) this is the
+ * reference's back-transform to diffusive time with FFTW_FORWARD
+ * (satan_i/fwi3d_cpu_function_v1.4.c:1507-1527, fwi3d_cpu_convert_v1.4.c:84-93);
+ * with sign +1 the one of ver0040/cpu_f_to_d_e.f90:281-320 (FFTW_BACKWARD);
+ * with scale 1 a plain un-normalised FFTW transform. */
+int fwi3d_dft_rows(const double *in, int rows, int nd, int sign, double scale, double *out);
+/* satan_i/fwi3d_cpu_function_v1.4.c:1430-1564 (laplace_fft) for every receiver
+ * component of the shot: E_n, J_n (J_0 := 2 omega0) and G_n = E_n/J_n at all
+ * bins n < it, then the back-transform x_t[k] = 2/(it dt) sum_n X_n
+ * exp(-2 pi i k n / it).  e_t, g_t: complex [3][nrec][it]; j_t: complex [it]. */
+int fwi3d_laplace_fft(fwi3d_shot *s, double *e_t, double *j_t, double *g_t);
+/* satan_i/fwi3d_cpu_function_v1.4.c:1566-1635 (convolution_GJ_to_E): the
+ * spectrum of E as the product FFTW_BACKWARD(G_t)[k] * FFTW_BACKWARD(J_t)[k]
+ * for each of `rows` complex rows g_t [rows][nd] and the real source j_t [nd].
+ * out: complex [rows][nd]. */
+int fwi3d_convolve_gj(const double *g_t, int rows, int nd, const double *j_t, double *out);
+
 /* ---- differentiation ------------------------------------------------------- */
 
 /* Exact discrete adjoint of the forward step for a receiver misfit.  No

@@ -76,6 +76,9 @@ _SIGS = {
                                         C.c_void_p, C.c_void_p]),
     "fwi3d_laplace_to_freq": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p]),
+    "fwi3d_dft_rows": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]),
+    "fwi3d_laplace_fft": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fwi3d_convolve_gj": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "fwi3d_grad_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "fwi3d_grad_create_levels": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "fwi3d_grad_plan_bytes": (C.c_longlong, [C.c_void_p, C.c_int]),

@@ -17,6 +17,7 @@
 
 #include "../../include/mittet_fdtd.h"
 #include "fdtd_dev.h"
+#include "fdtd_transform.h"
 
 using mt::ShotDev;
 
@@ -1096,14 +1097,18 @@ int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal
     if (!signal || !omegas || !out || it < 1 || nfreq <= 0)
         return fail(FWI3D_EINVAL, "bad source spectrum request");
     double *d_sig = nullptr;
-    if (dalloc(&d_sig, (size_t)it) != hipSuccess)
-        return fail(FWI3D_ENOMEM, "signal buffer");
+    hipStream_t st = nullptr;
+    if (new_stream(&st) != hipSuccess)
+        return fail(FWI3D_EDEVICE, "stream for the source spectrum");
     int rc = FWI3D_OK;
-    if (hcopy(d_sig, signal, sizeof(double) * it) != hipSuccess)
+    if (dalloc(&d_sig, (size_t)it) != hipSuccess)
+        rc = fail(FWI3D_ENOMEM, "signal buffer");
+    else if (hcopy(d_sig, signal, sizeof(double) * it) != hipSuccess)
         rc = fail(FWI3D_EDEVICE, "signal upload");
     if (!rc)
-        rc = device_rows_dft(nullptr, d_sig, 1, it, dt, omega0, nfreq, omegas, out);
+        rc = device_rows_dft(st, d_sig, 1, it, dt, omega0, nfreq, omegas, out);
     dfree(d_sig);
+    del_stream(st);
     if (rc)
         return rc;
     for (int f = 0; f < nfreq; ++f) {
@@ -1112,6 +1117,133 @@ int fwi3d_source_spectrum(double dt, double omega0, int it, const double *signal
         out[2 * f] = v.real();
         out[2 * f + 1] = v.imag();
     }
+    return FWI3D_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- complex-row DFTs on device
+
+// DFT of `rows` complex device rows of length nd into d_out, on stream st
+static int dft_rows_dev(hipStream_t st, const double *d_in, int rows, int nd, int sign,
+                        double scale, double *d_out)
+{
+    double *d_tw = nullptr;
+    if (dalloc(&d_tw, 2 * (size_t)nd) != hipSuccess)
+        return fail(FWI3D_ENOMEM, "twiddle table");
+    hipError_t e = mt::launch_twiddle(nd, d_tw, st);
+    if (e == hipSuccess)
+        e = mt::launch_dft_rows(d_in, rows, nd, sign, scale, d_tw, d_out, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    dfree(d_tw);
+    if (e != hipSuccess)
+        return fail(FWI3D_EDEVICE, "row DFT: %s", hipGetErrorString(e));
+    return FWI3D_OK;
+}
+
+// Host rows in -> device DFT -> host rows out, on a private stream.  With
+// `jrow` (complex, nd), that row is transformed the same way and multiplies
+// every transformed row before the download.
+static int dft_rows_host(const double *in, int rows, int nd, int sign, double scale, double *out,
+                         const double *jrow = nullptr)
+{
+    const size_t n = 2 * (size_t)rows * nd;
+    hipStream_t st = nullptr;
+    double *d_in = nullptr, *d_out = nullptr, *d_j = nullptr, *d_jw = nullptr;
+    int rc = FWI3D_OK;
+    if (new_stream(&st) != hipSuccess)
+        return fail(FWI3D_EDEVICE, "stream for a row DFT");
+    if (dalloc(&d_in, n) != hipSuccess || dalloc(&d_out, n) != hipSuccess ||
+        (jrow && (dalloc(&d_j, 2 * (size_t)nd) != hipSuccess ||
+                  dalloc(&d_jw, 2 * (size_t)nd) != hipSuccess)))
+        rc = fail(FWI3D_ENOMEM, "row DFT buffers (%d x %d)", rows, nd);
+    if (!rc && (hcopy(d_in, in, sizeof(double) * n) != hipSuccess ||
+                (jrow && hcopy(d_j, jrow, sizeof(double) * 2 * nd) != hipSuccess)))
+        rc = fail(FWI3D_EDEVICE, "row DFT upload");
+    if (!rc)
+        rc = dft_rows_dev(st, d_in, rows, nd, sign, scale, d_out);
+    if (!rc && jrow)
+        rc = dft_rows_dev(st, d_j, 1, nd, sign, scale, d_jw);
+    if (!rc) {
+        hipError_t e = jrow ? mt::launch_cmul_rows(d_out, d_jw, rows, nd, st) : hipSuccess;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+        if (e != hipSuccess)
+            rc = fail(FWI3D_EDEVICE, "row DFT: %s", hipGetErrorString(e));
+    }
+    dfree(d_in);
+    dfree(d_out);
+    dfree(d_j);
+    dfree(d_jw);
+    del_stream(st);
+    return rc;
+}
+
+extern "C" {
+
+int fwi3d_dft_rows(const double *in, int rows, int nd, int sign, double scale, double *out)
+{
+    if (!in || !out || rows <= 0 || nd <= 0 || (sign != 1 && sign != -1))
+        return fail(FWI3D_EINVAL, "bad row DFT request");
+    if (rows > 65535)
+        return fail(FWI3D_EINVAL, "row DFT: %d rows exceed 65535", rows);
+    return dft_rows_host(in, rows, nd, sign, scale, out);
+}
+
+int fwi3d_convolve_gj(const double *g_t, int rows, int nd, const double *j_t, double *out)
+{
+    if (!g_t || !j_t || !out || rows <= 0 || nd <= 0)
+        return fail(FWI3D_EINVAL, "bad G*J request");
+    if (rows > 65535)
+        return fail(FWI3D_EINVAL, "G*J: %d rows exceed 65535", rows);
+    std::vector<double> jc(2 * (size_t)nd, 0.0);
+    for (int n = 0; n < nd; ++n)
+        jc[2 * n] = j_t[n];
+    return dft_rows_host(g_t, rows, nd, 1, 1.0, out, jc.data());
+}
+
+int fwi3d_laplace_fft(fwi3d_shot *s, double *e_t, double *j_t, double *g_t)
+{
+    if (!s || !e_t || !j_t || !g_t)
+        return fail(FWI3D_EINVAL, "null argument");
+    if (s->signal.empty())
+        return fail(FWI3D_ESTATE, "shot has no source waveform");
+    if (s->nrec == 0)
+        return fail(FWI3D_ESTATE, "shot has no receivers");
+    const int it = s->it, rows = 3 * s->nrec;
+    if (2 * rows + 1 > 65535)
+        return fail(FWI3D_EINVAL, "laplace_fft: too many receivers");
+    const double dt = s->m->p.dt, omega0 = s->m->p.omega0;
+    const double om = 2.0 * M_PI / it / dt;
+    std::vector<double> omg(it);
+    for (int n = 0; n < it; ++n)
+        omg[n] = om * n;
+    // spectra at every bin: rows of E, then G = E/J, then J
+    std::vector<double> spec(2 * (size_t)(2 * rows + 1) * it);
+    double *ew = spec.data(), *gw = ew + 2 * (size_t)rows * it, *jw = gw + 2 * (size_t)rows * it;
+    int rc = device_rows_dft(s->st, s->d_traces, rows, it, dt, omega0, it, omg.data(), ew);
+    if (!rc)
+        rc = fwi3d_source_spectrum(dt, omega0, it, s->signal.data(), it, omg.data(), jw);
+    if (rc)
+        return rc;
+    for (int r = 0; r < rows; ++r)
+        for (int n = 0; n < it; ++n) {
+            const size_t q = (size_t)r * it + n;
+            const std::complex<double> g = std::complex<double>(ew[2 * q], ew[2 * q + 1]) /
+                                           std::complex<double>(jw[2 * n], jw[2 * n + 1]);
+            gw[2 * q] = g.real();
+            gw[2 * q + 1] = g.imag();
+        }
+    std::vector<double> t(spec.size());
+    if ((rc = dft_rows_host(spec.data(), 2 * rows + 1, it, -1, 2.0 / it / dt, t.data())))
+        return rc;
+    const size_t blk = 2 * (size_t)rows * it;
+    std::memcpy(e_t, t.data(), sizeof(double) * blk);
+    std::memcpy(g_t, t.data() + blk, sizeof(double) * blk);
+    std::memcpy(j_t, t.data() + 2 * blk, sizeof(double) * 2 * it);
     return FWI3D_OK;
 }
 

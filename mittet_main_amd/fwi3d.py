@@ -274,6 +274,49 @@ class Shot:
                                                 _dp(o)))
         return {"E": e, "J": j, "G": g, "OBS": o}
 
+    def laplace_fft(self) -> dict:
+        """satan_i/fwi3d_cpu_function_v1.4.c:1430-1564: E, J, G at every bin,
+        back-transformed to diffusive time.  E_t, G_t complex (3, nrec, it);
+        J_t complex (it,)."""
+        e = np.zeros((3, self.nrec, self.it), dtype=np.complex128)
+        g = np.zeros_like(e)
+        j = np.zeros(self.it, dtype=np.complex128)
+        check(_lib.load().fwi3d_laplace_fft(self._h, _dp(e), _dp(j), _dp(g)))
+        return {"E_t": e, "J_t": j, "G_t": g}
+
+
+def dft_rows(x: np.ndarray, sign: int, scale: float = 1.0) -> np.ndarray:
+    """scale * sum_n x[..., n] exp(sign 2 pi i k n / nd) over the last axis, on
+    the GPU (fwi3d_dft_rows).  sign -1 is FFTW_FORWARD, +1 FFTW_BACKWARD."""
+    a = np.ascontiguousarray(x, dtype=np.complex128)
+    nd = a.shape[-1]
+    out = np.zeros_like(a)
+    check(_lib.load().fwi3d_dft_rows(_dp(a), a.size // nd, nd, int(sign), float(scale), _dp(out)))
+    return out
+
+
+def to_diffusive_time(spec: np.ndarray, dt: float, sign: int = -1) -> np.ndarray:
+    """Diffusive-time signal from spectra at the bins n 2 pi/(nd dt), n < nd:
+    x_t[k] = 2/(nd dt) sum_n X[n] exp(sign 2 pi i k n / nd).  sign -1 is the C
+    code's FFTW_FORWARD (satan_i/fwi3d_cpu_function_v1.4.c:1507-1527), +1 the
+    Fortran f_to_d_e's FFTW_BACKWARD (ver0040/cpu_f_to_d_e.f90:281-320)."""
+    nd = np.shape(spec)[-1]
+    return dft_rows(spec, sign, 2.0 / nd / dt)
+
+
+def convolve_gj(g_t: np.ndarray, j_t: np.ndarray) -> np.ndarray:
+    """convolution_GJ_to_E (satan_i/fwi3d_cpu_function_v1.4.c:1566-1635):
+    FFTW_BACKWARD(G_t) * FFTW_BACKWARD(J_t) per row; g_t complex (..., nd),
+    j_t real (nd,)."""
+    g = np.ascontiguousarray(g_t, dtype=np.complex128)
+    j = np.ascontiguousarray(j_t, dtype=np.float64)
+    nd = g.shape[-1]
+    if j.shape != (nd,):
+        raise ValueError(f"j_t must have {nd} samples")
+    out = np.zeros_like(g)
+    check(_lib.load().fwi3d_convolve_gj(_dp(g), g.size // nd, nd, _dp(j), _dp(out)))
+    return out
+
 
 def source_spectrum(signal: np.ndarray, dt: float, omega0: float, omegas) -> np.ndarray:
     """J(omega) = csqrt(-2 omega0/(i omega)) sum J[n] w_n, J(0) := 2 omega0

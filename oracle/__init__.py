@@ -153,6 +153,59 @@ def fft_backward_real(x):
     return out
 
 
+def laplace_fft(trace, jsig, dt, omega0):
+    """laplace_fft for one trace (satan_i/fwi3d_cpu_function_v1.4.c:1430-1564):
+    E, J, G at every bin n < it (laplace_to_freq), then the FFTW_FORWARD
+    back-transform scaled by 2/(it dt), as numpy's forward FFT."""
+    it = len(trace)
+    lap = laplace_to_freq(trace, jsig, np.zeros(it), dt, omega0, it)
+    back = lambda x: np.fft.fft(x) * 2.0 / it / dt
+    return {"E_t": back(lap["E"]), "J_t": back(lap["J"]), "G_t": back(lap["G"])}
+
+
+def efld_fic(rel, t, cval, fmax_w, mu0=MU0_C):
+    """E_x of a unit x-directed dipole in a homogeneous whole space, fictitious
+    (wave) domain, at offset rel = (x, y, z) m and times t: the retarded dipole
+    field of satan_i/fwi3d_cpu_function_v1.4.c:1776-1818 (fwi3d_cpu_convert_v1.4.c:
+    207-240), E_x = mu0/(4 pi r) [(nx^2 - 1) p'' + (3 nx^2 - 1)(c/r p' + (c/r)^2 p)],
+    p(t) = sqrt(beta/pi) exp(-beta (t - r/c - t0)^2), beta = pi fmax^2, t0 = pi/fmax."""
+    x, y, z = (float(v) for v in rel)
+    r = np.sqrt(x * x + y * y + z * z)
+    beta, t0 = np.pi * fmax_w ** 2, np.pi / fmax_w
+    u = np.asarray(t, dtype=np.float64) - r / cval - t0
+    g = np.sqrt(beta / np.pi) * np.exp(-beta * u ** 2)
+    g1 = -2 * beta * u * g
+    g2 = g * (4 * beta ** 2 * u ** 2 - 2 * beta)
+    nx2 = (x / r) ** 2
+    return mu0 / (4 * np.pi * r) * ((nx2 - 1) * g2 + (3 * nx2 - 1) * (cval / r * g1 + (cval / r) ** 2 * g))
+
+
+def efld_dip(rel, t, sig=1.0, mu0=MU0_C):
+    """E_x(t) of a unit x-directed dipole in a whole space of conductivity sig,
+    diffusive domain (satan_i/fwi3d_cpu_function_v1.4.c:1726-1774,
+    fwi3d_cpu_convert_v1.4.c:164-201): with theta = sqrt(mu0 sig/(4 t)) and
+    u = theta r, E_x = A nx^2 - B, A = [(4 u^3 + 6 u) e^{-u^2}/sqrt(pi) +
+    3 erfc u]/(4 pi sig r^3), B = [(4 u^3 + 2 u) e^{-u^2}/sqrt(pi) +
+    erfc u]/(4 pi sig r^3); E_x(0) = 0.  Returns (E_x, its forward
+    difference dE_x/dt on the same samples, 0 at t[0])."""
+    from math import erfc
+    x, y, z = (float(v) for v in rel)
+    r = np.sqrt(x * x + y * y + z * z)
+    t = np.asarray(t, dtype=np.float64)
+    ex = np.zeros(t.shape)
+    for q, tq in enumerate(t):
+        if tq <= 0:
+            continue
+        u = np.sqrt(mu0 * sig / 4.0 / tq) * r
+        e = np.exp(-u * u)
+        a = ((4 / np.sqrt(np.pi) * u ** 3 + 6 / np.sqrt(np.pi) * u) * e + 3 * erfc(u)) / (4 * np.pi * sig * r ** 3)
+        b = ((4 / np.sqrt(np.pi) * u ** 3 + 2 / np.sqrt(np.pi) * u) * e + erfc(u)) / (4 * np.pi * sig * r ** 3)
+        ex[q] = a * (x / r) ** 2 - b
+    dex = np.zeros(t.shape)
+    dex[1:] = np.diff(ex) / np.diff(t)
+    return ex, dex
+
+
 def laplace_to_freq(trace, jsig, strue, dt, omega0, nbins):
     """Receiver spectra at bins n < nbins (satan_i/fwi3d_cpu_function_v1.4.c:1976-2089)."""
     it = len(trace)

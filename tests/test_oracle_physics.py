@@ -174,22 +174,32 @@ def test_whole_space_dipole_matches_retarded_field():
     rec1 = np.array([[c0 + 1 + a, c0 + 1 + b, c0 + 1 + c] for a, b, c in offs])
     tr, _ = S.forward(0, s, src1, nt, rec1=rec1, it=nt + 1)
     cval = np.sqrt(2 * OM0 / oracle.MU0_C / 1.0)
-    beta, t0 = np.pi * fmax ** 2, np.pi / fmax
     for r_i, (a, b, c) in enumerate(offs):
-        # Ex sits half a cell further along x than the cell centre
-        rx, ry, rz = (a) * h, b * h, c * h
-        r = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
-        nx2 = (rx / r) ** 2
-        t = (np.arange(nt) + 1) * dt - r / cval - t0   # sample n holds E after n+1 updates
-        g = np.sqrt(beta / np.pi) * np.exp(-beta * t ** 2)
-        g1 = -2 * beta * t * g
-        g2 = g * (4 * beta ** 2 * t ** 2 - 2 * beta)
-        ex = oracle.MU0_C / (4 * np.pi * r) * ((nx2 - 1) * g2 + (3 * nx2 - 1) * (cval / r * g1 + (cval / r) ** 2 * g))
+        # sample n holds E after n + 1 updates
+        ex = oracle.efld_fic((a * h, b * h, c * h), (np.arange(nt) + 1) * dt, cval, fmax)
         sim = tr[0, r_i, :nt]
         err = np.linalg.norm(sim - ex) / np.linalg.norm(ex)
         # 6-10 cells from a point source the lattice near field differs from the
         # continuum one by a few percent; a sign or scale error gives >= 50 %.
         assert err < 0.12, (offs[r_i], err)
+
+
+def test_efld_dip_limits_are_the_static_and_zero_fields():
+    """Long after the step the diffusive dipole field is the static field of
+    a current dipole in a conductor, (3 nx^2 - 1)/(4 pi sig r^3); just after
+    it the field has not arrived (erfc -> 0, e^{-u^2} -> 0)."""
+    sig = 0.7
+    for rel in ((1500.0, 0.0, 0.0), (0.0, 900.0, 0.0), (600.0, 800.0, -300.0)):
+        r = np.linalg.norm(rel)
+        nx2 = (rel[0] / r) ** 2
+        ex, dex = oracle.efld_dip(rel, np.array([0.0, 1e-6, 1e6]), sig=sig)
+        assert ex[0] == 0.0 and abs(ex[1]) < 1e-300
+        static = (3 * nx2 - 1) / (4 * np.pi * sig * r ** 3)
+        assert abs(ex[2] - static) <= 1e-4 * abs(static)
+    # forward difference of the samples
+    t = np.linspace(0.0, 2.0, 50)
+    ex, dex = oracle.efld_dip((1000.0, 0.0, 0.0), t)
+    np.testing.assert_allclose(np.cumsum(dex[1:] * np.diff(t)), ex[1:], rtol=1e-12, atol=1e-25)
 
 
 def test_dft_weights_closed_form():
