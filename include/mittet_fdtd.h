@@ -35,6 +35,9 @@ extern "C" {
 #define FWI3D_ESTATE (-4)   /* call not valid in the object's current state */
 #define FWI3D_EBUDGET (-5)  /* request exceeds the admission budget */
 
+/* src_comp of a Fortran-variant shot: magnetic source on Hz */
+#define FWI3D_SRC_HZ 5
+
 /* Field / trace component ids */
 enum { FWI3D_EX = 0, FWI3D_EY, FWI3D_EZ, FWI3D_HX, FWI3D_HY, FWI3D_HZ };
 
@@ -95,6 +98,25 @@ double fwi3d_lattice_time_3d(double minval, double omega0, double mu0,
  * satan_i/fwi3d_cpu_cpmlabc_v1.4.c:7-175).  stream: hipStream_t or NULL. */
 int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **out);
 int fwi3d_model_destroy(fwi3d_model *m);
+/* Variant 1 is the Fortran lineage (ver0040), beside the C v1.4 semantics of
+ * variant 0 (fwi3d_model_create):
+ *   profiles of ver0040/cpu_cpml4.f90:8-248 (1-based ramps, high side mirrored;
+ *     pass opt_to_max {1,1,1}, rcoef 0.01, alpha_max 0.2, c2 -0.04167 for the
+ *     Fortran constants, cpu_parameter_wafe.f90:35-37, 127-131);
+ *   per-cell permeability in the H coefficients (cpu_eh_field_cpml42.f90:51-53,
+ *     fwi3d_model_set_mu; mu0 everywhere until set);
+ *   shots with the magnetic source (src_comp FWI3D_SRC_HZ): Hz -= s[n] dt/mu/dx/dy/dz
+ *     at up to 4 points before the E update of step n (cpu_waveform.f90:46-51,
+ *     cpu_main.f90:104-148; step n takes s[n], the Fortran istep = n + 1);
+ *   E/H updates over the Fortran loop ranges and CPML over the Fortran shells,
+ *     psi_Hyx stencil as written there (cpu_eh_field_cpml42.f90:116-252,
+ *     cpu_cpml4.f90:271-474).
+ * Complex fields (the Fortran arrays are complex(8)) are two real runs: every
+ * coefficient is real, so the real and imaginary parts never mix. */
+int fwi3d_model_create_variant(const fwi3d_grid_params *p, int variant, void *stream,
+                               fwi3d_model **out);
+int fwi3d_model_set_mu(fwi3d_model *m, const double *mu);
+
 /* host copy of one 1-D profile (length ix/iy/iz) */
 int fwi3d_model_get_profile(const fwi3d_model *m, int axis, int which, double *out);
 
@@ -108,6 +130,9 @@ int fwi3d_media_coeff_3d(fwi3d_model *m, const double *sigma);
 
 int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **out);
 int fwi3d_shot_destroy(fwi3d_shot *s);
+/* Fortran variant: one more Hz source point with the shot's waveform (the
+ * second transmitter of ver0040/cpu_waveform.f90:49-51); at most 4 points. */
+int fwi3d_shot_add_source(fwi3d_shot *s, int i, int j, int k);
 /* device bytes this shot would allocate (admission control) */
 long long fwi3d_shot_bytes(const fwi3d_grid_params *p, const fwi3d_shot_params *sp);
 
@@ -161,7 +186,8 @@ int fwi3d_get_volume_dft(fwi3d_shot *s, double *out);
  *   X(omega) = sum_{n<it} trace[n] * dt*exp(-a n dt)*exp(i a n dt), a = sqrt(omega0*omega).
  * comps: bit mask of FWI3D_EX..FWI3D_HZ; out complex [6][nrec][nfreq] (unused comps 0).
  * Magnetic components get the f_to_d_h factor sqrt(-2*omega0/(i*omega))
- * (ver0040/cpu_f_to_d_h.f90:146-148) when h_factor != 0. */
+ * (ver0040/cpu_f_to_d_h.f90:146-148) when h_factor != 0; at omega = 0 (the
+ * reference's bin k = 0, where it divides by zero) they are NaN. */
 int fwi3d_receiver_dft(fwi3d_shot *s, int nfreq, const double *omegas, int h_factor,
                        double *out);
 /* f_to_d_e / f_to_d_h on already-accumulated spectra (host arrays, complex):

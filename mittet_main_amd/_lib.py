@@ -76,6 +76,10 @@ _SIGS = {
                                         C.c_void_p, C.c_void_p]),
     "fwi3d_laplace_to_freq": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p]),
+    "fwi3d_model_create_variant": (C.c_int, [C.POINTER(GridParams), C.c_int, C.c_void_p,
+                                             C.POINTER(C.c_void_p)]),
+    "fwi3d_model_set_mu": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fwi3d_shot_add_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "fwi3d_dft_rows": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]),
     "fwi3d_laplace_fft": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fwi3d_convolve_gj": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmittet_fdtd.so")
 SOURCES = [os.path.join(HERE, "csrc", f)
            for f in ("fdtd_kernels.hip", "fdtd_adjoint.hip", "fdtd_compat.hip", "fdtd_transform.hip",
-                     "fdtd_api.hip")]
+                     "fdtd_fortran.hip", "fdtd_api.hip")]
 HEADERS = [os.path.join(HERE, "csrc", "fdtd_dev.h"), os.path.join(HERE, "csrc", "fdtd_transform.h"),
            os.path.join(ROOT, "include", "mittet_fdtd.h")]
 

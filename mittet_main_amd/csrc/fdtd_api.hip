@@ -101,7 +101,7 @@ double axis_step(const fwi3d_grid_params &p, int a) { return a == 0 ? p.dx : (a 
 // c = sigma*(b-1)/(sigma+kappa*alpha)/kappa.  E quantities sit on integer
 // offsets, H quantities half a cell inward; offsets are formed in single
 // precision as the reference does, so the outermost cell exceeds sigma_max.
-void cpml_axis(const fwi3d_grid_params &p, int a, std::vector<double> &out)
+void cpml_axis(const fwi3d_grid_params &p, int a, std::vector<double> &out, int variant = 0)
 {
     const int n = axis_len(p, a), nc = p.ncpml;
     const double h = axis_step(p, a);
@@ -109,11 +109,24 @@ void cpml_axis(const fwi3d_grid_params &p, int a, std::vector<double> &out)
     auto at = [&](int row, int i) -> double & { return out[(size_t)row * n + i]; };
     const double sig_max = (p.nn + p.order + 1.0) * p.cmax * std::log(1.0 / p.rcoef) /
                            (2.0 * (nc * h)) * p.opt_to_max[a];
-    const double den = (double)((float)nc - 1.0f);
+    // Fortran ver0040 (cpu_cpml4.f90:21-60): 1-based ramps in double, the high
+    // side mirrors the low side, no float-typed offsets
+    const double den = variant == 1 ? (double)nc - 1.0 : (double)((float)nc - 1.0f);
     for (int i = 0; i < n; ++i) {
         bool shell = true;
         double oe = 0, om = 0, ae = 0, am = 0;  // E/H offsets for sigma/kappa and alpha
-        if (i < nc) {
+        const int lo = i < nc ? i : n - 1 - i;  // variant 1: mirrored 0-based position
+        if (variant == 1) {
+            if (i < nc || i >= n - nc) {
+                const double i1 = (double)(lo + 1);
+                oe = (double)nc - i1;
+                om = (double)nc - i1 - 0.5;
+                ae = i1;
+                am = i1 + 0.5;
+            } else {
+                shell = false;
+            }
+        } else if (i < nc) {
             oe = (double)(nc - i);
             om = (double)((float)(nc - i) - 0.5f);
             ae = (double)i;
@@ -193,8 +206,11 @@ struct fwi3d_model {
     double *d_eps = nullptr;
     double *d_cbi = nullptr;
     std::vector<double> sig;  // host copy (source scaling)
-    int version = 0;          // bumps on every media_coeff_3d
+    int version = 0;          // bumps on every media_coeff_3d (and set_mu)
     int nshots = 0;
+    int variant = 0;          // 0: C v1.4 semantics; 1: Fortran ver0040
+    double *d_mu = nullptr;   // variant 1: per-cell permeability
+    std::vector<double> mu;   // host copy (source scaling)
 };
 
 struct fwi3d_shot {
@@ -202,6 +218,7 @@ struct fwi3d_shot {
     hipStream_t st = nullptr;
     int it = 0, nrec = 0, nf = 0, ring = 1;
     int src_comp = 0, src_cell = -1;
+    std::vector<int> src_h;   // variant 1: Hz source cells
     std::vector<double> signal, omega;
     std::vector<int> rec;
     double *d_e = nullptr, *d_h = nullptr, *d_psi[3] = {nullptr, nullptr, nullptr};
@@ -251,9 +268,25 @@ int ensure_amp(fwi3d_shot *s)
     fwi3d_model *m = s->m;
     if (m->version == 0)
         return fail(FWI3D_ESTATE, "fwi3d_media_coeff_3d has not been called on the model");
-    if (s->amp_version == m->version || s->src_cell < 0)
+    if (s->amp_version == m->version || (s->src_cell < 0 && s->src_h.empty()))
         return FWI3D_OK;
     const fwi3d_grid_params &p = m->p;
+    if (m->variant == 1) {
+        // Hz -= signal(istep)*dt/myu/dx/dy/dz at every point (ver0040/cpu_waveform.f90:46-51);
+        // step n takes s[n] (the Fortran istep = n + 1)
+        const size_t L = (size_t)std::max(s->it - 1, 1);
+        std::vector<double> amp(4 * L, 0.0);
+        for (size_t q = 0; q < s->src_h.size(); ++q) {
+            const double mu = m->mu[(size_t)s->src_h[q]];
+            for (int n = 0; n + 1 < s->it; ++n)
+                amp[q * L + n] = s->signal[n] * p.dt / mu / p.dx / p.dy / p.dz;
+        }
+        HIP_TRY(hipMemcpyAsync(s->d_amp, amp.data(), sizeof(double) * amp.size(),
+                               hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipStreamSynchronize(s->st));
+        s->amp_version = m->version;
+        return FWI3D_OK;
+    }
     std::vector<double> amp((size_t)std::max(s->it - 1, 1), 0.0);
     const double sg = m->sig[(size_t)s->src_cell];
     for (int n = 0; n + 1 < s->it; ++n)
@@ -293,6 +326,14 @@ struct Fam {
 int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t st)
 {
     hipError_t e;
+    if (f.kind == 4) {
+        if (!h)
+            HIP_TRY(mt::launch_source_hz(d, ns, j, st));
+        e = h ? mt::launch_hfield_v(d, ns, f.pre, f.nblocks, j, st)
+              : mt::launch_efield_v(d, ns, f.pre, f.nblocks, j, st);
+        HIP_TRY(e);
+        return FWI3D_OK;
+    }
     if (f.kind == 2)
         e = h ? mt::launch_hfield_z(d, ns, f.pre, f.nblocks, j, 1, st)
               : mt::launch_efield_z(d, ns, f.pre, f.nblocks, j, 1, st);
@@ -366,9 +407,17 @@ double fwi3d_lattice_time_3d(double minval, double omega0, double mu0, double dx
 
 int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **out)
 {
+    return fwi3d_model_create_variant(p, 0, stream, out);
+}
+
+int fwi3d_model_create_variant(const fwi3d_grid_params *p, int variant, void *stream,
+                               fwi3d_model **out)
+{
     if (!p || !out)
         return fail(FWI3D_EINVAL, "null argument");
     *out = nullptr;
+    if (variant != 0 && variant != 1)
+        return fail(FWI3D_EINVAL, "variant must be 0 (C v1.4) or 1 (Fortran ver0040)");
     if (p->ncpml < 2)
         return fail(FWI3D_EINVAL, "ncpml must be >= 2 (got %d)", p->ncpml);
     for (int a = 0; a < 3; ++a) {
@@ -385,6 +434,7 @@ int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **o
         return fail(FWI3D_EINVAL, "dt, omega0, mu0 and cmax must be > 0");
     fwi3d_model *m = new fwi3d_model();
     m->p = *p;
+    m->variant = variant;
     if (stream) {
         m->stream = (hipStream_t)stream;
     } else {
@@ -396,7 +446,7 @@ int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **o
         m->own_stream = true;
     }
     for (int a = 0; a < 3; ++a) {
-        cpml_axis(*p, a, m->prof[a]);
+        cpml_axis(*p, a, m->prof[a], variant);
         std::vector<double> dp;
         const int n = axis_len(*p, a);
         pack_device_profile(m->prof[a], n, dp);
@@ -413,7 +463,34 @@ int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **o
         fwi3d_model_destroy(m);
         return fail(FWI3D_ENOMEM, "cannot allocate %zu-cell model", N);
     }
+    if (variant == 1) {
+        // permeability mu0 everywhere until fwi3d_model_set_mu
+        m->mu.assign(N, p->mu0);
+        if (dalloc(&m->d_mu, N) != hipSuccess ||
+            hcopy(m->d_mu, m->mu.data(), sizeof(double) * N) != hipSuccess) {
+            fwi3d_model_destroy(m);
+            return fail(FWI3D_ENOMEM, "cannot allocate the permeability volume");
+        }
+    }
     *out = m;
+    return FWI3D_OK;
+}
+
+int fwi3d_model_set_mu(fwi3d_model *m, const double *mu)
+{
+    if (!m || !mu)
+        return fail(FWI3D_EINVAL, "null argument");
+    if (m->variant != 1)
+        return fail(FWI3D_EINVAL, "per-cell permeability belongs to the Fortran variant (1)");
+    const size_t N = (size_t)m->p.ix * m->p.iy * m->p.iz;
+    for (size_t c = 0; c < N; ++c)
+        if (!(mu[c] > 0) || !std::isfinite(mu[c]))
+            return fail(FWI3D_EINVAL, "mu[%zu] must be finite and > 0", c);
+    m->mu.assign(mu, mu + N);
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    if (hcopy(m->d_mu, mu, sizeof(double) * N) != hipSuccess)
+        return fail(FWI3D_EDEVICE, "permeability upload failed");
+    m->version++;   // source amplitudes depend on mu
     return FWI3D_OK;
 }
 
@@ -428,6 +505,7 @@ int fwi3d_model_destroy(fwi3d_model *m)
     dfree(m->d_sig);
     dfree(m->d_eps);
     dfree(m->d_cbi);
+    dfree(m->d_mu);
     if (m->own_stream && m->stream)
         del_stream(m->stream);
     delete m;
@@ -483,8 +561,12 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     const fwi3d_grid_params &p = m->p;
     if (sp->it < 2)
         return fail(FWI3D_EINVAL, "it must be >= 2");
-    if (sp->src_comp < 0 || sp->src_comp > 2)
+    const bool magnetic = m->variant == 1;
+    if (!magnetic && (sp->src_comp < 0 || sp->src_comp > 2))
         return fail(FWI3D_EINVAL, "src_comp must be 0, 1 or 2");
+    if (magnetic && sp->src_comp != FWI3D_SRC_HZ)
+        return fail(FWI3D_EINVAL, "Fortran-variant shots take the magnetic Hz source (src_comp %d)",
+                    FWI3D_SRC_HZ);
     const bool has_src = sp->signal != nullptr;
     if (has_src && (sp->src_i < 0 || sp->src_i >= p.ix || sp->src_j < 0 || sp->src_j >= p.iy ||
                     sp->src_k < 0 || sp->src_k >= p.iz))
@@ -521,7 +603,11 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     s->ring = ring;
     s->src_comp = sp->src_comp;
     if (has_src) {
-        s->src_cell = (sp->src_k * p.iy + sp->src_j) * p.ix + sp->src_i;
+        const int cell = (sp->src_k * p.iy + sp->src_j) * p.ix + sp->src_i;
+        if (magnetic)
+            s->src_h.push_back(cell);
+        else
+            s->src_cell = cell;
         s->signal.assign(sp->signal, sp->signal + sp->it);
     }
     for (int r = 0; r < sp->nrec; ++r) {
@@ -546,7 +632,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
               dalloc(&s->d_psi[0], mt::PSI_N * slab[0]) == hipSuccess &&
               dalloc(&s->d_psi[1], mt::PSI_N * slab[1]) == hipSuccess &&
               dalloc(&s->d_psi[2], mt::PSI_N * slab[2]) == hipSuccess &&
-              dalloc(&s->d_amp, (size_t)sp->it) == hipSuccess &&
+              dalloc(&s->d_amp, (size_t)sp->it * (magnetic ? 4 : 1)) == hipSuccess &&
               dalloc(&s->d_traces, std::max<size_t>(1, 6 * (size_t)sp->nrec * sp->it)) == hipSuccess &&
               dalloc(&s->d_rec, std::max<size_t>(1, (size_t)sp->nrec)) == hipSuccess &&
               dalloc(&s->d_step, 1) == hipSuccess && dalloc(&s->d_pre, 2) == hipSuccess &&
@@ -644,8 +730,22 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.w = s->d_w;
     d.acc = s->d_acc;
     d.step = s->d_step;
+    d.variant = m->variant;
+    d.mu = m->d_mu;
+    d.nsrc_h = (int)s->src_h.size();
+    for (int q = 0; q < d.nsrc_h; ++q)
+        d.src_h[q] = s->src_h[q];
     s->nblocks_op = d.nbx * d.nby * p.iz;
     s->kind = sp->zchunk > 0 ? 2 : d.pair;
+    if (magnetic) {
+        // every 32 x 8 tile, one cell per lane (fdtd_fortran.hip)
+        d.pair = 0;
+        d.ftx = tiles(p.ix, mt::ZX);
+        d.fty = tiles(p.iy, mt::ZY);
+        d.fb = d.ftx * d.fty * p.iz;
+        d.fb_pad = tiles(d.fb, 8) * 8;
+        s->kind = 4;
+    }
     // family 3: LDS z-marching kernels over the CPML-free interior (rows start
     // 128-byte aligned, at ncpml rounded up to 16 cells), per-cell kernels over
     // the tiles that hold any other cell
@@ -711,6 +811,28 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     return FWI3D_OK;
 }
 
+int fwi3d_shot_add_source(fwi3d_shot *s, int i, int j, int k)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    if (s->m->variant != 1)
+        return fail(FWI3D_EINVAL, "extra source points belong to the Fortran variant (1)");
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot is in a batch");
+    if (s->signal.empty())
+        return fail(FWI3D_ESTATE, "shot has no source waveform");
+    const fwi3d_grid_params &p = s->m->p;
+    if (i < 0 || i >= p.ix || j < 0 || j >= p.iy || k < 0 || k >= p.iz)
+        return fail(FWI3D_EINVAL, "source cell (%d,%d,%d) outside the grid", i, j, k);
+    if (s->src_h.size() >= 4)
+        return fail(FWI3D_EINVAL, "at most 4 source points");
+    s->src_h.push_back((k * p.iy + j) * p.ix + i);
+    s->desc.nsrc_h = (int)s->src_h.size();
+    s->desc.src_h[s->desc.nsrc_h - 1] = s->src_h.back();
+    s->amp_version = -1;
+    return upload_desc(s);
+}
+
 int fwi3d_shot_destroy(fwi3d_shot *s)
 {
     if (!s)
@@ -771,6 +893,9 @@ static int single_op(fwi3d_shot *s, int which, int sign, int step)
     int rc = check_shot_ready(s);
     if (rc)
         return rc;
+    if (s->m->variant != 0)
+        return fail(FWI3D_EINVAL, "the per-operator calls follow the C reference; Fortran-variant "
+                                  "shots run through fwi3d_step_n / fwi3d_run");
     if (s->batch)
         return fail(FWI3D_ESTATE, "shot is in a batch");
     if (sign != 1 && sign != -1)
@@ -1049,7 +1174,11 @@ static int device_rows_dft(hipStream_t st, const double *d_rows, int nrows, int 
 
 static std::complex<double> h_factor(double omega0, double om)
 {
-    // sqrt(-2*omega0/(i*omega)), ver0040/cpu_f_to_d_h.f90:146-148
+    // sqrt(-2*omega0/(i*omega)), ver0040/cpu_f_to_d_h.f90:146-148.  At omega = 0
+    // (the reference's bin k = 0) the factor divides by zero and grows as
+    // omega^(-1/2): no finite value exists, so the bin is NaN, not inf.
+    if (om == 0.0)
+        return {std::nan(""), std::nan("")};
     return std::sqrt(std::complex<double>(-2.0 * omega0, 0.0) / std::complex<double>(0.0, 1.0) / om);
 }
 
@@ -1493,6 +1622,8 @@ static int grad_new(fwi3d_shot *s, int levels, int segment, fwi3d_grad **out)
     *out = nullptr;
     if (s->nf || s->ring != 1)
         return fail(FWI3D_EINVAL, "gradient shots must not carry a volume DFT");
+    if (s->m->variant != 0)
+        return fail(FWI3D_EINVAL, "the adjoint follows the C reference's step (variant 0)");
     if (s->batch)
         return fail(FWI3D_ESTATE, "shot is in a batch");
     if (levels < 1 || levels > 8)

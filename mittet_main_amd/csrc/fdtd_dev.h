@@ -69,6 +69,10 @@ struct ShotDev {
     const double *w;          // complex [it][nf]
     double *acc;              // complex [3][nf] rows of astride
     int *step;                // absolute step of local index j = 0 of the current chunk
+    int variant;              // 0: C v1.4 semantics; 1: Fortran ver0040 (fdtd_fortran.hip)
+    int nsrc_h;               // variant 1: magnetic (Hz) source points; amp[q * (it - 1) + n]
+    int src_h[4];             // their cells
+    const double *mu;         // variant 1: per-cell permeability
 };
 
 enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
@@ -121,6 +125,13 @@ hipError_t launch_efield_l(const ShotDev *d, int ns, const int *pre, int nblocks
                            int sign, hipStream_t st);
 hipError_t launch_hfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
+// Fortran-variant step (fdtd_fortran.hip): Hz source points before the E
+// update, then per-cell E + CPML-E and H + CPML-H over 32 x 8 tiles
+hipError_t launch_source_hz(const ShotDev *d, int ns, int j, hipStream_t st);
+hipError_t launch_efield_v(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           hipStream_t st);
+hipError_t launch_hfield_v(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                           hipStream_t st);
 hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
                             hipStream_t st);
 hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int L,

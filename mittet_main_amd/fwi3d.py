@@ -33,6 +33,14 @@ RCOEF = float(np.float32(0.01))           # satan_i/fwi3d_cpu_cpmlabc_v1.4.c:13
 ALPHA_MAX = float(np.float32(0.2))        # satan_i/fwi3d_cpu_cpmlabc_v1.4.c:21
 DT_FACTOR_REF = float(np.float32(0.999))  # satan_i/fwi3d_cpu_lattice_v1.4.c:39
 
+# The Fortran lineage (model variant 1): constants of ver0040/cpu_parameter_wafe.f90
+SRC_HZ = 5                                # magnetic source on Hz (FWI3D_SRC_HZ)
+F_C2 = -0.04167                           # :37, a double literal there
+F_MU0 = 1.2566370614e-6                   # :86
+F_PI = 3.14159265358979                   # :32, the pi of every Fortran formula
+F_RCOEF = 0.01                            # :131
+F_ALPHA_MAX = 0.2                         # :127
+
 EX, EY, EZ, HX, HY, HZ = range(6)
 PROFILES = ("ESIG", "MSIG", "EKAP", "MKAP", "AE", "AM", "BE", "BH", "CE", "CH", "KED", "KHD")
 
@@ -101,16 +109,63 @@ class Grid:
                           self.alpha_max, (C.c_double * 3)(*self.opt_to_max))
 
 
-class Model:
-    """Grid + conductivity + CPML profiles resident in HBM."""
+def fortran_grid(ix: int, iy: int, iz: int, ncpml: int, dx: float, dy: float, dz: float,
+                 dt: float, omega0: float, cmax: float) -> "Grid":
+    """Grid with the Fortran lineage's constants: c2 = -0.04167, mu0 =
+    1.2566370614e-6, Rcoef 0.01, alpha_max 0.2, no sigma_max factor on x
+    (ver0040/cpu_parameter_wafe.f90:35-37, 86, 127-131; cpu_cpml4.f90:21)."""
+    return Grid(ix, iy, iz, ncpml, dx, dy, dz, dt, omega0, cmax, mu0=F_MU0, c1=1.125, c2=F_C2,
+                rcoef=F_RCOEF, alpha_max=F_ALPHA_MAX, opt_to_max=(1.0, 1.0, 1.0))
 
-    def __init__(self, grid: Grid, stream: Optional[int] = None):
+
+def fortran_dt(dx: float, cmax: float) -> float:
+    """dt = 0.999 dx/cmax/sqrt(3)/1.16667 (ver0040/cpu_parameter_wafe.f90:118)."""
+    return 0.999 * dx / cmax / np.sqrt(3.0) / 1.16667
+
+
+def fortran_waveform(nstep: int, dt: float, fmax: float) -> np.ndarray:
+    """The Fortran source s(istep), istep = 1..nstep, as written in
+    ver0040/cpu_waveform.f90:29: -(2 beta (t - t0) sqrt(beta/pi)) exp(-beta (t - t0)**2),
+    t = istep dt, t0 = pi/fmax, beta = pi fmax**2 with the Fortran pi.  Element n
+    drives step n of a variant-1 shot."""
+    t0 = F_PI / fmax
+    beta = F_PI * fmax ** 2.0
+    t = np.arange(1, nstep + 1) * dt
+    return -(2.0 * beta * (t - t0) * np.sqrt(beta / F_PI)) * np.exp(-beta * (t - t0) ** 2.0)
+
+
+def time_reversed_residual(observed: np.ndarray, computed: np.ndarray) -> np.ndarray:
+    """Drive signal of the residual back-injection (read_source_3d_res,
+    cw_19/ver0030/cpu_waveform.f90:62-85): the receiver residual
+    observed - computed, reversed in time.  A variant-1 shot with this signal
+    at the receiver's Hz points is that injection."""
+    o = np.asarray(observed, dtype=np.float64)
+    c = np.asarray(computed, dtype=np.float64)
+    if o.shape != c.shape:
+        raise ValueError("observed and computed traces differ in length")
+    return np.ascontiguousarray((o - c)[::-1])
+
+
+class Model:
+    """Grid + conductivity + CPML profiles resident in HBM.  variant 1 selects
+    the Fortran lineage's operators (fwi3d_model_create_variant)."""
+
+    def __init__(self, grid: Grid, stream: Optional[int] = None, variant: int = 0):
         self.grid = grid
+        self.variant = int(variant)
         self._p = grid.to_c()
         h = C.c_void_p()
-        check(_lib.load().fwi3d_model_create(C.byref(self._p), stream, C.byref(h)))
+        check(_lib.load().fwi3d_model_create_variant(C.byref(self._p), self.variant, stream,
+                                                     C.byref(h)))
         self._h = h
         self._shots = []
+
+    def set_mu(self, mu: np.ndarray) -> None:
+        """Per-cell permeability (variant 1), [iz, iy, ix] H/m."""
+        m = np.ascontiguousarray(mu, dtype=np.float64).reshape(-1)
+        if m.size != self.grid.ncells:
+            raise ValueError(f"mu has {m.size} cells, grid has {self.grid.ncells}")
+        check(_lib.load().fwi3d_model_set_mu(self._h, _dp(m)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -190,6 +245,10 @@ class Shot:
     __del__ = close
 
     # ---- per-step operators, reference names
+    def add_source(self, i: int, j: int, k: int) -> None:
+        """Another Hz source point with this shot's waveform (variant 1)."""
+        check(_lib.load().fwi3d_shot_add_source(self._h, int(i), int(j), int(k)))
+
     def set_zero_eh(self):
         check(_lib.load().fwi3d_set_zero_eh(self._h))
 

@@ -104,6 +104,12 @@ def lib():
         L.or_fft_backward_real.argtypes = [i, dp, dp]
         L.or_laplace_to_freq.argtypes = [i, d, d, i, dp, dp, dp, dp, dp, dp, dp]
         L.or_num_threads.restype = i
+        for nm in ("orf_init_cpml", "orf_e_field_cpml4", "orf_cpml_e4", "orf_h_field_cpml4",
+                   "orf_cpml_h4"):
+            getattr(L, nm).argtypes = [vp]
+        L.orf_media_coeff.argtypes = [vp, dp]
+        L.orf_source_hz.argtypes = [vp, i, ip, dp, d]
+        L.orf_forward.argtypes = [vp, i, ip, dp, dp, i, i, ip, i, dp]
         L.or_laplace_volume.argtypes = [i, i, i, dp, dp, dp, dp]
         L.or_residual_e.argtypes = [i, i, dp, dp, dp, dp]
         L.or_sensitivity.argtypes = [i, i, i, dp, dp, dp]
@@ -259,6 +265,42 @@ class OracleSim:
 
     def set_zero_eh(self):
         lib().or_set_zero_eh(self._s)
+
+    # ---- Fortran lineage (ver0040), see fwi3d_oracle.h
+    def f_init_cpml(self):
+        lib().orf_init_cpml(self._s)
+
+    def f_media_coeff(self, mu):
+        self._mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(-1)
+        lib().orf_media_coeff(self._s, _dp(self._mu))
+
+    def f_source_hz(self, cells, sval):
+        c = np.ascontiguousarray(cells, dtype=np.int32)
+        lib().orf_source_hz(self._s, c.size, _ip(c), _dp(self._mu), float(sval))
+
+    def f_e_field_cpml4(self):
+        lib().orf_e_field_cpml4(self._s)
+
+    def f_cpml_e4(self):
+        lib().orf_cpml_e4(self._s)
+
+    def f_h_field_cpml4(self):
+        lib().orf_h_field_cpml4(self._s)
+
+    def f_cpml_h4(self):
+        lib().orf_cpml_h4(self._s)
+
+    def f_forward(self, cells, signal, nsteps, rec_cells, it):
+        """Fortran-lineage steps (source before E); traces (6, nrec, it) of E
+        and H after each step.  cells / rec_cells: 0-based flat indices."""
+        c = np.ascontiguousarray(cells, dtype=np.int32)
+        r = np.ascontiguousarray(rec_cells, dtype=np.int32)
+        sig = None if signal is None else np.ascontiguousarray(signal, dtype=np.float64)
+        tr = np.zeros((6, max(r.size, 1), it))
+        lib().orf_forward(self._s, c.size, _ip(c), _dp(self._mu),
+                          _dp(sig) if sig is not None else None, nsteps, r.size,
+                          _ip(r) if r.size else None, it, _dp(tr))
+        return tr[:, :r.size]
 
     def e_field_cpml42(self, sign=1):
         lib().or_e_field_cpml42(self._s, sign)

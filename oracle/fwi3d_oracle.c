@@ -732,3 +732,281 @@ void or_update_para2(int ix, int iy, int iz, int nc, int iseabed, const double *
             }
     memset(grad, 0, sizeof(double) * (size_t)ix * iy * iz);
 }
+
+/* ===================================================================== */
+/* Fortran lineage (ver0040), restated operator by operator over the full */
+/* volume in the Fortran loop order; 1-based Fortran ranges written as    */
+/* 0-based half-open ones.  Parity of this part is pinned by mathematics  */
+/* only (no Fortran compiler in this image): see tests/test_oracle_fortran.py. */
+/* ===================================================================== */
+
+#define FIDX(i, j, k) (((long)(k) * iy + (j)) * ix + (i))
+
+/* ver0040/cpu_cpml4.f90:8-248: ramps on the 1-based index i = 1..ncpml,
+ * sigma_max without a per-axis factor, high side = mirror of the low side */
+void orf_init_cpml(or_state *s)
+{
+    const or_params *p = &s->p;
+    const int nc = p->ncpml;
+    for (int a = 0; a < 3; ++a) {
+        const int n = axis_len(p, a);
+        const double h = axis_step(p, a);
+        const double delta = nc * h;
+        const double smax = (p->nn + p->order + 1.0) * p->cmax * log(1.0 / p->rcoef) / (2.0 * delta);
+        double **q = s->prof[a];
+        for (int r = 0; r < n; ++r) {
+            q[OR_P_ESIG][r] = q[OR_P_MSIG][r] = 0.0;
+            q[OR_P_EKAP][r] = q[OR_P_MKAP][r] = 1.0;
+            q[OR_P_AE][r] = q[OR_P_AM][r] = 0.0;
+            q[OR_P_BE][r] = q[OR_P_BH][r] = q[OR_P_CE][r] = q[OR_P_CH][r] = 0.0;
+        }
+        for (int i = 1; i <= nc; ++i) {
+            const int r = i - 1;
+            const double den = (double)nc - 1.0;
+            q[OR_P_ESIG][r] = smax * pow(((double)nc - (double)i) / den, p->nn + p->order);
+            q[OR_P_MSIG][r] = smax * pow(((double)nc - (double)i - 0.5) / den, p->nn + p->order);
+            q[OR_P_EKAP][r] = 1.0 + (p->kappa_max - 1.0) * pow(((double)nc - (double)i) / den, p->nn);
+            q[OR_P_MKAP][r] = 1.0 + (p->kappa_max - 1.0) * pow(((double)nc - (double)i - 0.5) / den, p->nn);
+            q[OR_P_AE][r] = p->alpha_max * pow(((double)i) / den, p->ma);
+            q[OR_P_AM][r] = p->alpha_max * pow(((double)i + 0.5) / den, p->ma);
+            q[OR_P_BE][r] = exp(-(q[OR_P_ESIG][r] / q[OR_P_EKAP][r] + q[OR_P_AE][r]) * p->dt);
+            q[OR_P_BH][r] = exp(-(q[OR_P_MSIG][r] / q[OR_P_MKAP][r] + q[OR_P_AM][r]) * p->dt);
+            q[OR_P_CE][r] = q[OR_P_ESIG][r] * (q[OR_P_BE][r] - 1.0) /
+                            (q[OR_P_ESIG][r] + q[OR_P_EKAP][r] * q[OR_P_AE][r]) / q[OR_P_EKAP][r];
+            q[OR_P_CH][r] = q[OR_P_MSIG][r] * (q[OR_P_BH][r] - 1.0) /
+                            (q[OR_P_MSIG][r] + q[OR_P_MKAP][r] * q[OR_P_AM][r]) / q[OR_P_MKAP][r];
+        }
+        for (int i = 1; i <= nc; ++i)
+            for (int id = 0; id < OR_NPROFILES; ++id)
+                q[id][n - i] = q[id][i - 1];
+        for (int r = 0; r < n; ++r) {
+            q[OR_P_KED][r] = q[OR_P_EKAP][r] * h;
+            q[OR_P_KHD][r] = q[OR_P_MKAP][r] * h;
+        }
+    }
+}
+
+/* ver0040/cpu_eh_field_cpml42.f90:8-57: coefficient volumes, per-cell mu in db */
+void orf_media_coeff(or_state *s, const double *mu)
+{
+    const or_params *p = &s->p;
+    const int ix = p->ix, iy = p->iy, iz = p->iz;
+    const double dt = p->dt;
+    for (int k = 0; k < iz; ++k)
+        for (int j = 0; j < iy; ++j)
+            for (int i = 0; i < ix; ++i) {
+                const long c = FIDX(i, j, k);
+                const double eps2 = s->vol[OR_SIG][c] / 2.0 / p->omega0;
+                const int idx[3] = {i, j, k};
+                for (int d = 0; d < 3; ++d) {
+                    const double es = s->prof[d][OR_P_ESIG][idx[d]];
+                    const double ms = s->prof[d][OR_P_MSIG][idx[d]];
+                    s->vol[OR_CA_X + d][c] = (1.0 - ((es * dt) / (2.0 * eps2))) / (1.0 + ((es * dt) / (2.0 * eps2)));
+                    s->vol[OR_CB_X + d][c] = dt / eps2 / (1.0 + (es * dt) / (2.0 * eps2));
+                    s->vol[OR_DA_X + d][c] = (1.0 - ((ms * dt) / (2.0 * eps2))) / (1.0 + ((ms * dt) / (2.0 * eps2)));
+                    s->vol[OR_DB_X + d][c] = dt / mu[c] / (1.0 + (ms * dt) / (2.0 * eps2));
+                }
+            }
+}
+
+/* ver0040/cpu_waveform.f90:46-51: Hz(src) -= signal*dt/myu(src)/dx/dy/dz, each point in turn */
+void orf_source_hz(or_state *s, int npts, const int *cells, const double *mu, double sval)
+{
+    const or_params *p = &s->p;
+    for (int q = 0; q < npts; ++q) {
+        const long c = cells[q];
+        s->vol[OR_HZ][c] = s->vol[OR_HZ][c] - sval * p->dt / mu[c] / p->dx / p->dy / p->dz;
+    }
+}
+
+/* ver0040/cpu_eh_field_cpml42.f90:116-156 */
+void orf_e_field_cpml4(or_state *s)
+{
+    const or_params *p = &s->p;
+    const int ix = p->ix, iy = p->iy, iz = p->iz;
+    const long sy = ix, sz = (long)ix * iy;
+    const double c1 = p->c1, c2 = p->c2;
+    double *Ex = s->vol[OR_EX], *Ey = s->vol[OR_EY], *Ez = s->vol[OR_EZ];
+    const double *Hx = s->vol[OR_HX], *Hy = s->vol[OR_HY], *Hz = s->vol[OR_HZ];
+    const double *kedx = s->prof[0][OR_P_KED], *kedy = s->prof[1][OR_P_KED], *kedz = s->prof[2][OR_P_KED];
+    for (int k = 2; k < iz - 1; ++k)
+        for (int j = 2; j < iy - 1; ++j)
+            for (int i = 0; i < ix; ++i) {
+                const long c = FIDX(i, j, k);
+                Ex[c] = s->vol[OR_CA_X][c] * Ex[c] +
+                        s->vol[OR_CB_X][c] * ((c1 * Hz[c] - c1 * Hz[c - sy] + c2 * Hz[c + sy] - c2 * Hz[c - 2 * sy]) / kedy[j] -
+                                              (c1 * Hy[c] - c1 * Hy[c - sz] + c2 * Hy[c + sz] - c2 * Hy[c - 2 * sz]) / kedz[k]);
+            }
+    for (int k = 2; k < iz - 1; ++k)
+        for (int j = 0; j < iy; ++j)
+            for (int i = 2; i < ix - 1; ++i) {
+                const long c = FIDX(i, j, k);
+                Ey[c] = s->vol[OR_CA_Y][c] * Ey[c] +
+                        s->vol[OR_CB_Y][c] * ((c1 * Hx[c] - c1 * Hx[c - sz] + c2 * Hx[c + sz] - c2 * Hx[c - 2 * sz]) / kedz[k] -
+                                              (c1 * Hz[c] - c1 * Hz[c - 1] + c2 * Hz[c + 1] - c2 * Hz[c - 2]) / kedx[i]);
+            }
+    for (int k = 0; k < iz; ++k)
+        for (int j = 2; j < iy - 1; ++j)
+            for (int i = 2; i < ix - 1; ++i) {
+                const long c = FIDX(i, j, k);
+                Ez[c] = s->vol[OR_CA_Z][c] * Ez[c] +
+                        s->vol[OR_CB_Z][c] * ((c1 * Hy[c] - c1 * Hy[c - 1] + c2 * Hy[c + 1] - c2 * Hy[c - 2]) / kedx[i] -
+                                              (c1 * Hx[c] - c1 * Hx[c - sy] + c2 * Hx[c + sy] - c2 * Hx[c - 2 * sy]) / kedy[j]);
+            }
+}
+
+/* ver0040/cpu_cpml4.f90:271-358: shells [3, ncpml+1] and [n-ncpml+1, n-1]
+ * (1-based) on their axis, every other axis in full */
+void orf_cpml_e4(or_state *s)
+{
+    const or_params *p = &s->p;
+    const int ix = p->ix, iy = p->iy, iz = p->iz, nc = p->ncpml;
+    const long sy = ix, sz = (long)ix * iy;
+    const double c1 = p->c1, c2 = p->c2;
+    double *Ex = s->vol[OR_EX], *Ey = s->vol[OR_EY], *Ez = s->vol[OR_EZ];
+    const double *Hx = s->vol[OR_HX], *Hy = s->vol[OR_HY], *Hz = s->vol[OR_HZ];
+    double *pEzx = s->vol[OR_PSI_EZX], *pEyx = s->vol[OR_PSI_EYX], *pExy = s->vol[OR_PSI_EXY];
+    double *pEzy = s->vol[OR_PSI_EZY], *pEyz = s->vol[OR_PSI_EYZ], *pExz = s->vol[OR_PSI_EXZ];
+    const double *bex = s->prof[0][OR_P_BE], *cex = s->prof[0][OR_P_CE];
+    const double *bey = s->prof[1][OR_P_BE], *cey = s->prof[1][OR_P_CE];
+    const double *bez = s->prof[2][OR_P_BE], *cez = s->prof[2][OR_P_CE];
+    const int xr[2][2] = {{2, nc + 1}, {ix - nc, ix - 1}};
+    for (int side = 0; side < 2; ++side)
+        for (int k = 0; k < iz; ++k)
+            for (int j = 0; j < iy; ++j)
+                for (int i = xr[side][0]; i < xr[side][1]; ++i) {
+                    const long c = FIDX(i, j, k);
+                    pEzx[c] = bex[i] * pEzx[c] + cex[i] * (c1 * Hy[c] - c1 * Hy[c - 1] + c2 * Hy[c + 1] - c2 * Hy[c - 2]) / p->dx;
+                    pEyx[c] = bex[i] * pEyx[c] + cex[i] * (c1 * Hz[c] - c1 * Hz[c - 1] + c2 * Hz[c + 1] - c2 * Hz[c - 2]) / p->dx;
+                    Ez[c] = Ez[c] + s->vol[OR_CB_Z][c] * pEzx[c];
+                    Ey[c] = Ey[c] - s->vol[OR_CB_Y][c] * pEyx[c];
+                }
+    const int yr[2][2] = {{2, nc + 1}, {iy - nc, iy - 1}};
+    for (int side = 0; side < 2; ++side)
+        for (int k = 0; k < iz; ++k)
+            for (int j = yr[side][0]; j < yr[side][1]; ++j)
+                for (int i = 0; i < ix; ++i) {
+                    const long c = FIDX(i, j, k);
+                    pExy[c] = bey[j] * pExy[c] + cey[j] * (c1 * Hz[c] - c1 * Hz[c - sy] + c2 * Hz[c + sy] - c2 * Hz[c - 2 * sy]) / p->dy;
+                    pEzy[c] = bey[j] * pEzy[c] + cey[j] * (c1 * Hx[c] - c1 * Hx[c - sy] + c2 * Hx[c + sy] - c2 * Hx[c - 2 * sy]) / p->dy;
+                    Ex[c] = Ex[c] + s->vol[OR_CB_X][c] * pExy[c];
+                    Ez[c] = Ez[c] - s->vol[OR_CB_Z][c] * pEzy[c];
+                }
+    const int zr[2][2] = {{2, nc + 1}, {iz - nc, iz - 1}};
+    for (int side = 0; side < 2; ++side)
+        for (int k = zr[side][0]; k < zr[side][1]; ++k)
+            for (int j = 0; j < iy; ++j)
+                for (int i = 0; i < ix; ++i) {
+                    const long c = FIDX(i, j, k);
+                    pEyz[c] = bez[k] * pEyz[c] + cez[k] * (c1 * Hx[c] - c1 * Hx[c - sz] + c2 * Hx[c + sz] - c2 * Hx[c - 2 * sz]) / p->dz;
+                    pExz[c] = bez[k] * pExz[c] + cez[k] * (c1 * Hy[c] - c1 * Hy[c - sz] + c2 * Hy[c + sz] - c2 * Hy[c - 2 * sz]) / p->dz;
+                    Ey[c] = Ey[c] + s->vol[OR_CB_Y][c] * pEyz[c];
+                    Ex[c] = Ex[c] - s->vol[OR_CB_X][c] * pExz[c];
+                }
+}
+
+/* ver0040/cpu_eh_field_cpml42.f90:213-252 */
+void orf_h_field_cpml4(or_state *s)
+{
+    const or_params *p = &s->p;
+    const int ix = p->ix, iy = p->iy, iz = p->iz;
+    const long sy = ix, sz = (long)ix * iy;
+    const double c1 = p->c1, c2 = p->c2;
+    const double *Ex = s->vol[OR_EX], *Ey = s->vol[OR_EY], *Ez = s->vol[OR_EZ];
+    double *Hx = s->vol[OR_HX], *Hy = s->vol[OR_HY], *Hz = s->vol[OR_HZ];
+    const double *khdx = s->prof[0][OR_P_KHD], *khdy = s->prof[1][OR_P_KHD], *khdz = s->prof[2][OR_P_KHD];
+    for (int k = 1; k < iz - 2; ++k)
+        for (int j = 1; j < iy - 2; ++j)
+            for (int i = 0; i < ix; ++i) {
+                const long c = FIDX(i, j, k);
+                Hx[c] = s->vol[OR_DA_X][c] * Hx[c] -
+                        s->vol[OR_DB_X][c] * ((c1 * Ez[c + sy] - c1 * Ez[c] + c2 * Ez[c + 2 * sy] - c2 * Ez[c - sy]) / khdy[j] -
+                                              (c1 * Ey[c + sz] - c1 * Ey[c] + c2 * Ey[c + 2 * sz] - c2 * Ey[c - sz]) / khdz[k]);
+            }
+    for (int k = 1; k < iz - 2; ++k)
+        for (int j = 0; j < iy; ++j)
+            for (int i = 1; i < ix - 2; ++i) {
+                const long c = FIDX(i, j, k);
+                Hy[c] = s->vol[OR_DA_Y][c] * Hy[c] -
+                        s->vol[OR_DB_Y][c] * ((c1 * Ex[c + sz] - c1 * Ex[c] + c2 * Ex[c + 2 * sz] - c2 * Ex[c - sz]) / khdz[k] -
+                                              (c1 * Ez[c + 1] - c1 * Ez[c] + c2 * Ez[c + 2] - c2 * Ez[c - 1]) / khdx[i]);
+            }
+    for (int k = 0; k < iz; ++k)
+        for (int j = 1; j < iy - 2; ++j)
+            for (int i = 1; i < ix - 2; ++i) {
+                const long c = FIDX(i, j, k);
+                Hz[c] = s->vol[OR_DA_Z][c] * Hz[c] -
+                        s->vol[OR_DB_Z][c] * ((c1 * Ey[c + 1] - c1 * Ey[c] + c2 * Ey[c + 2] - c2 * Ey[c - 1]) / khdx[i] -
+                                              (c1 * Ex[c + sy] - c1 * Ex[c] + c2 * Ex[c + 2 * sy] - c2 * Ex[c - sy]) / khdy[j]);
+            }
+}
+
+/* ver0040/cpu_cpml4.f90:384-474: shells [2, ncpml] and [n-ncpml, n-2] (1-based);
+ * psi_Hyx as written there, with c2 where the difference has c1 (:400, 413) */
+void orf_cpml_h4(or_state *s)
+{
+    const or_params *p = &s->p;
+    const int ix = p->ix, iy = p->iy, iz = p->iz, nc = p->ncpml;
+    const long sy = ix, sz = (long)ix * iy;
+    const double c1 = p->c1, c2 = p->c2;
+    const double *Ex = s->vol[OR_EX], *Ey = s->vol[OR_EY], *Ez = s->vol[OR_EZ];
+    double *Hx = s->vol[OR_HX], *Hy = s->vol[OR_HY], *Hz = s->vol[OR_HZ];
+    double *pHzx = s->vol[OR_PSI_HZX], *pHyx = s->vol[OR_PSI_HYX], *pHxy = s->vol[OR_PSI_HXY];
+    double *pHzy = s->vol[OR_PSI_HZY], *pHyz = s->vol[OR_PSI_HYZ], *pHxz = s->vol[OR_PSI_HXZ];
+    const double *bhx = s->prof[0][OR_P_BH], *chx = s->prof[0][OR_P_CH];
+    const double *bhy = s->prof[1][OR_P_BH], *chy = s->prof[1][OR_P_CH];
+    const double *bhz = s->prof[2][OR_P_BH], *chz = s->prof[2][OR_P_CH];
+    const int xr[2][2] = {{1, nc}, {ix - nc - 1, ix - 2}};
+    for (int side = 0; side < 2; ++side)
+        for (int k = 0; k < iz; ++k)
+            for (int j = 0; j < iy; ++j)
+                for (int i = xr[side][0]; i < xr[side][1]; ++i) {
+                    const long c = FIDX(i, j, k);
+                    pHzx[c] = bhx[i] * pHzx[c] + chx[i] * (c1 * Ey[c + 1] - c1 * Ey[c] + c2 * Ey[c + 2] - c2 * Ey[c - 1]) / p->dx;
+                    pHyx[c] = bhx[i] * pHyx[c] + chx[i] * (c1 * Ez[c + 1] - c2 * Ez[c] + c2 * Ez[c + 2] - c2 * Ez[c - 1]) / p->dx;
+                    Hz[c] = Hz[c] - s->vol[OR_DB_Z][c] * pHzx[c];
+                    Hy[c] = Hy[c] + s->vol[OR_DB_Y][c] * pHyx[c];
+                }
+    const int yr[2][2] = {{1, nc}, {iy - nc - 1, iy - 2}};
+    for (int side = 0; side < 2; ++side)
+        for (int k = 0; k < iz; ++k)
+            for (int j = yr[side][0]; j < yr[side][1]; ++j)
+                for (int i = 0; i < ix; ++i) {
+                    const long c = FIDX(i, j, k);
+                    pHxy[c] = bhy[j] * pHxy[c] + chy[j] * (c1 * Ez[c + sy] - c1 * Ez[c] + c2 * Ez[c + 2 * sy] - c2 * Ez[c - sy]) / p->dy;
+                    pHzy[c] = bhy[j] * pHzy[c] + chy[j] * (c1 * Ex[c + sy] - c1 * Ex[c] + c2 * Ex[c + 2 * sy] - c2 * Ex[c - sy]) / p->dy;
+                    Hx[c] = Hx[c] - s->vol[OR_DB_X][c] * pHxy[c];
+                    Hz[c] = Hz[c] + s->vol[OR_DB_Z][c] * pHzy[c];
+                }
+    const int zr[2][2] = {{1, nc}, {iz - nc - 1, iz - 2}};
+    for (int side = 0; side < 2; ++side)
+        for (int k = zr[side][0]; k < zr[side][1]; ++k)
+            for (int j = 0; j < iy; ++j)
+                for (int i = 0; i < ix; ++i) {
+                    const long c = FIDX(i, j, k);
+                    pHyz[c] = bhz[k] * pHyz[c] + chz[k] * (c1 * Ex[c + sz] - c1 * Ex[c] + c2 * Ex[c + 2 * sz] - c2 * Ex[c - sz]) / p->dz;
+                    pHxz[c] = bhz[k] * pHxz[c] + chz[k] * (c1 * Ey[c + sz] - c1 * Ey[c] + c2 * Ey[c + 2 * sz] - c2 * Ey[c - sz]) / p->dz;
+                    Hy[c] = Hy[c] - s->vol[OR_DB_Y][c] * pHyz[c];
+                    Hx[c] = Hx[c] + s->vol[OR_DB_X][c] * pHxz[c];
+                }
+}
+
+/* ver0040/cpu_main.f90:104-148: per step source, E, CPML_E, H, CPML_H, output;
+ * signal[n] drives step n (the Fortran istep = n + 1); traces [6][nrec][it]
+ * hold E and H at the receivers after step n (0-based cells) */
+void orf_forward(or_state *s, int npts, const int *cells, const double *mu,
+                 const double *signal, int nsteps, int nrec, const int *rec, int it,
+                 double *traces)
+{
+    for (int n = 0; n < nsteps; ++n) {
+        if (signal)
+            orf_source_hz(s, npts, cells, mu, signal[n]);
+        orf_e_field_cpml4(s);
+        orf_cpml_e4(s);
+        orf_h_field_cpml4(s);
+        orf_cpml_h4(s);
+        for (int c = 0; c < 6 && traces; ++c)
+            for (int r = 0; r < nrec; ++r)
+                traces[((long)c * nrec + r) * it + n] = s->vol[OR_EX + c][rec[r]];
+    }
+}

@@ -110,6 +110,20 @@ void or_update_para2(int ix, int iy, int iz, int nc, int iseabed, const double *
                      const double *k2, double scaler, double minval, double maxval,
                      double *sig, double *grad, double *alpha);
 
+/* Fortran lineage (ver0040): profiles, coefficients with per-cell mu, the
+ * Hz source points, E/H updates and CPML in the Fortran ranges, the step loop.
+ * Source and receiver cells are 0-based flat indices. */
+void orf_init_cpml(or_state *s);
+void orf_media_coeff(or_state *s, const double *mu);
+void orf_source_hz(or_state *s, int npts, const int *cells, const double *mu, double sval);
+void orf_e_field_cpml4(or_state *s);
+void orf_cpml_e4(or_state *s);
+void orf_h_field_cpml4(or_state *s);
+void orf_cpml_h4(or_state *s);
+void orf_forward(or_state *s, int npts, const int *cells, const double *mu,
+                 const double *signal, int nsteps, int nrec, const int *rec, int it,
+                 double *traces);
+
 #ifdef __cplusplus
 }
 #endif

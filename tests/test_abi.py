@@ -57,6 +57,23 @@ def test_f_to_d_h_factor():
     np.testing.assert_allclose(y, x * np.sqrt(-2 * 2 * np.pi / (1j * om)), rtol=1e-15)
 
 
+def test_f_to_d_h_bin_zero_is_nan():
+    """sqrt(-2 w0/(i w)) diverges at w = 0 (the reference divides by zero at
+    k = 0, ver0040/cpu_f_to_d_h.f90:146-148): that bin is NaN, the others finite."""
+    om = 2 * np.pi * np.array([0.0, 0.5])
+    y = f_to_d_h(np.ones((2, 2), dtype=complex), om, 2 * np.pi)
+    assert np.all(np.isnan(y[:, 0])) and np.all(np.isfinite(y[:, 1]))
+
+
+def test_time_reversed_residual():
+    from mittet_main_amd.fwi3d import time_reversed_residual
+    obs = np.arange(6.0)
+    calc = np.array([0.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    np.testing.assert_array_equal(time_reversed_residual(obs, calc), [4.0, 3.0, 2.0, 1.0, 0.0, 0.0])
+    with pytest.raises(ValueError):
+        time_reversed_residual(obs, calc[:-1])
+
+
 def grid(**kw):
     g = dict(ix=40, iy=40, iz=40, ncpml=10, dx=50.0, dy=50.0, dz=50.0, dt=7.7e-4,
              omega0=2 * np.pi, cmax=31622.0)
