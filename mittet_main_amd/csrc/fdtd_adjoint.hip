@@ -102,23 +102,66 @@ __device__ __forceinline__ double rprof(const AdjArgs &A, int a, bool h, int idx
     return ((const GLOBAL double *)A.rp[a])[(h ? n : 0) + idx];
 }
 
-// 32 x 8 tiles of one plane per block, numbered so that each XCD owns a
-// contiguous run of tiles (the numbering of the fused forward kernels)
+__device__ __forceinline__ bool in_region(const AdjArgs &A, int i, int j, int k)
+{
+    return i >= A.lo[0] && i < A.hi[0] && j >= A.lo[1] && j < A.hi[1] && k >= A.lo[2] &&
+           k < A.hi[2];
+}
+
+// Cell of this lane in the per-cell passes: 32 x 8 tiles of one plane per
+// block, or the tiles {x0, y0, k, yend} of the cells outside the LDS region
+// (16 x 16 tiles below yend when yend != 0); blocks numbered so that each XCD
+// owns a contiguous run of tiles (the numbering of the fused forward kernels).
+// Cells of the LDS region belong to the LDS kernels.
 __device__ __forceinline__ bool cell_of(const AdjArgs &A, int &i, int &j, int &k, int &c)
 {
     const int b = blockIdx.x;
     const int L = (b & 7) * (A.fb_pad >> 3) + (b >> 3);
     if (L >= A.fb)
         return false;
-    const int tx = L % A.ftx, r = L / A.ftx;
-    const int ty = r % A.fty;
-    k = r / A.fty;
-    i = tx * ZX + (threadIdx.x & (ZX - 1));
-    j = ty * ZY + threadIdx.x / ZX;
-    if (i >= A.ix || j >= A.iy)
+    if (A.shell) {
+        const GLOBAL int *t = (const GLOBAL int *)(A.shell + L);
+        const int x0 = t[0], y0 = t[1], yend = t[3];
+        k = t[2];
+        if (yend) {
+            i = x0 + (threadIdx.x & 15);
+            j = y0 + (threadIdx.x >> 4);
+            if (j >= yend)
+                return false;
+        } else {
+            i = x0 + (threadIdx.x & (ZX - 1));
+            j = y0 + threadIdx.x / ZX;
+        }
+        if (k < 0)
+            return false;
+    } else {
+        const int tx = L % A.ftx, r = L / A.ftx;
+        const int ty = r % A.fty;
+        k = r / A.fty;
+        i = tx * ZX + (threadIdx.x & (ZX - 1));
+        j = ty * ZY + threadIdx.x / ZX;
+    }
+    if (i >= A.ix || j >= A.iy || in_region(A, i, j, k))
         return false;
     c = (k * A.iy + j) * A.ix + i;
     return true;
+}
+
+// Weights of the transposed stencils at a cell (flat index c), component d.
+// H phase: db lH with db = dt/mu0 in the LDS region, where the per-cell pass
+// leaves t3 unwritten; t3 elsewhere.  E phase: cb lE with cb = dt/eps' there.
+__device__ __forceinline__ double wgt_h(const AdjArgs &A, int d, int i, int j, int k, int c)
+{
+    const bool in = in_region(A, i, j, k);
+    const double v = ((const GLOBAL double *)(in ? A.lH : A.t3))[(size_t)d * A.n + c];
+    return in ? A.dt / A.mu0 * v : v;
+}
+
+__device__ __forceinline__ double wgt_e(const AdjArgs &A, int d, int i, int j, int k, int c)
+{
+    const bool in = in_region(A, i, j, k);
+    const double v = ((const GLOBAL double *)(in ? A.lE : A.t3))[(size_t)d * A.n + c];
+    return in ? ((const GLOBAL double *)A.cbi)[c] * v : v;
 }
 
 // Cell-local CPML-free interior [nc, n - nc): ca = da = 1, cb = dt/eps',
@@ -149,10 +192,6 @@ __device__ __forceinline__ double dminus_at(const GLOBAL double *w, int c, int s
     return c1 * w[c] - c1 * w[c - s] + c2 * w[c + s] - c2 * w[c - 2 * s];
 }
 
-__device__ __forceinline__ double dplus_at(const GLOBAL double *w, int c, int s, double c1, double c2)
-{
-    return c1 * w[c + s] - c1 * w[c] + c2 * w[c + 2 * s] - c2 * w[c - s];
-}
 
 }  // namespace
 
@@ -243,15 +282,17 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
         A.grad[c] += g;
 }
 
-// weighted value at a neighbour for a transposed stencil (0 outside the grid)
-__device__ __forceinline__ double wnb(const AdjArgs &A, const double *T, int comp, int ni, int nj,
-                                      int nk, int axis, bool h)
+// weighted value at a neighbour for a transposed stencil (0 outside the grid);
+// h: H-phase weights and 1/khd, else E-phase weights and 1/ked
+__device__ __forceinline__ double wnb(const AdjArgs &A, int comp, int ni, int nj, int nk, int axis,
+                                      bool h)
 {
     if (ni < 0 || nj < 0 || nk < 0 || ni >= A.ix || nj >= A.iy || nk >= A.iz)
         return 0.0;
     const int c = (nk * A.iy + nj) * A.ix + ni;
     const int id[3] = {ni, nj, nk};
-    return T[(size_t)comp * A.n + c] * rprof(A, axis, h, id[axis]);
+    const double w = h ? wgt_h(A, comp, ni, nj, nk, c) : wgt_e(A, comp, ni, nj, nk, c);
+    return w * rprof(A, axis, h, id[axis]);
 }
 
 __device__ __forceinline__ double wpsi(const AdjArgs &A, int a, int arr, int ni, int nj, int nk,
@@ -285,19 +326,26 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
     };
     if (A.kappa_one && __all(core(A, i, j, k))) {
         const int st[3] = {1, A.ix, A.ix * A.iy};
-        const GLOBAL double *T = (const GLOBAL double *)A.t3;
+        // D-_a of the weights of component d (stencil inside the grid)
+        auto dm = [&](int d, int a) {
+            int o[3] = {0, 0, 0};
+            auto at = [&](int off) {
+                o[a] = off;
+                return wgt_h(A, d, i + o[0], j + o[1], k + o[2], c + off * st[a]);
+            };
+            return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
+        };
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
-            const GLOBAL double *Td = T + (size_t)d * N;
-            acc[b] += dminus_at(Td, c, st[a], c1, c2) * A.rdl[a];
-            acc[a] -= dminus_at(Td, c, st[b], c1, c2) * A.rdl[b];
+            acc[b] += dm(d, a) * A.rdl[a];
+            acc[a] -= dm(d, b) * A.rdl[b];
         }
     } else {
         auto dminus = [&](int comp, int axis) {
             int o[3] = {0, 0, 0};
             auto at = [&](int off) {
                 o[axis] = off;
-                return wnb(A, A.t3, comp, i + o[0], j + o[1], k + o[2], axis, true);
+                return wnb(A, comp, i + o[0], j + o[1], k + o[2], axis, true);
             };
             return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
         };
@@ -422,19 +470,26 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
     };
     if (A.kappa_one && __all(core(A, i, j, k))) {
         const int st[3] = {1, A.ix, A.ix * A.iy};
-        const GLOBAL double *T = (const GLOBAL double *)A.t3;
+        // D+_a of the weights of component d (stencil inside the grid)
+        auto dp = [&](int d, int a) {
+            int o[3] = {0, 0, 0};
+            auto at = [&](int off) {
+                o[a] = off;
+                return wgt_e(A, d, i + o[0], j + o[1], k + o[2], c + off * st[a]);
+            };
+            return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
+        };
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
-            const GLOBAL double *Td = T + (size_t)d * N;
-            acc[b] -= dplus_at(Td, c, st[a], c1, c2) * A.rdl[a];
-            acc[a] += dplus_at(Td, c, st[b], c1, c2) * A.rdl[b];
+            acc[b] -= dp(d, a) * A.rdl[a];
+            acc[a] += dp(d, b) * A.rdl[b];
         }
     } else {
         auto dplus = [&](int comp, int axis) {
             int o[3] = {0, 0, 0};
             auto at = [&](int off) {
                 o[axis] = off;
-                return wnb(A, A.t3, comp, i + o[0], j + o[1], k + o[2], axis, false);
+                return wnb(A, comp, i + o[0], j + o[1], k + o[2], axis, false);
             };
             return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
         };
@@ -458,6 +513,405 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
         A.lH[(size_t)d * N + c] += acc[d];
 }
 
+// ------------------------------------------------------------------------
+// LDS z-marching kernels of the adjoint over its region, where ca = da = 1,
+// cb = dt/eps', db = dt/mu0, ked = khd = dl and no psi stencil reaches.  A
+// block owns a 64 x LY column and marches lkb planes along z, as the forward
+// LDS kernels.  The H-phase kernel adds the transposed curl+ of the weights
+// db lH to lE (the stencil pattern of the forward E kernel); the E-phase
+// kernel adds the transposed curl- of the weights cb lE to lH (the pattern
+// of the forward H kernel) and accumulates the E-phase gradient
+// lE . dcb/de . curl-(H^n) of the cell.  Weights of cells outside the region
+// are the per-cell passes' t3, which run first.  Term order is the per-cell
+// kernels'.
+
+namespace {
+
+constexpr int ANT = LX * LY;
+
+template <int XL, int XR, int YL, int YR> struct ATile {
+    static constexpr int W = LX + XL + XR, H = LY + YL + YR, N = W * H;
+    static constexpr int R = (N + ANT - 1) / ANT;   // elements per lane
+};
+
+// weights of component d on plane kk of a tile (halo XL/YL before the tile),
+// elements beyond (xmax, ymax) left 0
+template <class T, int XL, int YL, bool HPHASE>
+__device__ __forceinline__ void wload(const AdjArgs &A, int d, int kk, int x0, int y0, int xmax,
+                                      int ymax, double (&r)[T::R])
+{
+    const double dbi = A.dt / A.mu0;
+#pragma unroll
+    for (int q = 0; q < T::R; ++q) {
+        const int e = threadIdx.x + q * ANT;
+        double v = 0.0;
+        if (e < T::N) {
+            const int gy = y0 - YL + e / T::W, gx = x0 - XL + e % T::W;
+            if (gx <= xmax && gy <= ymax) {
+                // one load from the field or from t3, then the region's scaling
+                const int c = (kk * A.iy + gy) * A.ix + gx;
+                const bool in = in_region(A, gx, gy, kk);
+                const GLOBAL double *f = (const GLOBAL double *)(in ? (HPHASE ? A.lH : A.lE) : A.t3);
+                v = f[(size_t)d * A.n + c];
+                if (HPHASE)
+                    v = in ? dbi * v : v;
+                else
+                    v = in ? ((const GLOBAL double *)A.cbi)[c] * v : v;
+            }
+        }
+        r[q] = v;
+    }
+}
+
+// values of array f on plane kk of a tile, elements beyond (xmax, ymax) left 0
+template <class T, int XL, int YL>
+__device__ __forceinline__ void fload(const GLOBAL double *f, int kk, int ix, int pl, int x0,
+                                      int y0, int xmax, int ymax, double (&r)[T::R])
+{
+#pragma unroll
+    for (int q = 0; q < T::R; ++q) {
+        const int e = threadIdx.x + q * ANT;
+        double v = 0.0;
+        if (e < T::N) {
+            const int gy = y0 - YL + e / T::W, gx = x0 - XL + e % T::W;
+            if (gx <= xmax && gy <= ymax)
+                v = f[(size_t)kk * pl + (size_t)gy * ix + gx];
+        }
+        r[q] = v;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void wstore(double *lds, const double (&r)[T::R])
+{
+#pragma unroll
+    for (int q = 0; q < T::R; ++q) {
+        const int e = threadIdx.x + q * ANT;
+        if (e < T::N)
+            lds[e] = r[q];
+    }
+}
+
+// block -> (region tile, z-chunk), XCD-contiguous numbering
+__device__ __forceinline__ bool acol(const AdjArgs &A, int &x0, int &y0, int &k0, int &k1)
+{
+    const int b = blockIdx.x;
+    const int L = (b & 7) * (A.lb_pad >> 3) + (b >> 3);
+    if (L >= A.lb)
+        return false;
+    const int tx = L % A.ltx, r = L / A.ltx;
+    const int ty = r % A.lty, kz = r / A.lty;
+    x0 = A.lo[0] + tx * LX;
+    y0 = A.lo[1] + ty * LY;
+    k0 = A.lo[2] + kz * A.lkb;
+    k1 = min(k0 + A.lkb, A.hi[2]);
+    return true;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(ANT) void k_adj_hl(AdjArgs A)
+{
+    // weights of Hz: x -2..+1, y -2..+1; of Hy: x -2..+1; of Hx: y -2..+1
+    using TZ = ATile<2, 1, 2, 1>;
+    using TY = ATile<2, 1, 0, 0>;
+    using TX = ATile<0, 0, 2, 1>;
+    __shared__ double sz[3][TZ::N], sy[3][TY::N], sx[3][TX::N];
+    int x0, y0, k0, k1;
+    if (!acol(A, x0, y0, k0, k1))
+        return;
+    const int lx = threadIdx.x % LX, ly = threadIdx.x / LX;
+    const int i = x0 + lx, jj = y0 + ly;
+    const bool valid = i < A.hi[0] && jj < A.hi[1];
+    // idle lanes of a partial tile work on their row's last cell (in bounds)
+    const int ci = min(i, A.hi[0] - 1), cj = min(jj, A.hi[1] - 1);
+    const int ix = A.ix, pl = A.ix * A.iy, col = cj * ix + ci;
+    const size_t N = (size_t)A.n;
+    const int xmax = min(x0 + LX, A.hi[0]), ymax = min(y0 + LY, A.hi[1]);
+    const double c1 = A.c1, c2 = A.c2, rx = A.rdl[0], ry = A.rdl[1], rz = A.rdl[2];
+    GLOBAL double *lE = (GLOBAL double *)A.lE;
+    // own column's weights of Hy and Hx at planes k-2, k-1
+    double yq2 = wgt_h(A, 1, ci, cj, k0 - 2, (k0 - 2) * pl + col);
+    double yq1 = wgt_h(A, 1, ci, cj, k0 - 1, (k0 - 1) * pl + col);
+    double xq2 = wgt_h(A, 0, ci, cj, k0 - 2, (k0 - 2) * pl + col);
+    double xq1 = wgt_h(A, 0, ci, cj, k0 - 1, (k0 - 1) * pl + col);
+    double rgz[TZ::R], rgy[TY::R], rgx[TX::R];
+    int b0 = 0, b1 = 1, b2 = 2;   // LDS buffers of planes k, k+1 and the one being filled
+    wload<TZ, 2, 2, true>(A, 2, k0, x0, y0, xmax, ymax, rgz);
+    wload<TY, 2, 0, true>(A, 1, k0, x0, y0, xmax, ymax, rgy);
+    wload<TX, 0, 2, true>(A, 0, k0, x0, y0, xmax, ymax, rgx);
+    wstore<TZ>(sz[b0], rgz);
+    wstore<TY>(sy[b0], rgy);
+    wstore<TX>(sx[b0], rgx);
+    wload<TZ, 2, 2, true>(A, 2, k0 + 1, x0, y0, xmax, ymax, rgz);
+    wload<TY, 2, 0, true>(A, 1, k0 + 1, x0, y0, xmax, ymax, rgy);
+    wload<TX, 0, 2, true>(A, 0, k0 + 1, x0, y0, xmax, ymax, rgx);
+    size_t c = (size_t)k0 * pl + col;
+    double l0 = lE[c], l1 = lE[N + c], l2 = lE[2 * N + c];
+    for (int k = k0; k < k1; ++k) {
+        wstore<TZ>(sz[b1], rgz);
+        wstore<TY>(sy[b1], rgy);
+        wstore<TX>(sx[b1], rgx);
+        __syncthreads();
+        const bool more = k + 1 < k1;
+        const size_t cn = c + pl;
+        double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+        if (more) {
+            wload<TZ, 2, 2, true>(A, 2, k + 2, x0, y0, xmax, ymax, rgz);
+            wload<TY, 2, 0, true>(A, 1, k + 2, x0, y0, xmax, ymax, rgy);
+            wload<TX, 0, 2, true>(A, 0, k + 2, x0, y0, xmax, ymax, rgx);
+            n0 = lE[cn];
+            n1 = lE[N + cn];
+            n2 = lE[2 * N + cn];
+        }
+        const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
+        const int zc = (ly + 2) * TZ::W + lx + 2;
+        const double z0 = Z[zc], zym1 = Z[zc - TZ::W], zyp1 = Z[zc + TZ::W], zym2 = Z[zc - 2 * TZ::W];
+        const double zxm1 = Z[zc - 1], zxp1 = Z[zc + 1], zxm2 = Z[zc - 2];
+        const int yc = ly * TY::W + lx + 2;
+        const double y0v = Y[yc], yxm1 = Y[yc - 1], yxp1 = Y[yc + 1], yxm2 = Y[yc - 2];
+        const double yzp1 = sy[b1][yc];
+        const int xc = (ly + 2) * TX::W + lx;
+        const double x0v = X[xc], xym1 = X[xc - TX::W], xyp1 = X[xc + TX::W], xym2 = X[xc - 2 * TX::W];
+        const double xzp1 = sx[b1][xc];
+        // D-_a of the weights of component d
+        const double dxy = c1 * x0v - c1 * xym1 + c2 * xyp1 - c2 * xym2;
+        const double dxz = c1 * x0v - c1 * xq1 + c2 * xzp1 - c2 * xq2;
+        const double dyz = c1 * y0v - c1 * yq1 + c2 * yzp1 - c2 * yq2;
+        const double dyx = c1 * y0v - c1 * yxm1 + c2 * yxp1 - c2 * yxm2;
+        const double dzx = c1 * z0 - c1 * zxm1 + c2 * zxp1 - c2 * zxm2;
+        const double dzy = c1 * z0 - c1 * zym1 + c2 * zyp1 - c2 * zym2;
+        if (valid) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            a2 += dxy * ry;
+            a1 -= dxz * rz;
+            a0 += dyz * rz;
+            a2 -= dyx * rx;
+            a1 += dzx * rx;
+            a0 -= dzy * ry;
+            const double e0 = l0 + a0, e1 = l1 + a1, e2 = l2 + a2;
+            lE[c] = e0;
+            lE[N + c] = e1;
+            lE[2 * N + c] = e2;
+            if ((long long)c == A.src_cell) {
+                const double le = A.src_comp == 0 ? e0 : (A.src_comp == 1 ? e1 : e2);
+                A.grad_src[0] += le * A.amp_n / A.sig_src;
+            }
+        }
+        yq2 = yq1;
+        yq1 = y0v;
+        xq2 = xq1;
+        xq1 = x0v;
+        l0 = n0;
+        l1 = n1;
+        l2 = n2;
+        c = cn;
+        const int t = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = t;
+    }
+}
+
+__global__ __launch_bounds__(ANT) void k_adj_el(AdjArgs A)
+{
+    // weights of Ez: x -1..+2, y -1..+2; of Ey: x -1..+2; of Ex: y -1..+2
+    using TZ = ATile<1, 2, 1, 2>;
+    using TY = ATile<1, 2, 0, 0>;
+    using TX = ATile<0, 0, 1, 2>;
+    __shared__ double sz[4][TZ::N], sy[4][TY::N], sx[4][TX::N];
+    int x0, y0, k0, k1;
+    if (!acol(A, x0, y0, k0, k1))
+        return;
+    const int lx = threadIdx.x % LX, ly = threadIdx.x / LX;
+    const int i = x0 + lx, jj = y0 + ly;
+    const bool valid = i < A.hi[0] && jj < A.hi[1];
+    const int ci = min(i, A.hi[0] - 1), cj = min(jj, A.hi[1] - 1);
+    const int ix = A.ix, pl = A.ix * A.iy, col = cj * ix + ci;
+    const size_t N = (size_t)A.n;
+    const int xmax = min(x0 + LX, A.hi[0]) + 1, ymax = min(y0 + LY, A.hi[1]) + 1;
+    const double c1 = A.c1, c2 = A.c2, rx = A.rdl[0], ry = A.rdl[1], rz = A.rdl[2];
+    GLOBAL double *lH = (GLOBAL double *)A.lH;
+    // own column's weights of Ey and Ex at plane k-1
+    double yq1 = wgt_e(A, 1, ci, cj, k0 - 1, (k0 - 1) * pl + col);
+    double xq1 = wgt_e(A, 0, ci, cj, k0 - 1, (k0 - 1) * pl + col);
+    size_t c = (size_t)k0 * pl + col;
+    double rgz[TZ::R], rgy[TY::R], rgx[TX::R];
+    int b0 = 0, b1 = 1, b2 = 2, b3 = 3;   // planes k, k+1, k+2 (being filled), free
+    for (int p = 0; p < 2; ++p) {
+        const int bp = p == 0 ? b0 : b1;
+        wload<TZ, 1, 1, false>(A, 2, k0 + p, x0, y0, xmax, ymax, rgz);
+        wload<TY, 1, 0, false>(A, 1, k0 + p, x0, y0, xmax, ymax, rgy);
+        wload<TX, 0, 1, false>(A, 0, k0 + p, x0, y0, xmax, ymax, rgx);
+        wstore<TZ>(sz[bp], rgz);
+        wstore<TY>(sy[bp], rgy);
+        wstore<TX>(sx[bp], rgx);
+    }
+    wload<TZ, 1, 1, false>(A, 2, k0 + 2, x0, y0, xmax, ymax, rgz);
+    wload<TY, 1, 0, false>(A, 1, k0 + 2, x0, y0, xmax, ymax, rgy);
+    wload<TX, 0, 1, false>(A, 0, k0 + 2, x0, y0, xmax, ymax, rgx);
+    double h0 = lH[c], h1 = lH[N + c], h2 = lH[2 * N + c];
+    for (int k = k0; k < k1; ++k) {
+        wstore<TZ>(sz[b2], rgz);
+        wstore<TY>(sy[b2], rgy);
+        wstore<TX>(sx[b2], rgx);
+        __syncthreads();
+        const bool more = k + 1 < k1;
+        const size_t cn = c + pl;
+        double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+        if (more) {
+            wload<TZ, 1, 1, false>(A, 2, k + 3, x0, y0, xmax, ymax, rgz);
+            wload<TY, 1, 0, false>(A, 1, k + 3, x0, y0, xmax, ymax, rgy);
+            wload<TX, 0, 1, false>(A, 0, k + 3, x0, y0, xmax, ymax, rgx);
+            n0 = lH[cn];
+            n1 = lH[N + cn];
+            n2 = lH[2 * N + cn];
+        }
+        const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
+        const int zc = (ly + 1) * TZ::W + lx + 1;
+        const double z0 = Z[zc], zyp1 = Z[zc + TZ::W], zyp2 = Z[zc + 2 * TZ::W], zym1 = Z[zc - TZ::W];
+        const double zxp1 = Z[zc + 1], zxp2 = Z[zc + 2], zxm1 = Z[zc - 1];
+        const int yc = ly * TY::W + lx + 1;
+        const double y0v = Y[yc], yxp1 = Y[yc + 1], yxp2 = Y[yc + 2], yxm1 = Y[yc - 1];
+        const double yzp1 = sy[b1][yc], yzp2 = sy[b2][yc];
+        const int xc = (ly + 1) * TX::W + lx;
+        const double x0v = X[xc], xyp1 = X[xc + TX::W], xyp2 = X[xc + 2 * TX::W], xym1 = X[xc - TX::W];
+        const double xzp1 = sx[b1][xc], xzp2 = sx[b2][xc];
+        // D+_a of the weights of component d
+        const double pxy = c1 * xyp1 - c1 * x0v + c2 * xyp2 - c2 * xym1;
+        const double pxz = c1 * xzp1 - c1 * x0v + c2 * xzp2 - c2 * xq1;
+        const double pyz = c1 * yzp1 - c1 * y0v + c2 * yzp2 - c2 * yq1;
+        const double pyx = c1 * yxp1 - c1 * y0v + c2 * yxp2 - c2 * yxm1;
+        const double pzx = c1 * zxp1 - c1 * z0 + c2 * zxp2 - c2 * zxm1;
+        const double pzy = c1 * zyp1 - c1 * z0 + c2 * zyp2 - c2 * zym1;
+        if (valid) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            a2 -= pxy * ry;
+            a1 += pxz * rz;
+            a0 -= pyz * rz;
+            a2 += pyx * rx;
+            a1 -= pzx * rx;
+            a0 += pzy * ry;
+            lH[c] = h0 + a0;
+            lH[N + c] = h1 + a1;
+            lH[2 * N + c] = h2 + a2;
+        }
+        yq1 = y0v;
+        xq1 = x0v;
+        h0 = n0;
+        h1 = n1;
+        h2 = n2;
+        c = cn;
+        const int t = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = t;
+    }
+}
+
+// E-phase gradient over the region: grad += sum_d lE_d dcb/de curl-_d(H^n)
+// with dcb/de = -dt/eps'^2, the forward E kernel's stencil pattern on H^n
+__global__ __launch_bounds__(ANT) void k_adj_gl(AdjArgs A)
+{
+    // H^n_z: x -2..+1, y -2..+1; H^n_y: x -2..+1; H^n_x: y -2..+1
+    using TZ = ATile<2, 1, 2, 1>;
+    using TY = ATile<2, 1, 0, 0>;
+    using TX = ATile<0, 0, 2, 1>;
+    __shared__ double sz[3][TZ::N], sy[3][TY::N], sx[3][TX::N];
+    int x0, y0, k0, k1;
+    if (!acol(A, x0, y0, k0, k1))
+        return;
+    const int lx = threadIdx.x % LX, ly = threadIdx.x / LX;
+    const int i = x0 + lx, jj = y0 + ly;
+    const bool valid = i < A.hi[0] && jj < A.hi[1];
+    const int ci = min(i, A.hi[0] - 1), cj = min(jj, A.hi[1] - 1);
+    const int ix = A.ix, pl = A.ix * A.iy, col = cj * ix + ci;
+    const size_t N = (size_t)A.n;
+    const int xmax = min(x0 + LX, A.hi[0]), ymax = min(y0 + LY, A.hi[1]);
+    const double c1 = A.c1, c2 = A.c2, rx = A.rdl[0], ry = A.rdl[1], rz = A.rdl[2];
+    const double dt = A.dt;
+    const GLOBAL double *Hx = (const GLOBAL double *)A.uH0;
+    const GLOBAL double *Hy = Hx + N, *Hz = Hx + 2 * N;
+    const GLOBAL double *lE = (const GLOBAL double *)A.lE;
+    const GLOBAL double *eps = (const GLOBAL double *)A.eps;
+    GLOBAL double *grad = (GLOBAL double *)A.grad;
+    size_t c = (size_t)k0 * pl + col;
+    double yq2 = Hy[c - 2 * pl], yq1 = Hy[c - pl], xq2 = Hx[c - 2 * pl], xq1 = Hx[c - pl];
+    double rgz[TZ::R], rgy[TY::R], rgx[TX::R];
+    int b0 = 0, b1 = 1, b2 = 2;
+    fload<TZ, 2, 2>(Hz, k0, ix, pl, x0, y0, xmax, ymax, rgz);
+    fload<TY, 2, 0>(Hy, k0, ix, pl, x0, y0, xmax, ymax, rgy);
+    fload<TX, 0, 2>(Hx, k0, ix, pl, x0, y0, xmax, ymax, rgx);
+    wstore<TZ>(sz[b0], rgz);
+    wstore<TY>(sy[b0], rgy);
+    wstore<TX>(sx[b0], rgx);
+    fload<TZ, 2, 2>(Hz, k0 + 1, ix, pl, x0, y0, xmax, ymax, rgz);
+    fload<TY, 2, 0>(Hy, k0 + 1, ix, pl, x0, y0, xmax, ymax, rgy);
+    fload<TX, 0, 2>(Hx, k0 + 1, ix, pl, x0, y0, xmax, ymax, rgx);
+    double e0 = lE[c], e1 = lE[N + c], e2 = lE[2 * N + c], ep = eps[c], g0 = grad[c];
+    for (int k = k0; k < k1; ++k) {
+        wstore<TZ>(sz[b1], rgz);
+        wstore<TY>(sy[b1], rgy);
+        wstore<TX>(sx[b1], rgx);
+        __syncthreads();
+        const bool more = k + 1 < k1;
+        const size_t cn = c + pl;
+        double n0 = 0.0, n1 = 0.0, n2 = 0.0, nep = 1.0, ng = 0.0;
+        if (more) {
+            fload<TZ, 2, 2>(Hz, k + 2, ix, pl, x0, y0, xmax, ymax, rgz);
+            fload<TY, 2, 0>(Hy, k + 2, ix, pl, x0, y0, xmax, ymax, rgy);
+            fload<TX, 0, 2>(Hx, k + 2, ix, pl, x0, y0, xmax, ymax, rgx);
+            n0 = lE[cn];
+            n1 = lE[N + cn];
+            n2 = lE[2 * N + cn];
+            nep = eps[cn];
+            ng = grad[cn];
+        }
+        const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
+        const int zc = (ly + 2) * TZ::W + lx + 2;
+        const double z0 = Z[zc], zym1 = Z[zc - TZ::W], zyp1 = Z[zc + TZ::W], zym2 = Z[zc - 2 * TZ::W];
+        const double zxm1 = Z[zc - 1], zxp1 = Z[zc + 1], zxm2 = Z[zc - 2];
+        const int yc = ly * TY::W + lx + 2;
+        const double y0v = Y[yc], yxm1 = Y[yc - 1], yxp1 = Y[yc + 1], yxm2 = Y[yc - 2];
+        const double yzp1 = sy[b1][yc];
+        const int xc = (ly + 2) * TX::W + lx;
+        const double x0v = X[xc], xym1 = X[xc - TX::W], xyp1 = X[xc + TX::W], xym2 = X[xc - 2 * TX::W];
+        const double xzp1 = sx[b1][xc];
+        const double hzy = c1 * z0 - c1 * zym1 + c2 * zyp1 - c2 * zym2;
+        const double hyz = c1 * y0v - c1 * yq1 + c2 * yzp1 - c2 * yq2;
+        const double hxz = c1 * x0v - c1 * xq1 + c2 * xzp1 - c2 * xq2;
+        const double hzx = c1 * z0 - c1 * zxm1 + c2 * zxp1 - c2 * zxm2;
+        const double hyx = c1 * y0v - c1 * yxm1 + c2 * yxp1 - c2 * yxm2;
+        const double hxy = c1 * x0v - c1 * xym1 + c2 * xyp1 - c2 * xym2;
+        if (valid) {
+            const double r = 1.0 / ep, cbd = -dt * r * r;
+            const double cu0 = hzy * ry - hyz * rz;
+            const double cu1 = hxz * rz - hzx * rx;
+            const double cu2 = hyx * rx - hxy * ry;
+            double g = 0.0;
+            g += e0 * cbd * cu0;
+            g += e1 * cbd * cu1;
+            g += e2 * cbd * cu2;
+            grad[c] = g0 + g;
+        }
+        yq2 = yq1;
+        yq1 = y0v;
+        xq2 = xq1;
+        xq1 = x0v;
+        e0 = n0;
+        e1 = n1;
+        e2 = n2;
+        ep = nep;
+        g0 = ng;
+        c = cn;
+        const int t = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = t;
+    }
+}
+
 hipError_t launch_adjoint_step(const AdjArgs &A, const double *g, int nrec, const int *rec,
                                hipStream_t st)
 {
@@ -467,8 +921,14 @@ hipError_t launch_adjoint_step(const AdjArgs &A, const double *g, int nrec, cons
     }
     hipLaunchKernelGGL(k_adj_h1, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
     hipLaunchKernelGGL(k_adj_h2, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
+    if (A.lb_pad > 0) {
+        hipLaunchKernelGGL(k_adj_hl, dim3(A.lb_pad), dim3(ANT), 0, st, A);
+        hipLaunchKernelGGL(k_adj_gl, dim3(A.lb_pad), dim3(ANT), 0, st, A);
+    }
     hipLaunchKernelGGL(k_adj_e1, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
     hipLaunchKernelGGL(k_adj_e2, dim3(A.fb_pad), dim3(ZX * ZY), 0, st, A);
+    if (A.lb_pad > 0)
+        hipLaunchKernelGGL(k_adj_el, dim3(A.lb_pad), dim3(ANT), 0, st, A);
     return hipGetLastError();
 }
 

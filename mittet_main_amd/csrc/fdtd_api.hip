@@ -179,6 +179,39 @@ void pack_device_profile(const std::vector<double> &hp, int n, std::vector<doubl
 
 int tiles(int n, int t) { return (n + t - 1) / t; }
 
+// Tiles {x0, y0, k, yend} of the per-cell kernels around an LDS region [lo, hi):
+// planes outside the region and rows of 32 x 8 tiles not inside it in y get
+// every tile; rows inside it in y get 16 x 16 tiles over the two x-edge
+// segments [0, lo_x) and [hi_x, ix) only.  Cells of the region that a tile
+// still holds are skipped by the kernels.  Padded to a multiple of 8 entries
+// with k = -1.
+std::vector<int4> complement_tiles(int ix, int iy, int iz, const int lo[3], const int hi[3])
+{
+    std::vector<int4> shell;
+    const int ftx = tiles(ix, mt::ZX), fty = tiles(iy, mt::ZY);
+    const int ya = tiles(lo[1], mt::ZY), yb = hi[1] / mt::ZY;   // tile rows inside [lo_y, hi_y)
+    for (int k = 0; k < iz; ++k) {
+        const bool inz = k >= lo[2] && k < hi[2];
+        for (int ty = 0; ty < fty; ++ty) {
+            if (inz && ty >= ya && ty < yb)
+                continue;
+            for (int tx = 0; tx < ftx; ++tx)
+                shell.push_back(int4{tx * mt::ZX, ty * mt::ZY, k, 0});
+        }
+        if (!inz || ya >= yb)
+            continue;
+        const int yend = yb * mt::ZY;
+        for (int y0 = ya * mt::ZY; y0 < yend; y0 += 16) {
+            for (int x0 = 0; x0 < lo[0]; x0 += 16)
+                shell.push_back(int4{x0, y0, k, yend});
+            for (int x0 = hi[0]; x0 < ix; x0 += 16)
+                shell.push_back(int4{x0, y0, k, yend});
+        }
+    }
+    shell.resize((size_t)tiles((int)shell.size(), 8) * 8, int4{0, 0, -1, 0});
+    return shell;
+}
+
 constexpr int kDefaultChunk = 32;
 constexpr int kDefaultZChunk = 32;
 
@@ -191,6 +224,23 @@ long long env_pad(const char *name, long long dflt)
     const char *v = std::getenv(name);
     return v ? std::atoll(v) : dflt;
 }
+// z-chunk length of the LDS kernels over a region of ltx x lty tiles and nz
+// planes: LKB planes, halved (down to 4) while a launch would hold fewer
+// than 512 blocks (two per CU).  At 152^3 (128^3 + 12-cell CPML) 8-plane
+// chunks (512 blocks) run the forward 3 % and the adjoint's LDS kernels 30 %
+// faster than 16 (256 blocks); 4 planes (1024 blocks) lose to the z-halo
+// reloads.  MITTET_FDTD_LKB overrides.
+int lds_chunk(int ltx, int lty, int nz)
+{
+    const long long e = env_pad("MITTET_FDTD_LKB", 0);
+    if (e > 0)
+        return (int)std::min<long long>(e, nz);
+    int kb = mt::LKB;
+    while (kb > 4 && (long long)ltx * lty * tiles(nz, kb) < 512)
+        kb /= 2;
+    return kb;
+}
+
 constexpr long long kSlotPad = 0;
 constexpr long long kAccPad = 0;
 
@@ -761,35 +811,13 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
         }
         d.ltx = tiles(hi[0] - lo[0], mt::LX);
         d.lty = tiles(hi[1] - lo[1], mt::LY);
-        d.lkz = tiles(hi[2] - lo[2], mt::LKB);
+        d.lkb = lds_chunk(d.ltx, d.lty, hi[2] - lo[2]);
+        d.lkz = tiles(hi[2] - lo[2], d.lkb);
         d.lb = d.ltx * d.lty * d.lkz;
         d.lb_pad = tiles(d.lb, 8) * 8;
-        // planes outside the region and rows of 32 x 8 tiles not inside it in y:
-        // every tile; rows inside it in y: 16 x 16 tiles over the two x-edge
-        // segments [0, lo_x) and [hi_x, ix) only
-        std::vector<int4> shell;
-        const int ya = tiles(lo[1], mt::ZY), yb = hi[1] / mt::ZY;   // tile rows inside [lo_y, hi_y)
-        for (int k = 0; k < p.iz; ++k) {
-            const bool inz = k >= lo[2] && k < hi[2];
-            for (int ty = 0; ty < d.fty; ++ty) {
-                if (inz && ty >= ya && ty < yb)
-                    continue;
-                for (int tx = 0; tx < d.ftx; ++tx)
-                    shell.push_back(int4{tx * mt::ZX, ty * mt::ZY, k, 0});
-            }
-            if (!inz || ya >= yb)
-                continue;
-            const int yend = yb * mt::ZY;
-            for (int y0 = ya * mt::ZY; y0 < yend; y0 += 16) {
-                for (int x0 = 0; x0 < lo[0]; x0 += 16)
-                    shell.push_back(int4{x0, y0, k, yend});
-                for (int x0 = hi[0]; x0 < p.ix; x0 += 16)
-                    shell.push_back(int4{x0, y0, k, yend});
-            }
-        }
+        std::vector<int4> shell = complement_tiles(p.ix, p.iy, p.iz, lo, hi);
         d.fb = (int)shell.size();
-        d.fb_pad = tiles(d.fb, 8) * 8;
-        shell.resize(d.fb_pad, int4{0, 0, -1, 0});
+        d.fb_pad = d.fb;
         hipError_t e = dalloc(&s->d_shell, shell.size());
         if (e == hipSuccess)
             e = hcopy(s->d_shell, shell.data(), sizeof(int4) * shell.size());
@@ -1498,6 +1526,9 @@ struct fwi3d_grad {
     double *g = nullptr;              // [it][3][nrec] trace sensitivities
     double *rp = nullptr;             // per axis: 1/ked, 1/khd
     int kappa_one = 0;                // ked = khd = dl on every axis
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};   // LDS region of the adjoint (empty: lo = hi)
+    int4 *tiles = nullptr;            // per-cell tiles outside it (ntiles, multiple of 8)
+    int ntiles = 0;
     hipGraphExec_t replay[2] = {nullptr, nullptr};   // segment replays: full length, last
     int replay_len[2] = {0, 0};
 };
@@ -1604,14 +1635,58 @@ static size_t state_doubles(const fwi3d_shot *s)
     return o;
 }
 
+// ked = khd = dl on every axis (kappa = 1): the transposed stencils then
+// multiply by 1/dl
+static bool kappa_is_one(const fwi3d_shot *s)
+{
+    bool one = true;
+    for (int a = 0; a < 3; ++a) {
+        const int n = a == 0 ? s->desc.ix : (a == 1 ? s->desc.iy : s->desc.iz);
+        for (int row : {FWI3D_P_KED, FWI3D_P_KHD})
+            for (int q = 0; q < n; ++q)
+                one &= s->m->prof[a][(size_t)row * n + q] == s->desc.dl[a];
+    }
+    return one;
+}
+
+// LDS region of the adjoint: [ncpml + 1, n - ncpml - 1) on each axis (x start
+// rounded up to 8 cells), where the update is CPML-free and no transposed psi
+// stencil (offsets -2..+2 from a CPML face cell) reaches; empty without
+// kappa = 1 or when the LDS kernels are switched off
+static bool adj_region(const fwi3d_shot *s, int lo[3], int hi[3])
+{
+    const ShotDev &d = s->desc;
+    const int n[3] = {d.ix, d.iy, d.iz};
+    bool ok = d.nc >= 2 && kappa_is_one(s) && !env_pad("MITTET_FDTD_NOLDS", 0);
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = a == 0 ? (d.nc + 1 + 7) / 8 * 8 : d.nc + 1;
+        hi[a] = n[a] - d.nc - 1;
+        ok = ok && lo[a] < hi[a];
+    }
+    if (!ok)
+        for (int a = 0; a < 3; ++a)
+            lo[a] = hi[a] = 0;
+    return ok;
+}
+
+// per-cell tiles of the adjoint: around its LDS region, or none (every tile)
+static std::vector<int4> adj_tiles(const fwi3d_shot *s)
+{
+    int lo[3], hi[3];
+    if (!adj_region(s, lo, hi))
+        return {};
+    return complement_tiles(s->desc.ix, s->desc.iy, s->desc.iz, lo, hi);
+}
+
 // device bytes a gradient with this plan allocates (states, scratch, gradient,
-// trace sensitivities, profile reciprocals)
+// trace sensitivities, profile reciprocals, per-cell tile list)
 static long long grad_plan_bytes(const fwi3d_shot *s, const GradPlan &P)
 {
     const size_t N = (size_t)s->desc.n, st = state_doubles(s);
     const size_t tmp = st - 3 * N;
     const size_t other = tmp + N + 1 + std::max<size_t>(1, 3 * (size_t)s->nrec * s->it) +
-                         2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz);
+                         2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz) +
+                         2 * adj_tiles(s).size();
     return 8LL * (long long)(P.states * st + other);
 }
 
@@ -1661,17 +1736,22 @@ static int grad_new(fwi3d_shot *s, int levels, int segment, fwi3d_grad **out)
     // reciprocal ked / khd rows: the transposed stencils multiply where the
     // forward divides
     std::vector<double> rp;
-    g->kappa_one = 1;
     for (int a = 0; a < 3; ++a) {
         const int n = a == 0 ? s->desc.ix : (a == 1 ? s->desc.iy : s->desc.iz);
         for (int row : {FWI3D_P_KED, FWI3D_P_KHD})
-            for (int q = 0; q < n; ++q) {
-                const double v = s->m->prof[a][(size_t)row * n + q];
-                rp.push_back(1.0 / v);
-                g->kappa_one &= v == s->desc.dl[a];
-            }
+            for (int q = 0; q < n; ++q)
+                rp.push_back(1.0 / s->m->prof[a][(size_t)row * n + q]);
     }
-    if (hcopy(g->rp, rp.data(), sizeof(double) * rp.size()) != hipSuccess) {
+    g->kappa_one = kappa_is_one(s);
+    adj_region(s, g->lo, g->hi);
+    const std::vector<int4> tl = adj_tiles(s);
+    g->ntiles = (int)tl.size();
+    if (g->ntiles && dalloc(&g->tiles, tl.size()) != hipSuccess) {
+        fwi3d_grad_destroy(g);
+        return fail(FWI3D_ENOMEM, "adjoint tile list");
+    }
+    if (hcopy(g->rp, rp.data(), sizeof(double) * rp.size()) != hipSuccess ||
+        (g->ntiles && hcopy(g->tiles, tl.data(), sizeof(int4) * tl.size()) != hipSuccess)) {
         fwi3d_grad_destroy(g);
         return fail(FWI3D_EDEVICE, "profile upload failed");
     }
@@ -1724,6 +1804,7 @@ int fwi3d_grad_destroy(fwi3d_grad *g)
     dfree(g->grad_src);
     dfree(g->g);
     dfree(g->rp);
+    dfree(g->tiles);
     for (auto &r : g->replay)
         if (r)
             (void)hipGraphExecDestroy(r);
@@ -1842,6 +1923,22 @@ int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
     A.fty = tiles(d.iy, mt::ZY);
     A.fb = A.ftx * A.fty * d.iz;
     A.fb_pad = tiles(A.fb, 8) * 8;
+    A.shell = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        A.lo[a] = g->lo[a];
+        A.hi[a] = g->hi[a];
+    }
+    A.ltx = A.lty = A.lkz = A.lb = A.lb_pad = A.lkb = 0;
+    if (g->ntiles) {
+        A.shell = g->tiles;
+        A.fb = A.fb_pad = g->ntiles;
+        A.ltx = tiles(A.hi[0] - A.lo[0], mt::LX);
+        A.lty = tiles(A.hi[1] - A.lo[1], mt::LY);
+        A.lkb = lds_chunk(A.ltx, A.lty, A.hi[2] - A.lo[2]);
+        A.lkz = tiles(A.hi[2] - A.lo[2], A.lkb);
+        A.lb = A.ltx * A.lty * A.lkz;
+        A.lb_pad = tiles(A.lb, 8) * 8;
+    }
     // [nc, n - nc) is free of CPML terms when nc >= 2 (the update ranges start
     // at index 1 or 2); with a thinner CPML no cell takes the fast paths
     for (int a = 0; a < 3; ++a) {
@@ -1861,6 +1958,7 @@ int fwi3d_grad_backward(fwi3d_grad *g, const double *djdt, double *djds)
         A.lpsi[a] = g->adj + g->off_psi[a];
     }
     A.eps = d.eps;
+    A.cbi = d.cbi;
     A.lE = g->adj;
     A.lH = g->adj + g->off_h;
     A.t3 = g->tmp;

@@ -21,9 +21,10 @@ constexpr int RING_MAX = 256;
 constexpr int ZX = 32;   // z-marching tile: 32 x 8 columns, one thread per column
 constexpr int ZY = 8;
 // LDS z-marching kernels of the CPML-free interior: 64 x MT_LY tiles marching
-// MT_LKB planes.  At 288^3, 64 x 8 tiles (half the y-halo re-reads of 64 x 4)
-// run the half-steps 6 % faster than 64 x 4; 64 x 16 gains nothing; the chunk
-// length moves the time by about 1 %.
+// z-chunks of at most MT_LKB planes (shorter on small grids, so that a launch
+// still fills the CUs).  At 288^3, 64 x 8 tiles (half the y-halo re-reads of
+// 64 x 4) run the half-steps 6 % faster than 64 x 4; 64 x 16 gains nothing;
+// the chunk length moves the time there by about 1 %.
 #ifndef MT_LKB
 #define MT_LKB 16
 #endif
@@ -56,6 +57,7 @@ struct ShotDev {
                               // padding; null: every 32 x 8 tile
     int lo[3], hi[3];         // LDS-kernel region [lo, hi) per axis (inside the CPML-free interior)
     int ltx, lty, lkz;        // LDS-kernel tiles along x, y and z-chunks
+    int lkb;                  // LDS-kernel z-chunk length (planes)
     int lb, lb_pad;           // LDS-kernel blocks, padded to a multiple of 8
     double *e;                // E ring: slot q, comp d at e + q*sstride + d*n
     long long sstride;        // doubles between ring slots (3n + padding)
@@ -86,6 +88,14 @@ enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
 struct AdjArgs {
     int ix, iy, iz, nc, n;
     int ftx, fty, fb, fb_pad;        // 32 x 8 tiles per plane, as the fused forward kernels
+    const int4 *shell;               // non-null: fb tiles {x0, y0, k, yend} of the cells
+                                     // outside the LDS region (ShotDev::shell's format)
+    // LDS region of the adjoint, [lo, hi) on each axis (empty: lo = hi = 0):
+    // cells one cell clear of every CPML face, so that neither CPML
+    // coefficients nor transposed psi stencils reach them
+    int lo[3], hi[3];
+    int ltx, lty, lkz, lb, lb_pad;   // 64 x LY tiles, z-chunks of lkb planes
+    int lkb;
     int inner_hi[3];                 // cells in [nc, inner_hi[a]) on every axis see no CPML term
     int kappa_one;                   // ked = khd = dl everywhere
     double dt, mu0, c1, c2;
@@ -93,6 +103,7 @@ struct AdjArgs {
     const double *prof[3];
     const double *rp[3];             // per axis: 1/ked (row 0) and 1/khd (row 1)
     const double *eps;
+    const double *cbi;               // dt/eps' per cell
     long long slab[3];
     double *lE, *lH, *lpsi[3];       // adjoint state (forward layout)
     double *t3, *tpsi[3];            // per-step weights for the transposed stencils

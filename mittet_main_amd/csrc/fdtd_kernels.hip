@@ -1123,8 +1123,8 @@ hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks
 
 
 // ------------------------------------------------------------------------
-// LDS z-marching kernels of the CPML-free interior.  A block owns a 64 x 4
-// column of the shot's LDS region and marches LKB planes along z: the x and y
+// LDS z-marching kernels of the CPML-free interior.  A block owns a 64 x LY
+// column of the shot's LDS region and marches a z-chunk: the x and y
 // stencil neighbours of a plane come from LDS tiles that the block fills
 // once per plane, the z neighbours from a register queue along each lane's
 // own column, so every plane is read from HBM once per column (plus halo)
@@ -1182,8 +1182,8 @@ __device__ __forceinline__ bool lcol(const ShotDev &S, int b, int &x0, int &y0, 
     const int ty = r % S.lty, kz = r / S.lty;
     x0 = S.lo[0] + tx * LX;
     y0 = S.lo[1] + ty * LY;
-    k0 = S.lo[2] + kz * LKB;
-    k1 = min(k0 + LKB, S.hi[2]);
+    k0 = S.lo[2] + kz * S.lkb;
+    k1 = min(k0 + S.lkb, S.hi[2]);
     return true;
 }
 

@@ -33,6 +33,7 @@ replayed in real time); throughput = shots / makespan.
 from __future__ import annotations
 
 import math
+import os
 import threading
 import time
 from dataclasses import dataclass, field
@@ -181,9 +182,32 @@ def gradient_bytes(req: Request, budget: float):
     return None
 
 
+def _adjoint_tile_doubles(grid: Grid) -> int:
+    """Doubles held by the adjoint's per-cell tile list: 16-byte entries for
+    the 32 x 8 tiles (16 x 16 over the x edges) around its LDS region
+    [ncpml + 1, n - ncpml - 1) (x start rounded up to 8); none without that
+    region (kappa != 1, ncpml < 2, LDS kernels switched off)."""
+    nc = grid.ncpml
+    n = (grid.ix, grid.iy, grid.iz)
+    if nc < 2 or grid.kappa_max != 1.0 or int(os.environ.get("MITTET_FDTD_NOLDS", "0") or 0):
+        return 0
+    lo = [(nc + 1 + 7) // 8 * 8, nc + 1, nc + 1]
+    hi = [v - nc - 1 for v in n]
+    if any(a >= b for a, b in zip(lo, hi)):
+        return 0
+    ftx, fty = -(-grid.ix // 32), -(-grid.iy // 8)
+    ya, yb = -(-lo[1] // 8), hi[1] // 8
+    inside = max(0, yb - ya)
+    edge = -(-lo[0] // 16) + -(-(grid.ix - hi[0]) // 16)
+    per_in = (fty - inside) * ftx + (-(-inside * 8 // 16) * edge if inside else 0)
+    count = (hi[2] - lo[2]) * per_in + (grid.iz - (hi[2] - lo[2])) * fty * ftx
+    return 2 * (-(-count // 8) * 8)
+
+
 def plan_bytes(grid: Grid, nt: int, nrec: int, levels: int) -> int:
     """Device bytes of a gradient's checkpoint storage (host-side mirror of
-    fwi3d_grad_plan_bytes: states of 6 fields + shell psi, scratch, gradient)."""
+    fwi3d_grad_plan_bytes: states of 6 fields + shell psi, scratch, gradient,
+    the adjoint's tile list)."""
     import ctypes as C
     sl = (C.c_int * levels)()
     ns, st = C.c_int(0), C.c_longlong(0)
@@ -191,7 +215,8 @@ def plan_bytes(grid: Grid, nt: int, nrec: int, levels: int) -> int:
     n, nc = grid.ncells, grid.ncpml
     psi = 4 * 2 * nc * (grid.iy * grid.iz + grid.ix * grid.iz + grid.ix * grid.iy)
     state = 6 * n + psi
-    other = (state - 3 * n) + n + 1 + max(1, 3 * nrec * (nt + 1)) + 2 * (grid.ix + grid.iy + grid.iz)
+    other = ((state - 3 * n) + n + 1 + max(1, 3 * nrec * (nt + 1)) +
+             2 * (grid.ix + grid.iy + grid.iz) + _adjoint_tile_doubles(grid))
     return 8 * (st.value * state + other)
 
 
