@@ -225,20 +225,22 @@ long long env_pad(const char *name, long long dflt)
     return v ? std::atoll(v) : dflt;
 }
 // z-chunk length of the LDS kernels over a region of ltx x lty tiles and nz
-// planes: LKB planes, halved (down to 4) while a launch would hold fewer
-// than 512 blocks (two per CU).  At 152^3 (128^3 + 12-cell CPML) 8-plane
-// chunks (512 blocks) run the forward 3 % and the adjoint's LDS kernels 30 %
-// faster than 16 (256 blocks); 4 planes (1024 blocks) lose to the z-halo
-// reloads.  MITTET_FDTD_LKB overrides.
+// planes: 16 planes while that gives at least 2048 blocks (eight per CU),
+// else 8 while that gives at least 512 (two per CU), else 4.  Measured on
+// single shots with a 12-cell CPML (Gcell-updates/s for 16 / 8 / 4 planes):
+// 120^3 13.6 / 14.5 / 15.0, 152^3 20.6 / 21.2 / 20.8, 216^3 19.9 / 20.3 /
+// 20.2, 280^3 22.1 / 22.0 / 22.0.  MITTET_FDTD_LKB overrides.
 int lds_chunk(int ltx, int lty, int nz)
 {
     const long long e = env_pad("MITTET_FDTD_LKB", 0);
     if (e > 0)
         return (int)std::min<long long>(e, nz);
-    int kb = mt::LKB;
-    while (kb > 4 && (long long)ltx * lty * tiles(nz, kb) < 512)
-        kb /= 2;
-    return kb;
+    const long long cols = (long long)ltx * lty;
+    if (mt::LKB >= 16 && cols * tiles(nz, 16) >= 2048)
+        return 16;
+    if (cols * tiles(nz, 8) >= 512)
+        return 8;
+    return 4;
 }
 
 constexpr long long kSlotPad = 0;

@@ -1735,35 +1735,59 @@ hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t
     return hipGetLastError();
 }
 
-__global__ void k_state_copy(StateSegs segs, double *__restrict__ stored, size_t total, int save)
+// Copies the shot's live state to a stored state (save) or back.  Segment
+// blockIdx.y, stored right after the segments before it; 16-byte accesses
+// when both sides of the segment start 16-byte aligned.
+__global__ void k_state_copy(StateSegs segs, double *__restrict__ stored, int save)
 {
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total;
-         q += (size_t)gridDim.x * blockDim.x) {
-        size_t off = q;
-        int k = 0;
-        while (k + 1 < segs.nseg && off >= segs.count[k]) {
-            off -= segs.count[k];
-            ++k;
+    const int k = blockIdx.y;
+    size_t base = 0;
+    for (int j = 0; j < k; ++j)
+        base += segs.count[j];
+    double *__restrict__ live = segs.live[k];
+    double *__restrict__ st = stored + base;
+    const size_t n = segs.count[k];
+    const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    if ((((uintptr_t)live | (uintptr_t)st) & 15) == 0) {
+        double2 *__restrict__ a = reinterpret_cast<double2 *>(live);
+        double2 *__restrict__ b = reinterpret_cast<double2 *>(st);
+        for (size_t q = t0; q < n / 2; q += stride) {
+            if (save)
+                b[q] = a[q];
+            else
+                a[q] = b[q];
         }
-        if (save)
-            stored[q] = segs.live[k][off];
-        else
-            segs.live[k][off] = stored[q];
+        if (t0 == 0 && (n & 1)) {
+            if (save)
+                st[n - 1] = live[n - 1];
+            else
+                live[n - 1] = st[n - 1];
+        }
+    } else {
+        for (size_t q = t0; q < n; q += stride) {
+            if (save)
+                st[q] = live[q];
+            else
+                live[q] = st[q];
+        }
     }
 }
 
 hipError_t launch_state_copy(const StateSegs &segs, double *stored, bool save, hipStream_t st)
 {
-    size_t total = 0;
+    size_t most = 0;
     for (int k = 0; k < segs.nseg; ++k)
-        total += segs.count[k];
-    if (total == 0)
+        most = segs.count[k] > most ? segs.count[k] : most;
+    if (most == 0)
         return hipSuccess;
-    size_t blocks = (total + 255) / 256;
-    if (blocks > 262144)
-        blocks = 262144;
-    hipLaunchKernelGGL(k_state_copy, dim3((unsigned)blocks), dim3(256), 0, st, segs, stored, total,
-                       save ? 1 : 0);
+    size_t blocks = (most / 2 + 255) / 256;
+    if (blocks > 65536)
+        blocks = 65536;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(k_state_copy, dim3((unsigned)blocks, (unsigned)segs.nseg), dim3(256), 0, st,
+                       segs, stored, save ? 1 : 0);
     return hipGetLastError();
 }
 
