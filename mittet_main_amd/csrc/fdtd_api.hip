@@ -832,7 +832,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
         s->nblocks_l = d.lb_pad;
     }
     s->nblocks = s->kind == 2 ? d.zb_pad : d.fb_pad;
-    s->ncellblocks = tiles((int)N, 256);
+    s->ncellblocks = tiles((int)N, mt::FLUSH_CELLS);
     if ((rc = upload_desc(s)) || (rc = fwi3d_set_zero_eh(s))) {
         fwi3d_shot_destroy(s);
         return rc;

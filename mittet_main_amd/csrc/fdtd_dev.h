@@ -18,13 +18,14 @@ constexpr int TY = 4;    // rows per block
 // E ring slots: the volume DFT reads each slot once per `ring` steps, so it
 // costs 24 + 768/ring bytes per cell and step at 8 frequencies
 constexpr int RING_MAX = 256;
+constexpr int FLUSH_CELLS = 512;   // cells per DFT-flush block (256 lanes x 2)
 constexpr int ZX = 32;   // z-marching tile: 32 x 8 columns, one thread per column
 constexpr int ZY = 8;
 // LDS z-marching kernels of the CPML-free interior: 64 x MT_LY tiles marching
 // z-chunks of at most MT_LKB planes (shorter on small grids, so that a launch
 // still fills the CUs).  At 288^3, 64 x 8 tiles (half the y-halo re-reads of
-// 64 x 4) run the half-steps 6 % faster than 64 x 4; 64 x 16 gains nothing;
-// the chunk length moves the time there by about 1 %.
+// 64 x 4) run the interior kernels 2-2.5 % faster than 64 x 4; 64 x 16 gains
+// nothing; the chunk length moves the time there by about 1 %.
 #ifndef MT_LKB
 #define MT_LKB 16
 #endif
