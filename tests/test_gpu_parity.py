@@ -254,6 +254,28 @@ def test_fused_run_bitwise_tilings(dims, pair, monkeypatch):
     np.testing.assert_array_equal(shot.volume_dft().reshape(3, len(om), -1), acc)
 
 
+@pytest.mark.parametrize("dims,lkb", [((41, 31, 41), "1"), ((41, 31, 41), "3"),
+                                      ((41, 31, 41), "16"), ((40, 30, 42), "5")])
+def test_lds_chunk_lengths_bitwise(dims, lkb, monkeypatch):
+    """The LDS kernels' z-chunk length (chosen per grid; MITTET_FDTD_LKB forces
+    it) only changes which block marches which planes: 1-, 3-, 5- and
+    16-plane chunks, ragged last chunks, and the DFT flush with an odd cell
+    count (one cell per lane) and an even one (cell pairs), give the oracle's
+    bits."""
+    monkeypatch.setenv("MITTET_FDTD_LKB", lkb)
+    ix, iy, iz = dims
+    it = 50
+    m, shot, S, sig, signal, src, rec, om, dt = setup_pair(ix=ix, iy=iy, iz=iz, seed=23, it=it,
+                                                            ring=8)
+    shot.run()
+    w = oracle.dft_weights(it, dt, OM0, om)
+    tr, acc = S.forward(0, signal, tuple(c + 1 for c in src), it - 1, rec1=rec + 1, it=it, weights=w)
+    np.testing.assert_array_equal(shot.traces()[:3], tr)
+    for q, nm in enumerate(oracle.FIELD_NAMES):
+        np.testing.assert_array_equal(shot.field(q), S.arr(nm), err_msg=nm)
+    np.testing.assert_array_equal(shot.volume_dft().reshape(3, len(om), -1), acc)
+
+
 def test_zmarch_kernels_bitwise():
     """The z-marching kernel family (zchunk > 0) gives the same bits."""
     it = 60
