@@ -295,14 +295,59 @@ __device__ __forceinline__ double wnb(const AdjArgs &A, int comp, int ni, int nj
     return w * rprof(A, axis, h, id[axis]);
 }
 
-__device__ __forceinline__ double wpsi(const AdjArgs &A, int a, int arr, int ni, int nj, int nk,
-                                       bool h)
+// Both psi weight rows (r0, r1) of axis a at the four neighbours of a cell
+// along a (offsets off[0..3]); 0 where the neighbour is not on a CPML face
+// (HF: the CPML-H faces [1,nc-1)U[n-nc,n-2), else the CPML-E faces) or lies
+// on the last plane of another axis.  The tests of the other two axes and the
+// slab-index base are done once per cell.
+template <bool HF>
+__device__ __forceinline__ void psi_rows(const AdjArgs &A, int a, int i, int j, int k, int r0,
+                                         int r1, const int (&off)[4], double (&w0)[4],
+                                         double (&w1)[4])
 {
-    if (ni < 0 || nj < 0 || nk < 0 || ni >= A.ix || nj >= A.iy || nk >= A.iz)
-        return 0.0;
-    if (h ? !face_h(A, a, ni, nj, nk) : !face_e(A, a, ni, nj, nk))
-        return 0.0;
-    return A.tpsi[a][(size_t)arr * A.slab[a] + slabq(a, A, ni, nj, nk)];
+    const int nc = A.nc;
+    int r, n;
+    size_t base, stride;
+    bool others;
+    if (a == 0) {
+        r = i;
+        n = A.ix;
+        base = ((size_t)k * A.iy + j) * (2 * nc);
+        stride = 1;
+        others = j < A.iy - 1 && k < A.iz - 1;
+    } else if (a == 1) {
+        r = j;
+        n = A.iy;
+        base = (size_t)k * (2 * nc) * A.ix + i;
+        stride = A.ix;
+        others = i < A.ix - 1 && k < A.iz - 1;
+    } else {
+        r = k;
+        n = A.iz;
+        base = (size_t)j * A.ix + i;
+        stride = (size_t)A.ix * A.iy;
+        others = i < A.ix - 1 && j < A.iy - 1;
+    }
+    const GLOBAL double *T = (const GLOBAL double *)A.tpsi[a];
+    const size_t L = (size_t)A.slab[a];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int rr = r + off[q];
+        bool face;
+        if (HF)
+            face = (rr >= 1 && rr < nc - 1) || (rr >= n - nc && rr < n - 2);
+        else
+            face = (rr >= 2 && rr < (a == 0 ? nc - 1 : nc)) || (rr >= n - nc + 1 && rr < n - 2);
+        double v0 = 0.0, v1 = 0.0;
+        if (others && face) {
+            const int sp = rr < nc ? rr : rr - (n - 2 * nc);
+            const size_t qq = base + (size_t)sp * stride;
+            v0 = T[(size_t)r0 * L + qq];
+            v1 = T[(size_t)r1 * L + qq];
+        }
+        w0[q] = v0;
+        w1[q] = v1;
+    }
 }
 
 // H-phase part 2 (gathers): lE += transposed curl+ of the weights and the
@@ -315,15 +360,6 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
     const double c1 = A.c1, c2 = A.c2;
     const int N = A.n;
     double acc[3] = {0.0, 0.0, 0.0};
-    // D-_a w (c) = c1 w(c) - c1 w(c-s) + c2 w(c+s) - c2 w(c-2s)
-    auto dminus_psi = [&](int a, int arr) {
-        int o[3] = {0, 0, 0};
-        auto at = [&](int off) {
-            o[a] = off;
-            return wpsi(A, a, arr, i + o[0], j + o[1], k + o[2], true);
-        };
-        return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
-    };
     if (A.kappa_one && __all(core(A, i, j, k))) {
         const int st[3] = {1, A.ix, A.ix * A.iy};
         // D-_a of the weights of component d (stencil inside the grid)
@@ -357,13 +393,18 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
         }
     }
     const int idx[3] = {i, j, k};
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!__any(near_face(A, a, idx[a])))
             continue;
         const int p = (a + 1) % 3, q = (a + 2) % 3;
         // psiH0 <- D+_a E_q, psiH1 <- D+_a E_p; transpose of D+ is -D-
-        acc[q] -= dminus_psi(a, 2);
-        acc[p] -= dminus_psi(a, 3);
+        // (offsets 0, -1, +1, -2 of D-)
+        const int off[4] = {0, -1, 1, -2};
+        double w2[4], w3[4];
+        psi_rows<true>(A, a, i, j, k, 2, 3, off, w2, w3);
+        acc[q] -= c1 * w2[0] - c1 * w2[1] + c2 * w2[2] - c2 * w2[3];
+        acc[p] -= c1 * w3[0] - c1 * w3[1] + c2 * w3[2] - c2 * w3[3];
     }
     for (int d = 0; d < 3; ++d)
         A.lE[(size_t)d * N + c] += acc[d];
@@ -459,15 +500,6 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
     const double c1 = A.c1, c2 = A.c2;
     const int N = A.n;
     double acc[3] = {0.0, 0.0, 0.0};
-    // D+_a w (c) = c1 w(c+s) - c1 w(c) + c2 w(c+2s) - c2 w(c-s)
-    auto dplus_psi = [&](int a, int arr) {
-        int o[3] = {0, 0, 0};
-        auto at = [&](int off) {
-            o[a] = off;
-            return wpsi(A, a, arr, i + o[0], j + o[1], k + o[2], false);
-        };
-        return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
-    };
     if (A.kappa_one && __all(core(A, i, j, k))) {
         const int st[3] = {1, A.ix, A.ix * A.iy};
         // D+_a of the weights of component d (stencil inside the grid)
@@ -501,13 +533,18 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
         }
     }
     const int idx[3] = {i, j, k};
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!__any(near_face(A, a, idx[a])))
             continue;
         const int p = (a + 1) % 3, q = (a + 2) % 3;
         // psiE0 <- D-_a H_p, psiE1 <- D-_a H_q; transpose of D- is -D+
-        acc[p] -= dplus_psi(a, 0);
-        acc[q] -= dplus_psi(a, 1);
+        // (offsets +1, 0, +2, -1 of D+)
+        const int off[4] = {1, 0, 2, -1};
+        double w0[4], w1[4];
+        psi_rows<false>(A, a, i, j, k, 0, 1, off, w0, w1);
+        acc[p] -= c1 * w0[0] - c1 * w0[1] + c2 * w0[2] - c2 * w0[3];
+        acc[q] -= c1 * w1[0] - c1 * w1[1] + c2 * w1[2] - c2 * w1[3];
     }
     for (int d = 0; d < 3; ++d)
         A.lH[(size_t)d * N + c] += acc[d];
