@@ -78,8 +78,10 @@ def kernel_bytes(g, nf, ring):
     }
 
 
-def survey_step_bytes(g, nf, nrec):
-    """B_step = N(6*16+8) + sum_axes 2*ncpml*n^2*64 + N*3*Nf*32 + Nrec*3*Nf*32."""
+def dft_every_step_bytes(g, nf, nrec):
+    """Bytes of one step when the volume DFT accumulators are read and written
+    every step (no ring): N(6*16+8) + sum_axes 2*ncpml*n^2*64 + N*3*Nf*32 +
+    Nrec*3*Nf*32."""
     N = g.ncells
     n2 = g.ix * g.iy
     return N * (6 * 16 + 8) + 3 * 2 * g.ncpml * n2 * 64 + N * 3 * nf * 32 + nrec * 3 * nf * 32
@@ -157,8 +159,8 @@ def serve_mode(args):
 
 
 def serve_fwi_mode(args):
-    """Served FWI gradients with checkpointing (BASELINE cfg5, bounded):
-    384^3 interior + 16-cell CPML (416^3 cells), a seeded Poisson stream of
+    """Served FWI gradients with checkpointing, bounded: 384^3 interior +
+    16-cell CPML (416^3 cells), a seeded Poisson stream of
     forward (70 %) and gradient (30 %) requests with 1-2 x-dipole shots each,
     8 frequencies, 200 GB HBM budget; Nt = --fwi-it (default 1000) instead of
     a full-length run, so the stream finishes in minutes.  At this size the
@@ -184,7 +186,7 @@ def serve_fwi_mode(args):
     strip = lambda d: {k: (round(v, 4) if isinstance(v, float) else v)
                        for k, v in d.items() if k != "results"}
     gshots = sum(r.nshots for r in reqs if r.kind == "gradient")
-    out = {"mode": "serve-fwi", "metric": "gradients/s, p99 latency, peak HBM (cfg5)",
+    out = {"mode": "serve-fwi", "metric": "gradients/s, p99 latency, peak HBM (416^3 served FWI)",
            "workload": f"fwi_stream_{args.fwi_interior}cube_fp64", "seed": args.seed,
            "cells": (args.fwi_interior + 32) ** 3, "nt": args.fwi_it, "budget_gb": 200,
            "requests": len(reqs), "gradient_shots": gshots,
@@ -276,7 +278,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(shots shard across GPUs by construction; see DESIGN.md)", file=sys.stderr)
+              "(shots shard across GPUs by construction)", file=sys.stderr)
         sys.exit(2)
     if args.mode == "serve":
         return serve_mode(args)
@@ -324,7 +326,7 @@ def main():
     achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
     ms_step = wall / args.steps * 1e3
     value = g.ncells * args.steps / wall / 1e9
-    sb = survey_step_bytes(g, nf, len(wl.receivers))
+    sb = dft_every_step_bytes(g, nf, len(wl.receivers))
     result = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -348,7 +350,7 @@ def main():
                      "traffic": measured_traffic(wl.name, args.ring, dom),
                      "bytes_per_launch": kb[dom], "ms_per_launch": round(dom_ms, 5)},
         "kernel_ms": {k: round(v, 5) for k, v in prof.items()},
-        "step_vs_survey_bytes": {"survey_bytes_per_step": sb,
+        "step_vs_dft_every_step_bytes": {"bytes_per_step": sb,
                                  "equivalent_GBps": round(sb / (ms_step * 1e-3) / 1e9, 1)},
         "interior_gcells": round((g.ix - 2 * g.ncpml) ** 3 * args.steps / wall / 1e9, 4),
     }
