@@ -89,7 +89,8 @@ def dft_every_step_bytes(g, nf, nrec):
 
 def cpu_baseline(wl, seconds):
     """The oracle (reference loop structure, OpenMP) on the same workload: one
-    untimed warm-up step, then whole steps until `seconds` of wall time."""
+    untimed warm-up step, then whole steps until `seconds` of wall time or the
+    workload's last step (it - 2), whichever comes first."""
     import oracle
     g = wl.grid
     S = oracle.OracleSim(g.ix, g.iy, g.iz, g.ncpml, g.dx, g.dy, g.dz, g.dt, g.omega0, g.cmax)
@@ -104,7 +105,7 @@ def cpu_baseline(wl, seconds):
               traces=traces)                      # warm-up step
     steps = 0
     t0 = time.perf_counter()
-    while steps < 2 or time.perf_counter() - t0 < seconds:
+    while (steps < 2 or time.perf_counter() - t0 < seconds) and 1 + steps <= wl.it - 2:
         S.forward(0, wl.signal, src1, 1, rec1=wl.receivers + 1, it=wl.it, step0=1 + steps,
                   weights=w, acc=acc, traces=traces)
         steps += 1

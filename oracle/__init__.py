@@ -346,6 +346,19 @@ class OracleSim:
             wp, ap = _dp(w.view(np.float64)), _dp(acc.view(np.float64))
         else:
             wp = ap = None
+        # steps n in [step0, step0 + nsteps) read signal[n + 1], write trace
+        # sample n and read weight row n: refuse what does not fit
+        end = step0 + nsteps
+        if step0 < 0 or nsteps < 0:
+            raise ValueError(f"step0 {step0} and nsteps {nsteps} must be >= 0")
+        if sig is not None and end + 1 > sig.size:
+            raise ValueError(f"steps up to {end - 1} need {end + 1} signal samples, got {sig.size}")
+        if nrec and (traces.shape != (3, nrec, it) or end > it):
+            raise ValueError(f"traces {traces.shape} cannot hold samples up to {end - 1} "
+                             f"of (3, {nrec}, {it})")
+        if nf and (end > w.shape[0] or acc.shape != (3, nf, self.n)):
+            raise ValueError(f"weights of {w.shape[0]} steps / accumulators {acc.shape} do not fit "
+                             f"steps up to {end - 1}")
         src = src1 if src1 is not None else (1, 1, 1)
         lib().or_forward(self._s, comp, _dp(sig) if sig is not None else None,
                          int(src[0]), int(src[1]), int(src[2]), step0, nsteps,
