@@ -243,6 +243,12 @@ int lds_chunk(int ltx, int lty, int nz)
     return 4;
 }
 
+// Grids from this many cells up take the LDS family.  Graph-captured steps
+// with a 4-frequency volume DFT, 16 receivers, Gcell-updates/s of the LDS /
+// per-cell family: 84^3 9.8 / 12.7, 120^3 14.3 / 16.1, 152^3 16.9 / 15.7,
+// 216^3 18.1 / 16.5.  MITTET_FDTD_LDS=1 takes it on any grid.
+constexpr long long kLdsMinCells = 2500000;
+
 constexpr long long kSlotPad = 0;
 constexpr long long kAccPad = 0;
 
@@ -800,11 +806,13 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     }
     // family 3: LDS z-marching kernels over the CPML-free interior (rows start
     // 128-byte aligned, at ncpml rounded up to 16 cells), per-cell kernels over
-    // the tiles that hold any other cell
+    // the tiles that hold any other cell.  Below kLdsMinCells the per-cell
+    // family alone (two launches per step instead of four) is faster.
     d.shell = nullptr;
     const int xlo = (p.ncpml + 15) / 16 * 16;
-    if (s->kind == 0 && p.ncpml >= 2 && xlo < p.ix - p.ncpml && p.ncpml < p.iy - p.ncpml &&
-        p.ncpml < p.iz - p.ncpml && !env_pad("MITTET_FDTD_NOLDS", 0)) {
+    const bool lds_size = (long long)N >= kLdsMinCells || env_pad("MITTET_FDTD_LDS", 0);
+    if (s->kind == 0 && lds_size && p.ncpml >= 2 && xlo < p.ix - p.ncpml &&
+        p.ncpml < p.iy - p.ncpml && p.ncpml < p.iz - p.ncpml && !env_pad("MITTET_FDTD_NOLDS", 0)) {
         const int lo[3] = {xlo, p.ncpml, p.ncpml};
         const int hi[3] = {p.ix - p.ncpml, p.iy - p.ncpml, p.iz - p.ncpml};
         for (int a = 0; a < 3; ++a) {

@@ -26,3 +26,11 @@ def hip_lib():
     """The HIP library, loaded; GPU tests fail loudly if it is missing."""
     from mittet_main_amd import _lib
     return _lib.load()
+
+
+@pytest.fixture(autouse=True)
+def _lds_family_on_small_grids(monkeypatch):
+    """The library takes the LDS kernel family only from 2.5 M cells up; the
+    parity tests run on small grids, so they force it there (tests of the
+    size-based choice itself remove the variable)."""
+    monkeypatch.setenv("MITTET_FDTD_LDS", "1")

@@ -92,8 +92,13 @@ def test_backprop_sign_variants_bitwise():
         np.testing.assert_array_equal(shot.field(q), S.arr(nm), err_msg=nm)
 
 
+@pytest.mark.parametrize("family", ["lds", "auto"])
 @pytest.mark.parametrize("ring,pieces", [(0, None), (8, (5, 3, 19, 8, 100)), (1, (7, 50, 100))])
-def test_fused_run_bitwise(ring, pieces):
+def test_fused_run_bitwise(ring, pieces, family, monkeypatch):
+    """Forced LDS family, and the size-based choice (per-cell kernels alone on
+    this small grid)."""
+    if family == "auto":
+        monkeypatch.delenv("MITTET_FDTD_LDS", raising=False)
     it = 90
     m, shot, S, sig, signal, src, rec, om, dt = setup_pair(seed=5, it=it, ring=ring)
     if pieces is None:
