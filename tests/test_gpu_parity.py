@@ -156,20 +156,31 @@ def test_laplace_to_freq_vs_oracle():
             np.testing.assert_allclose(got["J"], want["J"], rtol=1e-10)
 
 
-def test_batch_mixed_sizes_equal_alone():
+@pytest.mark.parametrize("families", ["lds", "mixed"])
+def test_batch_mixed_sizes_equal_alone(families, monkeypatch):
     """Shots of different grid sizes in one persistent launch give the same
-    bits as each run alone; one joins mid-way (continuous batching)."""
+    bits as each run alone; one joins mid-way (continuous batching).  "mixed":
+    the second shot runs the per-cell family alone (as grids below 2.5 M cells
+    do), the others the LDS family, in the same launches."""
     specs = [dict(ix=30, iy=26, iz=22, seed=10, it=70), dict(ix=24, iy=28, iz=20, seed=11, it=50),
              dict(ix=34, iy=22, iz=26, seed=12, it=90)]
+
+    def make(q, sp):
+        if families == "mixed" and q == 1:
+            monkeypatch.setenv("MITTET_FDTD_NOLDS", "1")
+        out = setup_pair(ring=8, **sp)
+        monkeypatch.delenv("MITTET_FDTD_NOLDS", raising=False)
+        return out
+
     alone, mixed = [], []
-    for sp in specs:
-        m, shot, *_ = setup_pair(ring=8, **sp)
+    for q, sp in enumerate(specs):
+        m, shot, *_ = make(q, sp)
         shot.run()
         alone.append((shot.traces(), shot.volume_dft(), [shot.field(q) for q in range(6)]))
     keep = []
     b = Batch(8, chunk=8)
     for q, sp in enumerate(specs):
-        m, shot, *_ = setup_pair(ring=8, **sp)
+        m, shot, *_ = make(q, sp)
         keep.append((m, shot))
     b.add(keep[0][1])
     b.add(keep[1][1])
