@@ -27,10 +27,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s + % HBM peak (256³ fp64); served shots/s, p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-# the launches that make up each timed half-step of the default (LDS) kernel family
+# the launches that make up each timed half-step: the LDS family from 2.5 M
+# cells up (kLdsMinCells in fdtd_api.hip), the per-cell family alone below
+LDS_MIN_CELLS = 2_500_000
 HALF_STEP_KERNELS = {
     "efield": "E half-step: k_efield_f (CPML-shell tiles) + k_efield_l (interior, LDS z-march)",
     "hfield": "H half-step: k_hfield_f (CPML-shell tiles) + k_hfield_l (interior, LDS z-march)",
+    "flush": "k_flush (volume DFT, per ring)",
+}
+HALF_STEP_KERNELS_PER_CELL = {
+    "efield": "E half-step: k_efield_f (per-cell tiles over the whole grid)",
+    "hfield": "H half-step: k_hfield_f (per-cell tiles over the whole grid)",
     "flush": "k_flush (volume DFT, per ring)",
 }
 
@@ -344,7 +351,9 @@ def main():
         "config": {"workload": wl.name, "interior": g.ix - 2 * g.ncpml, "ncpml": g.ncpml,
                    "cells": g.ncells, "nt": wl.it, "nf_vol": nf, "nrec": len(wl.receivers),
                    "ring": args.ring, "zchunk": args.zchunk, "sigma_sha256_16": configs.sigma_digest(wl.sigma)},
-        "roofline": {"bound": "hbm", "kernel": HALF_STEP_KERNELS.get(dom, dom),
+        "roofline": {"bound": "hbm",
+                     "kernel": (HALF_STEP_KERNELS if g.ncells >= LDS_MIN_CELLS
+                                else HALF_STEP_KERNELS_PER_CELL).get(dom, dom),
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
