@@ -281,6 +281,7 @@ struct fwi3d_shot {
     std::vector<int> rec;
     double *d_e = nullptr, *d_h = nullptr, *d_psi[3] = {nullptr, nullptr, nullptr};
     double *d_amp = nullptr, *d_traces = nullptr, *d_w = nullptr, *d_acc = nullptr;
+    double *d_acc_h = nullptr;   // H volume spectra (fwi3d_enable_volume_dft_h)
     int *d_rec = nullptr, *d_step = nullptr, *d_pre = nullptr;
     ShotDev desc{};
     ShotDev *d_desc = nullptr;
@@ -410,7 +411,7 @@ int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t
 
 // launches for local steps j in [0, L) followed by the DFT flush and step advance
 int launch_chunk(const ShotDev *d, int ns, const Fam &f, const int *pre_cells, int ncellblocks,
-                 int L, bool any_dft, bool any_rec, hipStream_t st)
+                 int L, bool any_dft, bool any_rec, hipStream_t st, bool any_hdft = false)
 {
     int rc;
     for (int j = 0; j < L; ++j) {
@@ -418,6 +419,8 @@ int launch_chunk(const ShotDev *d, int ns, const Fam &f, const int *pre_cells, i
             return rc;
         if (any_rec)
             HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
+        if (any_hdft)
+            HIP_TRY(mt::launch_hdft(d, ns, pre_cells, ncellblocks, j, st));
     }
     if (any_dft)
         HIP_TRY(mt::launch_flush(d, ns, pre_cells, ncellblocks, L, st));
@@ -427,12 +430,12 @@ int launch_chunk(const ShotDev *d, int ns, const Fam &f, const int *pre_cells, i
 
 int capture_chunk(hipGraphExec_t *out, const ShotDev *d, int ns, const Fam &f,
                   const int *pre_cells, int ncellblocks, int L, bool any_dft, bool any_rec,
-                  hipStream_t st)
+                  hipStream_t st, bool any_hdft = false)
 {
     hipGraph_t g = nullptr;
     std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = launch_chunk(d, ns, f, pre_cells, ncellblocks, L, any_dft, any_rec, st);
+    int rc = launch_chunk(d, ns, f, pre_cells, ncellblocks, L, any_dft, any_rec, st, any_hdft);
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc)
         return rc;
@@ -787,6 +790,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.nf = s->nf;
     d.w = s->d_w;
     d.acc = s->d_acc;
+    d.acc_h = nullptr;
     d.step = s->d_step;
     d.variant = m->variant;
     d.mu = m->d_mu;
@@ -889,6 +893,7 @@ int fwi3d_shot_destroy(fwi3d_shot *s)
     dfree(s->d_traces);
     dfree(s->d_w);
     dfree(s->d_acc);
+    dfree(s->d_acc_h);
     dfree(s->d_rec);
     dfree(s->d_step);
     dfree(s->d_pre);
@@ -915,6 +920,8 @@ int fwi3d_set_zero_eh(fwi3d_shot *s)
     HIP_TRY(mt::launch_fill(s->d_traces, std::max<size_t>(1, 6 * (size_t)s->nrec * s->it), 0.0, s->st));
     if (s->nf)
         HIP_TRY(mt::launch_fill(s->d_acc, 2 * 3 * (size_t)s->nf * s->desc.astride, 0.0, s->st));
+    if (s->d_acc_h)
+        HIP_TRY(mt::launch_fill(s->d_acc_h, 2 * 3 * (size_t)s->nf * s->desc.astride, 0.0, s->st));
     static const int kZero = 0;
     HIP_TRY(hipMemcpyAsync(s->d_step, &kZero, sizeof(int), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -997,12 +1004,13 @@ int fwi3d_step_n(fwi3d_shot *s, int nsteps)
         const int L = std::min(chunk, left);
         if (L == chunk) {
             if (!s->graph && (rc = capture_chunk(&s->graph, s->d_desc, 1, shot_fam(s), s->d_pre,
-                                                 s->ncellblocks, L, dft, rec, s->st)))
+                                                 s->ncellblocks, L, dft, rec, s->st,
+                                                 s->d_acc_h != nullptr)))
                 return rc;
             HIP_TRY(hipGraphLaunch(s->graph, s->st));
         } else {
             if ((rc = launch_chunk(s->d_desc, 1, shot_fam(s), s->d_pre, s->ncellblocks, L, dft, rec,
-                                   s->st)))
+                                   s->st, s->d_acc_h != nullptr)))
                 return rc;
         }
         s->step += L;
@@ -1058,6 +1066,9 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
                 });
             if (!rc && s->nrec)
                 rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
+            if (!rc && s->d_acc_h &&
+                mt::launch_hdft(s->d_desc, 1, s->d_pre, s->ncellblocks, j, s->st) != hipSuccess)
+                rc = fail(FWI3D_EDEVICE, "H DFT launch");
         }
         if (!rc && s->nf)
             rc = timed(3, [&] { return mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st); });
@@ -1179,6 +1190,46 @@ int fwi3d_get_volume_dft(fwi3d_shot *s, double *out)
     if (!s->nf)
         return fail(FWI3D_ESTATE, "shot has no volume DFT frequencies");
     HIP_TRY(hipMemcpy2DAsync(out, sizeof(double) * 2 * (size_t)s->desc.n, s->d_acc,
+                             sizeof(double) * 2 * (size_t)s->desc.astride,
+                             sizeof(double) * 2 * (size_t)s->desc.n, 3 * (size_t)s->nf,
+                             hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return FWI3D_OK;
+}
+
+int fwi3d_enable_volume_dft_h(fwi3d_shot *s)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    if (!s->nf)
+        return fail(FWI3D_ESTATE, "shot has no volume DFT frequencies");
+    if (s->batch)
+        return fail(FWI3D_ESTATE, "shot is in a batch");
+    if (s->step != 0)
+        return fail(FWI3D_ESTATE, "enable the H volume DFT before the first step (step %d)",
+                    s->step);
+    if (s->d_acc_h)
+        return FWI3D_OK;
+    const size_t count = 2 * 3 * (size_t)s->nf * s->desc.astride;
+    if (dalloc(&s->d_acc_h, count) != hipSuccess)
+        return fail(FWI3D_ENOMEM, "H volume accumulators (%zu doubles)", count);
+    HIP_TRY(mt::launch_fill(s->d_acc_h, count, 0.0, s->st));
+    s->desc.acc_h = s->d_acc_h;
+    s->desc_dirty = true;
+    if (s->graph) {   // recaptured with the H DFT launches
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    return upload_desc(s);
+}
+
+int fwi3d_get_volume_dft_h(fwi3d_shot *s, double *out)
+{
+    if (!s || !out)
+        return fail(FWI3D_EINVAL, "null argument");
+    if (!s->d_acc_h)
+        return fail(FWI3D_ESTATE, "shot has no H volume DFT (fwi3d_enable_volume_dft_h)");
+    HIP_TRY(hipMemcpy2DAsync(out, sizeof(double) * 2 * (size_t)s->desc.n, s->d_acc_h,
                              sizeof(double) * 2 * (size_t)s->desc.astride,
                              sizeof(double) * 2 * (size_t)s->desc.n, 3 * (size_t)s->nf,
                              hipMemcpyDeviceToHost, s->st));
@@ -2483,22 +2534,23 @@ int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
     if (ns == 0)
         return FWI3D_OK;
     int left = 0;
-    bool dft = false, rec = false;
+    bool dft = false, rec = false, hdft = false;
     for (auto *s : b->shots) {
         left = std::max(left, std::min(nsteps, s->it - 1 - s->step));
         dft = dft || s->nf > 0;
         rec = rec || s->nrec > 0;
+        hdft = hdft || s->d_acc_h != nullptr;
     }
     while (left > 0) {
         const int L = std::min(b->chunk, left);
         if (L == b->chunk) {
             if (!b->graph && (rc = capture_chunk(&b->graph, b->d_desc, ns, batch_fam(b),
                                                  b->d_pre_cells, b->ncellblocks, L, dft, rec,
-                                                 b->st)))
+                                                 b->st, hdft)))
                 return rc;
             HIP_TRY(hipGraphLaunch(b->graph, b->st));
         } else if ((rc = launch_chunk(b->d_desc, ns, batch_fam(b), b->d_pre_cells,
-                                      b->ncellblocks, L, dft, rec, b->st))) {
+                                      b->ncellblocks, L, dft, rec, b->st, hdft))) {
             return rc;
         }
         left -= L;

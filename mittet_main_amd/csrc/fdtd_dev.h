@@ -76,6 +76,7 @@ struct ShotDev {
     int nf;
     const double *w;          // complex [it][nf]
     double *acc;              // complex [3][nf] rows of astride
+    double *acc_h;            // H volume spectra, same layout (null: off)
     int *step;                // absolute step of local index j = 0 of the current chunk
     int variant;              // 0: C v1.4 semantics; 1: Fortran ver0040 (fdtd_fortran.hip)
     int nsrc_h;               // variant 1: magnetic (Hz) source points; amp[q * (it - 1) + n]
@@ -154,6 +155,9 @@ hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inpl
 hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int L,
                         hipStream_t st);
 hipError_t launch_advance(const ShotDev *d, int ns, int L, hipStream_t st);
+// H volume running DFT of local step j (shots with acc_h), after the H update
+hipError_t launch_hdft(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
+                       hipStream_t st);
 hipError_t launch_receiver_dft(const double *traces, int ncomp_rows, int it, double dt,
                                double omega0, int nfreq, const double *omegas,
                                double *out, hipStream_t st);

@@ -608,6 +608,44 @@ __global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__
     }
 }
 
+// H volume running DFT, one step: acc_h += H[n] * w[n] for the step just
+// completed, after the H update (the sample the H receiver traces take),
+// accumulated per step in step order like the E sums.  H is updated in
+// place, so there is no ring to defer this to.
+__global__ __launch_bounds__(256) void k_hdft(const ShotDev *__restrict__ D, int ns,
+                                              const int *__restrict__ pre, int j)
+{
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    const int n = *S.step + j;
+    if (!S.acc_h || n >= S.it - 1)
+        return;
+    const int nf = S.nf;
+    const size_t N = (size_t)S.n;
+    const GLOBAL double *__restrict__ W = gc(S.w) + 2 * (size_t)n * nf;
+    GLOBAL double *__restrict__ A = gm(S.acc_h);
+    const int base = (blockIdx.x - pre[s]) * FLUSH_CELLS;
+    for (int c = base + threadIdx.x; c < base + FLUSH_CELLS && c < S.n; c += blockDim.x) {
+        for (int d = 0; d < 3; ++d) {
+            const double h = gc(S.h)[(size_t)d * N + c];
+            for (int f = 0; f < nf; ++f) {
+                const size_t q = 2 * (((size_t)d * nf + f) * S.astride + c);
+                A[q] += h * W[2 * f];
+                A[q + 1] += h * W[2 * f + 1];
+            }
+        }
+    }
+}
+
+hipError_t launch_hdft(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
+                       hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_hdft, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, j);
+    return hipGetLastError();
+}
+
 __global__ void k_advance(const ShotDev *__restrict__ D, int ns, int L)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;

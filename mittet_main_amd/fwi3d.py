@@ -313,6 +313,18 @@ class Shot:
         check(_lib.load().fwi3d_get_volume_dft(self._h, _dp(out)))
         return out.reshape(3, self.nf, *g.shape)
 
+    def enable_volume_dft_h(self) -> None:
+        """Also accumulate H over the volume at the shot's volume-DFT
+        frequencies (sum_n H[n] w_n, H after step n); before the first step."""
+        check(_lib.load().fwi3d_enable_volume_dft_h(self._h))
+
+    def volume_dft_h(self) -> np.ndarray:
+        """(3, nf_vol, iz, iy, ix) complex: sum_n H[n] dt exp((i-1) sqrt(omega0 omega) n dt)."""
+        g = self.model.grid
+        out = np.zeros((3, self.nf, g.ncells), dtype=np.complex128)
+        check(_lib.load().fwi3d_get_volume_dft_h(self._h, _dp(out)))
+        return out.reshape(3, self.nf, *g.shape)
+
     def receiver_dft(self, omegas: Sequence[float], h_factor: bool = True) -> np.ndarray:
         """(6, nrec, nfreq) complex receiver spectra (f_to_d_e / f_to_d_h)."""
         om = np.ascontiguousarray(omegas, dtype=np.float64)
