@@ -262,10 +262,10 @@ def main():
     ap.add_argument("--fwi-interior", type=int, default=384)
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2048)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default 2048; 1024 for plain_64)")
     ap.add_argument("--warmup", type=int, default=128)
     ap.add_argument("--workload", default="roofline_256", choices=["roofline_256", "plain_64"])
-    ap.add_argument("--ring", type=int, default=128)
+    ap.add_argument("--ring", type=int, default=0, help="DFT ring slots (default 128; 64 for plain_64)")
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps timed per kernel after the timed region (default: one ring)")
@@ -300,6 +300,11 @@ def main():
     wl = configs.roofline_256() if args.workload == "roofline_256" else configs.plain_64()
     g = wl.grid
     nf = len(wl.omegas)
+    small = args.workload == "plain_64"   # 1999 steps in all
+    if args.steps <= 0:
+        args.steps = 1024 if small else 2048
+    if args.ring <= 0:
+        args.ring = 64 if small else 128
     if args.profile_steps <= 0:
         args.profile_steps = args.ring
     need = args.warmup + args.steps + args.profile_steps

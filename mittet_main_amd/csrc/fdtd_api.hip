@@ -382,7 +382,17 @@ struct Fam {
     int nblocks_l;
 };
 
-int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t st)
+// Families whose E half-step has a per-cell launch that can carry the
+// receiver gather of the previous step.
+bool rec_folds(const Fam &f)
+{
+    return (f.kind == 0 || f.kind == 3) && f.nblocks > 0;
+}
+
+// rec_prev: the E launch of step j also gathers step j - 1's receivers
+// (rec_folds(f) must hold)
+int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t st,
+              bool rec_prev = false)
 {
     hipError_t e;
     if (f.kind == 4) {
@@ -401,7 +411,7 @@ int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t
               : mt::launch_efield_p(d, ns, f.pre, f.nblocks, j, 1, st);
     else
         e = h ? mt::launch_hfield_f(d, ns, f.pre, f.nblocks, j, 1, st)
-              : mt::launch_efield_f(d, ns, f.pre, f.nblocks, j, 1, st);
+              : mt::launch_efield_f(d, ns, f.pre, f.nblocks, j, 1, st, rec_prev);
     HIP_TRY(e);
     if (f.kind == 3)
         HIP_TRY(h ? mt::launch_hfield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st)
@@ -414,10 +424,12 @@ int launch_chunk(const ShotDev *d, int ns, const Fam &f, const int *pre_cells, i
                  int L, bool any_dft, bool any_rec, hipStream_t st, bool any_hdft = false)
 {
     int rc;
+    const bool fold = any_rec && rec_folds(f);   // step j's gather rides on step j + 1's E launch
     for (int j = 0; j < L; ++j) {
-        if ((rc = launch_eh(f, d, ns, j, false, st)) || (rc = launch_eh(f, d, ns, j, true, st)))
+        if ((rc = launch_eh(f, d, ns, j, false, st, fold && j > 0)) ||
+            (rc = launch_eh(f, d, ns, j, true, st)))
             return rc;
-        if (any_rec)
+        if (any_rec && !(fold && j + 1 < L))
             HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
         if (any_hdft)
             HIP_TRY(mt::launch_hdft(d, ns, pre_cells, ncellblocks, j, st));

@@ -131,8 +131,9 @@ hipError_t launch_efield_z(const ShotDev *d, int ns, const int *pre, int nblocks
                            int sign, hipStream_t st);
 hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
+// rec_prev (j > 0): ns more blocks gather the receivers of local step j - 1
 hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
-                           int sign, hipStream_t st);
+                           int sign, hipStream_t st, bool rec_prev = false);
 hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_efield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,

@@ -385,6 +385,19 @@ __global__ void k_source(const ShotDev *__restrict__ D, int n)
     E[S.src_cell] -= gc(S.amp)[n];
 }
 
+// Trace samples of step n: E from ring slot so and H, one block per shot.
+__device__ __forceinline__ void rec_store(const ShotDev &S, int n, int so)
+{
+    const int tot = 6 * S.nrec;
+    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
+        const int comp = q / S.nrec, r = q - comp * S.nrec;
+        const int cell = S.rec_cell[r];
+        const double v = comp < 3 ? S.e[(size_t)so * S.sstride + (size_t)comp * S.n + cell]
+                                  : S.h[(size_t)(comp - 3) * S.n + cell];
+        S.traces[((size_t)comp * S.nrec + r) * S.it + n] = v;
+    }
+}
+
 // Receiver gather: E (from the slot written this step) and H after the step.
 __global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int inplace)
 {
@@ -397,14 +410,7 @@ __global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int
     }
     if (!inplace)
         so = (S.cur + j + 1) % S.ring;
-    const int tot = 6 * S.nrec;
-    for (int q = threadIdx.x; q < tot; q += blockDim.x) {
-        const int comp = q / S.nrec, r = q - comp * S.nrec;
-        const int cell = S.rec_cell[r];
-        const double v = comp < 3 ? S.e[(size_t)so * S.sstride + (size_t)comp * S.n + cell]
-                                  : S.h[(size_t)(comp - 3) * S.n + cell];
-        S.traces[((size_t)comp * S.nrec + r) * S.it + n] = v;
-    }
+    rec_store(S, n, so);
 }
 
 // Volume running DFT over the last L ring slots: acc += E[n] * w[n] in step
@@ -1109,16 +1115,33 @@ __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, 
              k < S.hi[2]);
 }
 
+// The first recn blocks (recn: 0, or ns rounded up to a multiple of 8 so that
+// the tile blocks keep their XCD) gather the receivers of the previous local
+// step j - 1, block s for shot s: its E slot is the one this step reads and H
+// is not written until the H half-step, so the samples are the ones a receiver
+// launch after step j - 1 takes, with one graph node less per step.  Leading,
+// they overlap the tile blocks instead of trailing the launch.
 __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__ D, int ns,
-                                                     const int *__restrict__ pre, int j, int sign)
+                                                     const int *__restrict__ pre, int j, int sign,
+                                                     int recn)
 {
-    const int s = find_shot(pre, ns, blockIdx.x);
+    if ((int)blockIdx.x < recn) {
+        if ((int)blockIdx.x < ns) {
+            const ShotDev &R = D[blockIdx.x];
+            const int n = *R.step + j - 1;
+            if (n < R.it - 1)
+                rec_store(R, n, (R.cur + j) % R.ring);
+        }
+        return;
+    }
+    const int b = blockIdx.x - recn;
+    const int s = find_shot(pre, ns, b);
     const ShotDev &S = D[s];
     const int n = *S.step + j;
     if (n >= S.it - 1)
         return;
     int i, jj, k;
-    if (!fcell(S, blockIdx.x - pre[s], i, jj, k))
+    if (!fcell(S, b - pre[s], i, jj, k))
         return;
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
     const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
@@ -1225,11 +1248,13 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
 }
 
 hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
-                           hipStream_t st)
+                           hipStream_t st, bool rec_prev)
 {
     if (nblocks <= 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(k_efield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+        return rec_prev ? hipErrorInvalidValue : hipSuccess;
+    const int recn = rec_prev && j > 0 ? (ns + 7) & ~7 : 0;
+    hipLaunchKernelGGL(k_efield_f, dim3(recn + nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign,
+                       recn);
     return hipGetLastError();
 }
 
