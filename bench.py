@@ -249,6 +249,25 @@ def gradient_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def forward_step_counts(args, workload_steps):
+    """Fills the forward mode's unset step counts (0) with the workload's
+    defaults: 2048 timed steps and a 128-slot ring at 288^3, 1024 and 64 on the
+    84^3 workload of 1999 steps.  Returns an error message when warm-up, timed
+    and profiled steps together exceed the workload, else None."""
+    small = args.workload == "plain_64"
+    if args.steps <= 0:
+        args.steps = 1024 if small else 2048
+    if args.ring <= 0:
+        args.ring = 64 if small else 128
+    if args.profile_steps <= 0:
+        args.profile_steps = args.ring
+    need = args.warmup + args.steps + args.profile_steps
+    if need > workload_steps:
+        return (f"--warmup + --steps + --profile-steps = {need} exceeds the {workload_steps} "
+                "steps of the workload")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mode", default="forward", choices=["forward", "serve", "gradient", "serve-fwi"])
@@ -300,17 +319,9 @@ def main():
     wl = configs.roofline_256() if args.workload == "roofline_256" else configs.plain_64()
     g = wl.grid
     nf = len(wl.omegas)
-    small = args.workload == "plain_64"   # 1999 steps in all
-    if args.steps <= 0:
-        args.steps = 1024 if small else 2048
-    if args.ring <= 0:
-        args.ring = 64 if small else 128
-    if args.profile_steps <= 0:
-        args.profile_steps = args.ring
-    need = args.warmup + args.steps + args.profile_steps
-    if need > wl.steps:
-        sys.exit(f"--warmup + --steps + --profile-steps = {need} exceeds the {wl.steps} steps "
-                 f"of the workload")
+    err = forward_step_counts(args, wl.steps)
+    if err:
+        sys.exit(err)
     log(f"[bench] workload {wl.name}: {g.ix}x{g.iy}x{g.iz} cells, it={wl.it}, nf={nf}, "
         f"nrec={len(wl.receivers)}, ring={args.ring}")
     model = Model(g)

@@ -1,7 +1,8 @@
 """CPU side of the measurement: the CPU-oracle baseline stays inside the
 workload's time steps however fast the host is, and the oracle refuses step
 ranges its arrays cannot hold (a range past the last sample once overran the
-signal, trace and weight arrays of the 84^3 workload on a 16-thread host)."""
+signal, trace and weight arrays of the 84^3 workload on a 16-thread host).
+The forward mode's default step counts fit each workload."""
 import os
 import sys
 
@@ -48,3 +49,16 @@ def test_oracle_refuses_steps_past_the_arrays():
         S.forward(0, None, src1, 2, it=wl.it + 4, step0=wl.it - 1, weights=w)
     with pytest.raises(ValueError):
         S.forward(0, wl.signal, src1, -1, it=wl.it)
+
+
+def test_forward_defaults_fit_each_workload():
+    import argparse
+    for name, make, steps, ring in (("roofline_256", configs.roofline_256, 2048, 128),
+                                    ("plain_64", configs.plain_64, 1024, 64)):
+        wl_steps = make().steps
+        a = argparse.Namespace(workload=name, steps=0, ring=0, warmup=128, profile_steps=0)
+        assert bench.forward_step_counts(a, wl_steps) is None
+        assert (a.steps, a.ring, a.profile_steps) == (steps, ring, ring)
+        assert a.warmup + a.steps + a.profile_steps <= wl_steps
+    a = argparse.Namespace(workload="plain_64", steps=2048, ring=0, warmup=128, profile_steps=0)
+    assert "exceeds" in bench.forward_step_counts(a, configs.plain_64().steps)
