@@ -44,3 +44,42 @@ def test_every_trace_sample_matches_oracle_steps(family, pieces, monkeypatch):
     assert got.shape == want.shape
     np.testing.assert_array_equal(got, want)
     assert np.abs(got[3:]).max() > 0
+
+
+@pytest.mark.parametrize("family", ["lds", "auto"])
+def test_ten_shot_batch_traces_equal_alone(family, monkeypatch):
+    """More shots than one group of 8 gather blocks: ten shots of four grid
+    sizes and three lengths, four of them joining after 11 steps, give every
+    trace sample, field and volume DFT of each run alone."""
+    from mittet_main_amd import Batch
+    if family == "auto":
+        monkeypatch.delenv("MITTET_FDTD_LDS", raising=False)
+    dims = [(26, 24, 20), (30, 22, 24), (24, 28, 22), (28, 26, 26)]
+    specs = [dict(ix=dims[q % 4][0], iy=dims[q % 4][1], iz=dims[q % 4][2], seed=50 + q,
+                  it=(30, 41, 52)[q % 3]) for q in range(10)]
+    print(f"ten-shot batch: {family} family", flush=True)
+    alone = []
+    for sp in specs:
+        m, shot, *_ = setup_pair(ring=8, **sp)
+        shot.run()
+        alone.append((shot.traces(), shot.volume_dft(), [shot.field(c) for c in range(6)]))
+    keep = [setup_pair(ring=8, **sp)[:2] for sp in specs]
+    b = Batch(12, chunk=8)
+    for m, shot in keep[:6]:
+        b.add(shot)
+    b.step(11)
+    for m, shot in keep[6:]:
+        b.add(shot)
+    for _ in range(8):
+        b.step(8)
+    b.sync()
+    for m, shot in keep:
+        b.remove(shot)
+    for q, (m, shot) in enumerate(keep):
+        tr, vol, f = alone[q]
+        assert shot.step == specs[q]["it"] - 1
+        np.testing.assert_array_equal(shot.traces(), tr, err_msg=f"shot {q}")
+        np.testing.assert_array_equal(shot.volume_dft(), vol, err_msg=f"shot {q}")
+        for c in range(6):
+            np.testing.assert_array_equal(shot.field(c), f[c], err_msg=f"shot {q} field {c}")
+    b.close()
