@@ -183,9 +183,10 @@ int fwi3d_get_volume_dft(fwi3d_shot *s, double *out);
  * H at the receivers only (ver0040/cpu_f_to_d_h.f90:134-167); this carries
  * the volume E sums of laplaceToFreq_2 (satan_i/fwi3d_cpu_function_v1.4.c:
  * 2091-2200) over to H.  The f_to_d_h factor sqrt(-2 omega0/(i omega)) is the
- * caller's (fwi3d_f_to_d_h).  Off by default: H is updated in place, so its
- * sums cost 2*3*nf_vol*16 bytes per cell and step.  Enable before the first
- * step; not in a batch. */
+ * caller's (fwi3d_f_to_d_h).  Off by default: H is updated in place, so
+ * enabling allocates a second ring of the shot's ring slots (ring*3*8 bytes
+ * per cell) that takes a copy of H each step and is summed once per ring like
+ * the E slots.  Enable before the first step; not in a batch. */
 int fwi3d_enable_volume_dft_h(fwi3d_shot *s);
 int fwi3d_get_volume_dft_h(fwi3d_shot *s, double *out);
 

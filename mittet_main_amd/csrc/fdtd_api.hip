@@ -282,6 +282,7 @@ struct fwi3d_shot {
     double *d_e = nullptr, *d_h = nullptr, *d_psi[3] = {nullptr, nullptr, nullptr};
     double *d_amp = nullptr, *d_traces = nullptr, *d_w = nullptr, *d_acc = nullptr;
     double *d_acc_h = nullptr;   // H volume spectra (fwi3d_enable_volume_dft_h)
+    double *d_hring = nullptr;   // H after each step of a chunk, for the H sums
     int *d_rec = nullptr, *d_step = nullptr, *d_pre = nullptr;
     ShotDev desc{};
     ShotDev *d_desc = nullptr;
@@ -432,10 +433,12 @@ int launch_chunk(const ShotDev *d, int ns, const Fam &f, const int *pre_cells, i
         if (any_rec && !(fold && j + 1 < L))
             HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
         if (any_hdft)
-            HIP_TRY(mt::launch_hdft(d, ns, pre_cells, ncellblocks, j, st));
+            HIP_TRY(mt::launch_hstore(d, ns, pre_cells, ncellblocks, j, st));
     }
     if (any_dft)
         HIP_TRY(mt::launch_flush(d, ns, pre_cells, ncellblocks, L, st));
+    if (any_hdft)
+        HIP_TRY(mt::launch_flush(d, ns, pre_cells, ncellblocks, L, st, true));
     HIP_TRY(mt::launch_advance(d, ns, L, st));
     return FWI3D_OK;
 }
@@ -803,6 +806,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.w = s->d_w;
     d.acc = s->d_acc;
     d.acc_h = nullptr;
+    d.hring = nullptr;
     d.step = s->d_step;
     d.variant = m->variant;
     d.mu = m->d_mu;
@@ -906,6 +910,7 @@ int fwi3d_shot_destroy(fwi3d_shot *s)
     dfree(s->d_w);
     dfree(s->d_acc);
     dfree(s->d_acc_h);
+    dfree(s->d_hring);
     dfree(s->d_rec);
     dfree(s->d_step);
     dfree(s->d_pre);
@@ -1079,11 +1084,14 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
             if (!rc && s->nrec)
                 rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
             if (!rc && s->d_acc_h &&
-                mt::launch_hdft(s->d_desc, 1, s->d_pre, s->ncellblocks, j, s->st) != hipSuccess)
-                rc = fail(FWI3D_EDEVICE, "H DFT launch");
+                mt::launch_hstore(s->d_desc, 1, s->d_pre, s->ncellblocks, j, s->st) != hipSuccess)
+                rc = fail(FWI3D_EDEVICE, "H ring store launch");
         }
         if (!rc && s->nf)
             rc = timed(3, [&] { return mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st); });
+        if (!rc && s->d_acc_h &&
+            mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st, true) != hipSuccess)
+            rc = fail(FWI3D_EDEVICE, "H flush launch");
         if (!rc)
             rc = timed(4, [&] { return mt::launch_advance(s->d_desc, 1, L, s->st); });
         if (rc)
@@ -1223,10 +1231,17 @@ int fwi3d_enable_volume_dft_h(fwi3d_shot *s)
     if (s->d_acc_h)
         return FWI3D_OK;
     const size_t count = 2 * 3 * (size_t)s->nf * s->desc.astride;
+    const size_t ring = (size_t)s->desc.sstride * s->ring;
     if (dalloc(&s->d_acc_h, count) != hipSuccess)
         return fail(FWI3D_ENOMEM, "H volume accumulators (%zu doubles)", count);
+    if (dalloc(&s->d_hring, ring) != hipSuccess) {
+        dfree(s->d_acc_h);
+        s->d_acc_h = nullptr;
+        return fail(FWI3D_ENOMEM, "H ring (%zu doubles)", ring);
+    }
     HIP_TRY(mt::launch_fill(s->d_acc_h, count, 0.0, s->st));
     s->desc.acc_h = s->d_acc_h;
+    s->desc.hring = s->d_hring;
     s->desc_dirty = true;
     if (s->graph) {   // recaptured with the H DFT launches
         (void)hipGraphExecDestroy(s->graph);

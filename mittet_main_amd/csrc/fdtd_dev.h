@@ -77,6 +77,7 @@ struct ShotDev {
     const double *w;          // complex [it][nf]
     double *acc;              // complex [3][nf] rows of astride
     double *acc_h;            // H volume spectra, same layout (null: off)
+    double *hring;            // with acc_h: H after each step, ring slots like e
     int *step;                // absolute step of local index j = 0 of the current chunk
     int variant;              // 0: C v1.4 semantics; 1: Fortran ver0040 (fdtd_fortran.hip)
     int nsrc_h;               // variant 1: magnetic (Hz) source points; amp[q * (it - 1) + n]
@@ -153,12 +154,13 @@ hipError_t launch_hfield_v(const ShotDev *d, int ns, const int *pre, int nblocks
                            hipStream_t st);
 hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
                             hipStream_t st);
+// h: the H ring into acc_h (shots with H sums) instead of the E ring into acc
 hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int L,
-                        hipStream_t st);
+                        hipStream_t st, bool h = false);
 hipError_t launch_advance(const ShotDev *d, int ns, int L, hipStream_t st);
-// H volume running DFT of local step j (shots with acc_h), after the H update
-hipError_t launch_hdft(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
-                       hipStream_t st);
+// H after local step j's H update into its H ring slot (shots with hring)
+hipError_t launch_hstore(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
+                         hipStream_t st);
 hipError_t launch_receiver_dft(const double *traces, int ncomp_rows, int it, double dt,
                                double omega0, int nfreq, const double *omegas,
                                double *out, hipStream_t st);

@@ -427,12 +427,12 @@ __global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int
 constexpr int FLUSH_NF_MAX = 8;
 
 template <int NF>
-__device__ __forceinline__ void flush_cell(const ShotDev &S, int c, int Lv,
-                                           const double2 *__restrict__ wsh)
+__device__ __forceinline__ void flush_cell(const ShotDev &S, const double *src, double *acc, int c,
+                                           int Lv, const double2 *__restrict__ wsh)
 {
-    constexpr int G = MT_FLUSH_G;  // E samples loaded together: G loads in flight per lane
+    constexpr int G = MT_FLUSH_G;  // samples loaded together: G loads in flight per lane
     const int N = S.n, ring = S.ring, cur = S.cur;
-    GLOBAL double *__restrict__ A = gm(S.acc);
+    GLOBAL double *__restrict__ A = gm(acc);
     for (int d = 0; d < 3; ++d) {
         double2 a[NF];
 #pragma unroll
@@ -445,7 +445,7 @@ __device__ __forceinline__ void flush_cell(const ShotDev &S, int c, int Lv,
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const int slot = (cur + 1 + t0 + g) % ring;
-                e[g] = t0 + g < Lv ? gc(S.e)[(size_t)slot * S.sstride + (size_t)d * N + c] : 0.0;
+                e[g] = t0 + g < Lv ? gc(src)[(size_t)slot * S.sstride + (size_t)d * N + c] : 0.0;
             }
 #pragma unroll
             for (int g = 0; g < G; ++g) {
@@ -471,12 +471,12 @@ __device__ __forceinline__ void flush_cell(const ShotDev &S, int c, int Lv,
 // so every access is a 16-byte one): the same sums as flush_cell for each
 // cell, with one weight read from LDS per sample for both.
 template <int NF>
-__device__ __forceinline__ void flush_pair(const ShotDev &S, int c, int Lv,
-                                           const double2 *__restrict__ wsh)
+__device__ __forceinline__ void flush_pair(const ShotDev &S, const double *src, double *acc, int c,
+                                           int Lv, const double2 *__restrict__ wsh)
 {
     constexpr int G = MT_FLUSH_G / 2;
     const int N = S.n, ring = S.ring, cur = S.cur;
-    GLOBAL double *__restrict__ A = gm(S.acc);
+    GLOBAL double *__restrict__ A = gm(acc);
     for (int d = 0; d < 3; ++d) {
         double2 a[NF], b[NF];
 #pragma unroll
@@ -495,7 +495,7 @@ __device__ __forceinline__ void flush_pair(const ShotDev &S, int c, int Lv,
                 const int slot = (cur + 1 + t0 + g) % ring;
                 double2 v{0.0, 0.0};
                 if (t0 + g < Lv) {
-                    const GLOBAL double2 *p = (const GLOBAL double2 *)(gc(S.e) + (size_t)slot * S.sstride +
+                    const GLOBAL double2 *p = (const GLOBAL double2 *)(gc(src) + (size_t)slot * S.sstride +
                                                                        (size_t)d * N + c);
                     v.x = (*p).x;
                     v.y = (*p).y;
@@ -528,11 +528,12 @@ __device__ __forceinline__ void flush_pair(const ShotDev &S, int c, int Lv,
 }
 
 // any frequency count: groups of 4 accumulators, weights from global memory
-__device__ __forceinline__ void flush_cell_any(const ShotDev &S, int c, int n0, int Lv)
+__device__ __forceinline__ void flush_cell_any(const ShotDev &S, const double *src, double *acc,
+                                               int c, int n0, int Lv)
 {
     const int N = S.n, ring = S.ring, cur = S.cur, nf = S.nf;
     const GLOBAL double *__restrict__ W = gc(S.w) + 2 * (size_t)n0 * nf;
-    GLOBAL double *__restrict__ A = gm(S.acc);
+    GLOBAL double *__restrict__ A = gm(acc);
     for (int d = 0; d < 3; ++d)
         for (int f0 = 0; f0 < nf; f0 += 4) {
             const int nq = nf - f0 < 4 ? nf - f0 : 4;
@@ -544,7 +545,7 @@ __device__ __forceinline__ void flush_cell_any(const ShotDev &S, int c, int n0, 
             int slot = cur;
             for (int t = 0; t < Lv; ++t) {
                 slot = slot + 1 == ring ? 0 : slot + 1;
-                const double e = gc(S.e)[(size_t)slot * S.sstride + (size_t)d * N + c];
+                const double e = gc(src)[(size_t)slot * S.sstride + (size_t)d * N + c];
                 for (int q = 0; q < nq; ++q) {
                     const double wx = W[2 * ((size_t)t * nf + f0 + q)];
                     const double wy = W[2 * ((size_t)t * nf + f0 + q) + 1];
@@ -559,6 +560,10 @@ __device__ __forceinline__ void flush_cell_any(const ShotDev &S, int c, int n0, 
         }
 }
 
+// HS = 0: the E ring into the E accumulators; HS = 1: the H ring into the H
+// accumulators (shots without H sums return), one instance each, so the E
+// flush compiles as it does without the H sums.
+template <int HS>
 __global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__restrict__ D, int ns,
                                                                const int *__restrict__ pre, int L)
 {
@@ -567,19 +572,21 @@ __global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__
     const ShotDev &S = D[s];
     const int nf = S.nf;
     const int n0 = *S.step;
+    const double *src = HS ? S.hring : S.e;
+    double *acc = HS ? S.acc_h : S.acc;
     int Lv = L;
     if (n0 + Lv > S.it - 1)
         Lv = S.it - 1 - n0;
-    if (nf == 0 || Lv <= 0)
+    if (nf == 0 || Lv <= 0 || !src || !acc)
         return;  // uniform over the block
     // a block covers FLUSH_CELLS = 2 * blockDim.x cells
     const int base = (blockIdx.x - pre[s]) * FLUSH_CELLS;
     const int ca = base + threadIdx.x, cb = ca + blockDim.x;   // lane-contiguous halves
     if (nf > FLUSH_NF_MAX || (nf != 1 && nf != 2 && nf != 3 && nf != 4 && nf != 8)) {
         if (ca < S.n)
-            flush_cell_any(S, ca, n0, Lv);
+            flush_cell_any(S, src, acc, ca, n0, Lv);
         if (cb < S.n)
-            flush_cell_any(S, cb, n0, Lv);
+            flush_cell_any(S, src, acc, cb, n0, Lv);
         return;
     }
     const GLOBAL double *__restrict__ W = gc(S.w) + 2 * (size_t)n0 * nf;
@@ -593,11 +600,11 @@ __global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__
         if (c >= S.n)
             return;
         switch (nf) {
-        case 1: flush_pair<1>(S, c, Lv, wsh); break;
-        case 2: flush_pair<2>(S, c, Lv, wsh); break;
-        case 3: flush_pair<3>(S, c, Lv, wsh); break;
-        case 4: flush_pair<4>(S, c, Lv, wsh); break;
-        default: flush_pair<8>(S, c, Lv, wsh); break;
+        case 1: flush_pair<1>(S, src, acc, c, Lv, wsh); break;
+        case 2: flush_pair<2>(S, src, acc, c, Lv, wsh); break;
+        case 3: flush_pair<3>(S, src, acc, c, Lv, wsh); break;
+        case 4: flush_pair<4>(S, src, acc, c, Lv, wsh); break;
+        default: flush_pair<8>(S, src, acc, c, Lv, wsh); break;
         }
         return;
     }
@@ -605,50 +612,45 @@ __global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__
         if (c >= S.n)
             return;
         switch (nf) {
-        case 1: flush_cell<1>(S, c, Lv, wsh); break;
-        case 2: flush_cell<2>(S, c, Lv, wsh); break;
-        case 3: flush_cell<3>(S, c, Lv, wsh); break;
-        case 4: flush_cell<4>(S, c, Lv, wsh); break;
-        default: flush_cell<8>(S, c, Lv, wsh); break;
+        case 1: flush_cell<1>(S, src, acc, c, Lv, wsh); break;
+        case 2: flush_cell<2>(S, src, acc, c, Lv, wsh); break;
+        case 3: flush_cell<3>(S, src, acc, c, Lv, wsh); break;
+        case 4: flush_cell<4>(S, src, acc, c, Lv, wsh); break;
+        default: flush_cell<8>(S, src, acc, c, Lv, wsh); break;
         }
     }
 }
 
-// H volume running DFT, one step: acc_h += H[n] * w[n] for the step just
-// completed, after the H update (the sample the H receiver traces take),
-// accumulated per step in step order like the E sums.  H is updated in
-// place, so there is no ring to defer this to.
-__global__ __launch_bounds__(256) void k_hdft(const ShotDev *__restrict__ D, int ns,
-                                              const int *__restrict__ pre, int j)
+// H volume running DFT, staging: H after step j's H update (the sample the H
+// receiver traces take) into slot (cur + j + 1) % ring of the H ring, the slot
+// the E half-step of that step writes in the E ring.  k_flush (hs = 1) then
+// sums the chunk's slots with the E sums' weights, each accumulator in
+// ascending step order, so the sums equal per-step running sums bit for bit.
+__global__ __launch_bounds__(256) void k_hstore(const ShotDev *__restrict__ D, int ns,
+                                                const int *__restrict__ pre, int j)
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
     const int n = *S.step + j;
-    if (!S.acc_h || n >= S.it - 1)
+    if (!S.hring || n >= S.it - 1)
         return;
-    const int nf = S.nf;
     const size_t N = (size_t)S.n;
-    const GLOBAL double *__restrict__ W = gc(S.w) + 2 * (size_t)n * nf;
-    GLOBAL double *__restrict__ A = gm(S.acc_h);
+    GLOBAL double *__restrict__ dst = gm(S.hring + (size_t)((S.cur + j + 1) % S.ring) * S.sstride);
+    const GLOBAL double *__restrict__ H = gc(S.h);
     const int base = (blockIdx.x - pre[s]) * FLUSH_CELLS;
     for (int c = base + threadIdx.x; c < base + FLUSH_CELLS && c < S.n; c += blockDim.x) {
-        for (int d = 0; d < 3; ++d) {
-            const double h = gc(S.h)[(size_t)d * N + c];
-            for (int f = 0; f < nf; ++f) {
-                const size_t q = 2 * (((size_t)d * nf + f) * S.astride + c);
-                A[q] += h * W[2 * f];
-                A[q + 1] += h * W[2 * f + 1];
-            }
-        }
+        dst[c] = H[c];
+        dst[N + c] = H[N + c];
+        dst[2 * N + c] = H[2 * N + c];
     }
 }
 
-hipError_t launch_hdft(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
-                       hipStream_t st)
+hipError_t launch_hstore(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
+                         hipStream_t st)
 {
     if (nblocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_hdft, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, j);
+    hipLaunchKernelGGL(k_hstore, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, j);
     return hipGetLastError();
 }
 
@@ -1833,13 +1835,16 @@ hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inpl
 }
 
 hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int L,
-                        hipStream_t st)
+                        hipStream_t st, bool h)
 {
     if (nblocks <= 0 || L <= 0)
         return hipSuccess;
     if (L > RING_MAX)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_flush, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, L);
+    if (h)
+        hipLaunchKernelGGL(k_flush<1>, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, L);
+    else
+        hipLaunchKernelGGL(k_flush<0>, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, L);
     return hipGetLastError();
 }
 

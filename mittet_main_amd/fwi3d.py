@@ -315,7 +315,8 @@ class Shot:
 
     def enable_volume_dft_h(self) -> None:
         """Also accumulate H over the volume at the shot's volume-DFT
-        frequencies (sum_n H[n] w_n, H after step n); before the first step."""
+        frequencies (sum_n H[n] w_n, H after step n); before the first step.
+        Allocates an H ring of the shot's ring slots (ring * 24 B per cell)."""
         check(_lib.load().fwi3d_enable_volume_dft_h(self._h))
 
     def volume_dft_h(self) -> np.ndarray:
