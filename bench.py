@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--workload", default="roofline_256", choices=["roofline_256", "plain_64"])
     ap.add_argument("--ring", type=int, default=0, help="DFT ring slots (default 128; 64 for plain_64)")
     ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--volume-dft-h", action="store_true",
+                    help="forward: also sum H over the volume (H ring, fwi3d_enable_volume_dft_h)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps timed per kernel after the timed region (default: one ring)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -328,6 +330,8 @@ def main():
     model.media_coeff_3d(wl.sigma)
     shot = model.shot(wl.it, wl.signal, wl.src, wl.src_comp, wl.receivers, wl.omegas,
                       ring=args.ring, zchunk=args.zchunk)
+    if args.volume_dft_h:
+        shot.enable_volume_dft_h()
     shot.step_n(args.warmup)
     shot.sync()
     log(f"[bench] warmup {args.warmup} steps done")
@@ -366,7 +370,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "interior": g.ix - 2 * g.ncpml, "ncpml": g.ncpml,
                    "cells": g.ncells, "nt": wl.it, "nf_vol": nf, "nrec": len(wl.receivers),
-                   "ring": args.ring, "zchunk": args.zchunk, "sigma_sha256_16": configs.sigma_digest(wl.sigma)},
+                   "ring": args.ring, "zchunk": args.zchunk, "volume_dft_h": args.volume_dft_h,
+                   "sigma_sha256_16": configs.sigma_digest(wl.sigma)},
         "roofline": {"bound": "hbm",
                      "kernel": (HALF_STEP_KERNELS if g.ncells >= LDS_MIN_CELLS
                                 else HALF_STEP_KERNELS_PER_CELL).get(dom, dom),
