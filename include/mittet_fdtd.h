@@ -156,9 +156,11 @@ int fwi3d_step_n(fwi3d_shot *s, int nsteps);
 int fwi3d_run(fwi3d_shot *s);
 int fwi3d_shot_sync(fwi3d_shot *s);
 /* Runs nsteps fused steps with HIP events around every launch on the shot's
- * stream and returns the mean duration in ms of each kernel kind:
- * ms[0] E+CPML-E+source, ms[1] H+CPML-H, ms[2] receiver gather,
- * ms[3] volume-DFT flush (per flush launch), ms[4] whole step (per step).
+ * stream and returns the mean duration in ms of each kernel kind (ms holds 7
+ * doubles): ms[0] E+CPML-E+source, ms[1] H+CPML-H, ms[2] receiver gather,
+ * ms[3] volume-DFT flush (per flush launch), ms[4] whole step (per step, every
+ * launch included), ms[5] H ring store and ms[6] H-sum flush (shots with
+ * fwi3d_enable_volume_dft_h; else 0).
  * Advances the shot like fwi3d_step_n.  No counterpart in the reference
  * (its timing is a clock() around main, rec2rec.c:18,65). */
 int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms);
@@ -189,6 +191,11 @@ int fwi3d_get_volume_dft(fwi3d_shot *s, double *out);
  * the E slots.  Enable before the first step; not in a batch. */
 int fwi3d_enable_volume_dft_h(fwi3d_shot *s);
 int fwi3d_get_volume_dft_h(fwi3d_shot *s, double *out);
+/* Device bytes fwi3d_enable_volume_dft_h adds to a shot of these parameters,
+ * beyond fwi3d_shot_bytes: the H ring (ring*3*N doubles) and the H
+ * accumulators (3*nf_vol*N complex).  0 when nf_vol is 0 (H sums need a
+ * volume DFT); -1 on a null argument.  Admission control adds the two. */
+long long fwi3d_volume_dft_h_bytes(const fwi3d_grid_params *p, const fwi3d_shot_params *sp);
 
 /* ---- fictitious -> diffusive transform (L3) ------------------------------- */
 

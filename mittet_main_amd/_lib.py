@@ -53,6 +53,7 @@ _SIGS = {
     "fwi3d_shot_create": (C.c_int, [C.c_void_p, C.POINTER(ShotParams), C.POINTER(C.c_void_p)]),
     "fwi3d_shot_destroy": (C.c_int, [C.c_void_p]),
     "fwi3d_shot_bytes": (C.c_longlong, [C.POINTER(GridParams), C.POINTER(ShotParams)]),
+    "fwi3d_volume_dft_h_bytes": (C.c_longlong, [C.POINTER(GridParams), C.POINTER(ShotParams)]),
     "fwi3d_set_zero_eh": (C.c_int, [C.c_void_p]),
     "fwi3d_e_field_cpml42": (C.c_int, [C.c_void_p, C.c_int]),
     "fwi3d_epml_abcs4": (C.c_int, [C.c_void_p]),

@@ -101,6 +101,29 @@ double axis_step(const fwi3d_grid_params &p, int a) { return a == 0 ? p.dx : (a 
 // c = sigma*(b-1)/(sigma+kappa*alpha)/kappa.  E quantities sit on integer
 // offsets, H quantities half a cell inward; offsets are formed in single
 // precision as the reference does, so the outermost cell exceeds sigma_max.
+// A Fortran real power whose exponent is a whole-number constant: the
+// compiler folds x**3.0d0 into the integer power, evaluated by repeated
+// squaring (x**3 = x * (x*x)), which differs from pow(x, 3.0) in the last
+// place for about half of the ramp values.  Other exponents: pow.
+double fortran_pow(double x, double e)
+{
+    if (e != std::floor(e) || std::fabs(e) > 64.0)
+        return std::pow(x, e);
+    long long k = (long long)std::fabs(e);
+    double r = 1.0, sq = x;
+    bool any = false;
+    while (k) {
+        if (k & 1) {
+            r = any ? r * sq : sq;
+            any = true;
+        }
+        k >>= 1;
+        if (k)
+            sq = sq * sq;
+    }
+    return e < 0 ? 1.0 / r : r;
+}
+
 void cpml_axis(const fwi3d_grid_params &p, int a, std::vector<double> &out, int variant = 0)
 {
     const int n = axis_len(p, a), nc = p.ncpml;
@@ -140,12 +163,17 @@ void cpml_axis(const fwi3d_grid_params &p, int a, std::vector<double> &out, int 
             shell = false;
         }
         if (shell) {
-            const double es = sig_max * std::pow(oe / den, p.nn + p.order);
-            const double ms = sig_max * std::pow(om / den, p.nn + p.order);
-            const double ek = 1.0 + (p.kappa_max - 1.0) * std::pow(oe / den, p.nn);
-            const double mk = 1.0 + (p.kappa_max - 1.0) * std::pow(om / den, p.nn);
-            const double ea = p.alpha_max * std::pow(ae / den, p.ma);
-            const double ma = p.alpha_max * std::pow(am / den, p.ma);
+            // C: pow at run time (cpmlabc.c:35-40); Fortran: constant exponents
+            // (cpu_cpml4.f90:25-30, nn, order and ma are parameters)
+            auto pw = [&](double x, double e) {
+                return variant == 1 ? fortran_pow(x, e) : std::pow(x, e);
+            };
+            const double es = sig_max * pw(oe / den, p.nn + p.order);
+            const double ms = sig_max * pw(om / den, p.nn + p.order);
+            const double ek = 1.0 + (p.kappa_max - 1.0) * pw(oe / den, p.nn);
+            const double mk = 1.0 + (p.kappa_max - 1.0) * pw(om / den, p.nn);
+            const double ea = p.alpha_max * pw(ae / den, p.ma);
+            const double ma = p.alpha_max * pw(am / den, p.ma);
             const double be = std::exp(-(es / ek + ea) * p.dt);
             const double bh = std::exp(-(ms / mk + ma) * p.dt);
             at(FWI3D_P_ESIG, i) = es;
@@ -338,8 +366,11 @@ int ensure_amp(fwi3d_shot *s)
         std::vector<double> amp(4 * L, 0.0);
         for (size_t q = 0; q < s->src_h.size(); ++q) {
             const double mu = m->mu[(size_t)s->src_h[q]];
+            // each division of a complex (the Fortran signal) by a real taken
+            // as the reference's amdflang build does: (x c) / (c c)
+            auto cdivr = [](double x, double c) { return (x * c) / (c * c); };
             for (int n = 0; n + 1 < s->it; ++n)
-                amp[q * L + n] = s->signal[n] * p.dt / mu / p.dx / p.dy / p.dz;
+                amp[q * L + n] = cdivr(cdivr(cdivr(cdivr(s->signal[n] * p.dt, mu), p.dx), p.dy), p.dz);
         }
         HIP_TRY(hipMemcpyAsync(s->d_amp, amp.data(), sizeof(double) * amp.size(),
                                hipMemcpyHostToDevice, s->st));
@@ -381,13 +412,18 @@ struct Fam {
     int nblocks;
     const int *pre_l;
     int nblocks_l;
+    bool rings;   // every shot keeps E in a ring of more than one slot
 };
 
 // Families whose E half-step has a per-cell launch that can carry the
-// receiver gather of the previous step.
+// receiver gather of the previous step.  Only where every shot has a ring of
+// more than one slot: step j reads slot (cur + j) % ring and writes the next
+// one, so the gather of step j - 1 (slot (cur + j) % ring) is not written by
+// that launch.  With one slot E is updated in place, and the leading gather
+// blocks would race the tile blocks that overwrite it.
 bool rec_folds(const Fam &f)
 {
-    return (f.kind == 0 || f.kind == 3) && f.nblocks > 0;
+    return (f.kind == 0 || f.kind == 3) && f.nblocks > 0 && f.rings;
 }
 
 // rec_prev: the E launch of step j also gathers step j - 1's receivers
@@ -627,6 +663,17 @@ long long fwi3d_shot_bytes(const fwi3d_grid_params *p, const fwi3d_shot_params *
     b += 8LL * sp->it + 8LL * 6 * sp->nrec * (long long)sp->it + 4LL * sp->nrec;
     b += 16LL * sp->nf_vol * (long long)sp->it + 16LL * 3 * sp->nf_vol * N;  // weights, acc
     return b;
+}
+
+long long fwi3d_volume_dft_h_bytes(const fwi3d_grid_params *p, const fwi3d_shot_params *sp)
+{
+    if (!p || !sp)
+        return -1;
+    if (sp->nf_vol <= 0)
+        return 0;
+    const long long N = (long long)p->ix * p->iy * p->iz;
+    const long long ring = sp->ring > 0 ? sp->ring : kDefaultChunk;
+    return 8LL * 3 * N * ring + 16LL * 3 * sp->nf_vol * N;
 }
 
 int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **out)
@@ -1002,7 +1049,7 @@ int fwi3d_output(fwi3d_shot *s, int step) { return single_op(s, 5, 1, step); }
 
 static Fam shot_fam(const fwi3d_shot *s)
 {
-    return {s->kind, s->d_pre, s->nblocks, s->d_pre, s->nblocks_l};
+    return {s->kind, s->d_pre, s->nblocks, s->d_pre, s->nblocks_l, s->ring > 1};
 }
 
 int fwi3d_step_n(fwi3d_shot *s, int nsteps)
@@ -1054,8 +1101,8 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
     hipEvent_t ev[2];
     HIP_TRY(hipEventCreate(&ev[0]));
     HIP_TRY(hipEventCreate(&ev[1]));
-    double acc[5] = {0, 0, 0, 0, 0};
-    int cnt[5] = {0, 0, 0, 0, 0};
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
     auto timed = [&](int slot, auto &&launch) -> int {
         HIP_TRY(hipEventRecord(ev[0], s->st));
         HIP_TRY(launch());
@@ -1083,15 +1130,17 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
                 });
             if (!rc && s->nrec)
                 rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
-            if (!rc && s->d_acc_h &&
-                mt::launch_hstore(s->d_desc, 1, s->d_pre, s->ncellblocks, j, s->st) != hipSuccess)
-                rc = fail(FWI3D_EDEVICE, "H ring store launch");
+            if (!rc && s->d_acc_h)
+                rc = timed(5, [&] {
+                    return mt::launch_hstore(s->d_desc, 1, s->d_pre, s->ncellblocks, j, s->st);
+                });
         }
         if (!rc && s->nf)
             rc = timed(3, [&] { return mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st); });
-        if (!rc && s->d_acc_h &&
-            mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st, true) != hipSuccess)
-            rc = fail(FWI3D_EDEVICE, "H flush launch");
+        if (!rc && s->d_acc_h)
+            rc = timed(6, [&] {
+                return mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st, true);
+            });
         if (!rc)
             rc = timed(4, [&] { return mt::launch_advance(s->d_desc, 1, L, s->st); });
         if (rc)
@@ -1107,9 +1156,12 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
     (void)hipEventDestroy(ev[1]);
     if (rc)
         return rc;
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 7; ++q)
         ms[q] = cnt[q] ? acc[q] / cnt[q] : 0.0;
-    ms[4] = steps_run ? (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / steps_run : 0.0;
+    double tot = 0.0;
+    for (int q = 0; q < 7; ++q)
+        tot += acc[q];
+    ms[4] = steps_run ? tot / steps_run : 0.0;
     return FWI3D_OK;
 }
 
@@ -2547,7 +2599,10 @@ static int batch_upload(fwi3d_batch *b)
 
 static Fam batch_fam(const fwi3d_batch *b)
 {
-    return {b->kind, b->d_pre, b->nblocks, b->d_pre_l, b->nblocks_l};
+    bool rings = true;
+    for (const auto *s : b->shots)
+        rings = rings && s->ring > 1;
+    return {b->kind, b->d_pre, b->nblocks, b->d_pre_l, b->nblocks_l, rings};
 }
 
 int fwi3d_batch_step(fwi3d_batch *b, int nsteps)

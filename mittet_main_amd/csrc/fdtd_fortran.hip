@@ -35,6 +35,14 @@ __device__ __forceinline__ int shot_of(const int *__restrict__ pre, int ns, int 
 
 __device__ __forceinline__ int slab_at(int r, int n, int nc) { return r < nc ? r : r - (n - 2 * nc); }
 
+// x / c of a complex x by a real c, as the reference's Fortran evaluates it
+// compiled with amdflang: c becomes the complex (c, 0) and each part of the
+// quotient is (x c) / (c c) (the textbook complex quotient with zero
+// imaginary divisor), which differs from x / c in the last place.  Applies to
+// every stencil division, psi update and the source scaling; the real and
+// imaginary runs of a complex field take it part by part.
+__device__ __forceinline__ double cdivr(double x, double c) { return (x * c) / (c * c); }
+
 // c1 F[c] - c1 F[c-s] + c2 F[c+s] - c2 F[c-2s]
 __device__ __forceinline__ double dm(const GLOBAL double *F, size_t c, size_t s, double c1, double c2)
 {
@@ -119,18 +127,18 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_v(const ShotDev *__restrict__
     // e_field_cpml4 (:125-155)
     if (k >= 2 && k < iz - 1 && jj >= 2 && jj < iy - 1) {
         const double ca = (1.0 - ((esx * dt) / (2.0 * eps2))) / (1.0 + ((esx * dt) / (2.0 * eps2)));
-        ex = ca * ex + cbx * (dm(Hz, c, sy, c1, c2) / py[PR_KED * iy + jj] -
-                              dm(Hy, c, sz, c1, c2) / pz[PR_KED * iz + k]);
+        ex = ca * ex + cbx * (cdivr(dm(Hz, c, sy, c1, c2), py[PR_KED * iy + jj]) -
+                              cdivr(dm(Hy, c, sz, c1, c2), pz[PR_KED * iz + k]));
     }
     if (k >= 2 && k < iz - 1 && i >= 2 && i < ix - 1) {
         const double ca = (1.0 - ((esy * dt) / (2.0 * eps2))) / (1.0 + ((esy * dt) / (2.0 * eps2)));
-        ey = ca * ey + cby * (dm(Hx, c, sz, c1, c2) / pz[PR_KED * iz + k] -
-                              dm(Hz, c, 1, c1, c2) / px[PR_KED * ix + i]);
+        ey = ca * ey + cby * (cdivr(dm(Hx, c, sz, c1, c2), pz[PR_KED * iz + k]) -
+                              cdivr(dm(Hz, c, 1, c1, c2), px[PR_KED * ix + i]));
     }
     if (jj >= 2 && jj < iy - 1 && i >= 2 && i < ix - 1) {
         const double ca = (1.0 - ((esz * dt) / (2.0 * eps2))) / (1.0 + ((esz * dt) / (2.0 * eps2)));
-        ez = ca * ez + cbz * (dm(Hy, c, 1, c1, c2) / px[PR_KED * ix + i] -
-                              dm(Hx, c, sy, c1, c2) / py[PR_KED * iy + jj]);
+        ez = ca * ez + cbz * (cdivr(dm(Hy, c, 1, c1, c2), px[PR_KED * ix + i]) -
+                              cdivr(dm(Hx, c, sy, c1, c2), py[PR_KED * iy + jj]));
     }
     // CPML_E4 (cpu_cpml4.f90:281-358): x, then y, then z
     if (e_shell(i, ix, nc)) {
@@ -138,8 +146,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_v(const ShotDev *__restrict__
         const size_t L = (size_t)S.slab[0];
         const size_t q = ((size_t)k * iy + jj) * (2 * nc) + slab_at(i, ix, nc);
         const double be = px[PR_BE * ix + i], ce = px[PR_CE * ix + i], h = S.dl[0];
-        const double p0 = be * P[q] + ce * dm(Hy, c, 1, c1, c2) / h;      // psi_Ezx
-        const double p1 = be * P[L + q] + ce * dm(Hz, c, 1, c1, c2) / h;  // psi_Eyx
+        const double p0 = be * P[q] + cdivr(ce * dm(Hy, c, 1, c1, c2), h);      // psi_Ezx
+        const double p1 = be * P[L + q] + cdivr(ce * dm(Hz, c, 1, c1, c2), h);  // psi_Eyx
         P[q] = p0;
         P[L + q] = p1;
         ez = ez + cbz * p0;
@@ -150,8 +158,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_v(const ShotDev *__restrict__
         const size_t L = (size_t)S.slab[1];
         const size_t q = ((size_t)k * (2 * nc) + slab_at(jj, iy, nc)) * ix + i;
         const double be = py[PR_BE * iy + jj], ce = py[PR_CE * iy + jj], h = S.dl[1];
-        const double p0 = be * P[q] + ce * dm(Hz, c, sy, c1, c2) / h;      // psi_Exy
-        const double p1 = be * P[L + q] + ce * dm(Hx, c, sy, c1, c2) / h;  // psi_Ezy
+        const double p0 = be * P[q] + cdivr(ce * dm(Hz, c, sy, c1, c2), h);      // psi_Exy
+        const double p1 = be * P[L + q] + cdivr(ce * dm(Hx, c, sy, c1, c2), h);  // psi_Ezy
         P[q] = p0;
         P[L + q] = p1;
         ex = ex + cbx * p0;
@@ -162,8 +170,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_v(const ShotDev *__restrict__
         const size_t L = (size_t)S.slab[2];
         const size_t q = ((size_t)slab_at(k, iz, nc) * iy + jj) * ix + i;
         const double be = pz[PR_BE * iz + k], ce = pz[PR_CE * iz + k], h = S.dl[2];
-        const double p0 = be * P[q] + ce * dm(Hx, c, sz, c1, c2) / h;      // psi_Eyz
-        const double p1 = be * P[L + q] + ce * dm(Hy, c, sz, c1, c2) / h;  // psi_Exz
+        const double p0 = be * P[q] + cdivr(ce * dm(Hx, c, sz, c1, c2), h);      // psi_Eyz
+        const double p1 = be * P[L + q] + cdivr(ce * dm(Hy, c, sz, c1, c2), h);  // psi_Exz
         P[q] = p0;
         P[L + q] = p1;
         ey = ey + cby * p0;
@@ -202,18 +210,18 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_v(const ShotDev *__restrict__
     // h_field_cpml4 (:221-251)
     if (k >= 1 && k < iz - 2 && jj >= 1 && jj < iy - 2) {
         const double da = (1.0 - ((msx * dt) / (2.0 * eps2))) / (1.0 + ((msx * dt) / (2.0 * eps2)));
-        hx = da * hx - dbx * (dp(Ez, c, sy, c1, c2) / py[PR_KHD * iy + jj] -
-                              dp(Ey, c, sz, c1, c2) / pz[PR_KHD * iz + k]);
+        hx = da * hx - dbx * (cdivr(dp(Ez, c, sy, c1, c2), py[PR_KHD * iy + jj]) -
+                              cdivr(dp(Ey, c, sz, c1, c2), pz[PR_KHD * iz + k]));
     }
     if (k >= 1 && k < iz - 2 && i >= 1 && i < ix - 2) {
         const double da = (1.0 - ((msy * dt) / (2.0 * eps2))) / (1.0 + ((msy * dt) / (2.0 * eps2)));
-        hy = da * hy - dby * (dp(Ex, c, sz, c1, c2) / pz[PR_KHD * iz + k] -
-                              dp(Ez, c, 1, c1, c2) / px[PR_KHD * ix + i]);
+        hy = da * hy - dby * (cdivr(dp(Ex, c, sz, c1, c2), pz[PR_KHD * iz + k]) -
+                              cdivr(dp(Ez, c, 1, c1, c2), px[PR_KHD * ix + i]));
     }
     if (jj >= 1 && jj < iy - 2 && i >= 1 && i < ix - 2) {
         const double da = (1.0 - ((msz * dt) / (2.0 * eps2))) / (1.0 + ((msz * dt) / (2.0 * eps2)));
-        hz = da * hz - dbz * (dp(Ey, c, 1, c1, c2) / px[PR_KHD * ix + i] -
-                              dp(Ex, c, sy, c1, c2) / py[PR_KHD * iy + jj]);
+        hz = da * hz - dbz * (cdivr(dp(Ey, c, 1, c1, c2), px[PR_KHD * ix + i]) -
+                              cdivr(dp(Ex, c, sy, c1, c2), py[PR_KHD * iy + jj]));
     }
     // CPML_H4 (cpu_cpml4.f90:394-471): x, then y, then z
     if (h_shell(i, ix, nc)) {
@@ -223,8 +231,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_v(const ShotDev *__restrict__
         const double bh = px[PR_BH * ix + i], ch = px[PR_CH * ix + i], h = S.dl[0];
         // psi_Hyx takes c2 where the stencil has c1 (cpu_cpml4.f90:400, 413)
         const double hyx = c1 * Ez[c + 1] - c2 * Ez[c] + c2 * Ez[c + 2] - c2 * Ez[c - 1];
-        const double p0 = bh * P[q] + ch * dp(Ey, c, 1, c1, c2) / h;  // psi_Hzx
-        const double p1 = bh * P[L + q] + ch * hyx / h;                // psi_Hyx
+        const double p0 = bh * P[q] + cdivr(ch * dp(Ey, c, 1, c1, c2), h);  // psi_Hzx
+        const double p1 = bh * P[L + q] + cdivr(ch * hyx, h);                // psi_Hyx
         P[q] = p0;
         P[L + q] = p1;
         hz = hz - dbz * p0;
@@ -235,8 +243,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_v(const ShotDev *__restrict__
         const size_t L = (size_t)S.slab[1];
         const size_t q = ((size_t)k * (2 * nc) + slab_at(jj, iy, nc)) * ix + i;
         const double bh = py[PR_BH * iy + jj], ch = py[PR_CH * iy + jj], h = S.dl[1];
-        const double p0 = bh * P[q] + ch * dp(Ez, c, sy, c1, c2) / h;      // psi_Hxy
-        const double p1 = bh * P[L + q] + ch * dp(Ex, c, sy, c1, c2) / h;  // psi_Hzy
+        const double p0 = bh * P[q] + cdivr(ch * dp(Ez, c, sy, c1, c2), h);      // psi_Hxy
+        const double p1 = bh * P[L + q] + cdivr(ch * dp(Ex, c, sy, c1, c2), h);  // psi_Hzy
         P[q] = p0;
         P[L + q] = p1;
         hx = hx - dbx * p0;
@@ -247,8 +255,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_v(const ShotDev *__restrict__
         const size_t L = (size_t)S.slab[2];
         const size_t q = ((size_t)slab_at(k, iz, nc) * iy + jj) * ix + i;
         const double bh = pz[PR_BH * iz + k], ch = pz[PR_CH * iz + k], h = S.dl[2];
-        const double p0 = bh * P[q] + ch * dp(Ex, c, sz, c1, c2) / h;      // psi_Hyz
-        const double p1 = bh * P[L + q] + ch * dp(Ey, c, sz, c1, c2) / h;  // psi_Hxz
+        const double p0 = bh * P[q] + cdivr(ch * dp(Ex, c, sz, c1, c2), h);      // psi_Hyz
+        const double p1 = bh * P[L + q] + cdivr(ch * dp(Ey, c, sz, c1, c2), h);  // psi_Hxz
         P[q] = p0;
         P[L + q] = p1;
         hy = hy - dby * p0;

@@ -211,11 +211,18 @@ def shot_params(it, signal, src, src_comp, receivers, omega_vol, ring, keep, zch
                       om.size, om.ctypes.data_as(C.POINTER(C.c_double)), int(ring), int(zchunk))
 
 
-def shot_bytes(grid: Grid, it: int, nrec: int = 0, nf_vol: int = 0, ring: int = 0) -> int:
+def shot_bytes(grid: Grid, it: int, nrec: int = 0, nf_vol: int = 0, ring: int = 0,
+               volume_dft_h: bool = False) -> int:
+    """Device bytes of one shot (fwi3d_shot_bytes), plus the H ring and H
+    accumulators of fwi3d_enable_volume_dft_h when volume_dft_h is set."""
     keep = []
     sp = shot_params(it, None, (0, 0, 0), 0, np.zeros((nrec, 3)), np.zeros(nf_vol), ring, keep)
     gp = grid.to_c()
-    return int(_lib.load().fwi3d_shot_bytes(C.byref(gp), C.byref(sp)))
+    L = _lib.load()
+    b = int(L.fwi3d_shot_bytes(C.byref(gp), C.byref(sp)))
+    if volume_dft_h:
+        b += int(L.fwi3d_volume_dft_h_bytes(C.byref(gp), C.byref(sp)))
+    return b
 
 
 class Shot:
@@ -282,9 +289,10 @@ class Shot:
 
     def profile_steps(self, nsteps: int) -> dict:
         """Mean kernel durations (ms) over nsteps event-bracketed fused steps."""
-        ms = np.zeros(5)
+        ms = np.zeros(7)
         check(_lib.load().fwi3d_profile_steps(self._h, nsteps, _dp(ms)))
-        return dict(zip(("efield", "hfield", "receivers", "flush", "step"), ms.tolist()))
+        return dict(zip(("efield", "hfield", "receivers", "flush", "step", "hstore", "hflush"),
+                        ms.tolist()))
 
     @property
     def step(self) -> int:

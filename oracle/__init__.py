@@ -108,6 +108,7 @@ def lib():
                    "orf_cpml_h4"):
             getattr(L, nm).argtypes = [vp]
         L.orf_media_coeff.argtypes = [vp, dp]
+        L.orf_waveform.argtypes = [i, d, d, d, dp]
         L.orf_source_hz.argtypes = [vp, i, ip, dp, d]
         L.orf_forward.argtypes = [vp, i, ip, dp, dp, i, i, ip, i, dp]
         L.or_laplace_volume.argtypes = [i, i, i, dp, dp, dp, dp]
@@ -136,6 +137,13 @@ def lattice_time_3d(minval, omega0, dx, dy, dz, factor=DT_FACTOR_REF, mu0=MU0_C)
 def firstderiv_gauss(nt, dt, fmax_w):
     out = np.zeros(nt)
     lib().or_firstderiv_gauss(nt, dt, fmax_w, _dp(out))
+    return out
+
+
+def fortran_waveform(nstep, dt, fmax, pi=3.14159265358979):
+    """s(istep), istep = 1..nstep, of ver0040/cpu_waveform.f90:20-26 (Fortran pi)."""
+    out = np.zeros(nstep)
+    lib().orf_waveform(nstep, dt, fmax, pi, _dp(out))
     return out
 
 

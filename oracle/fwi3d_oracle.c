@@ -736,11 +736,49 @@ void or_update_para2(int ix, int iy, int iz, int nc, int iseabed, const double *
 /* ===================================================================== */
 /* Fortran lineage (ver0040), restated operator by operator over the full */
 /* volume in the Fortran loop order; 1-based Fortran ranges written as    */
-/* 0-based half-open ones.  Parity of this part is pinned by mathematics  */
-/* only (no Fortran compiler in this image): see tests/test_oracle_fortran.py. */
+/* 0-based half-open ones.  Pinned against the reference's own ver0040   */
+/* subroutines compiled with amdflang (tests/test_oracle_fortran_ref.py)  */
+/* and by mathematics (tests/test_oracle_fortran.py).                     */
 /* ===================================================================== */
 
 #define FIDX(i, j, k) (((long)(k) * iy + (j)) * ix + (i))
+
+/* x / c for a complex x and a real c, as the reference's Fortran evaluates it
+ * when compiled with amdflang (checked against that build in
+ * tests/test_oracle_fortran_ref.py): c becomes the complex (c, 0) and the
+ * quotient takes the textbook formula ((xr c + xi 0) / (c c + 0 0),
+ * (xi c - xr 0) / (c c + 0 0)), i.e. (x c) / (c c) for each part of a finite
+ * x.  That differs from x / c in the last place.  Every complex-by-real
+ * division of the ver0040 stencils, psi updates and source takes it; the
+ * oracle's real and imaginary runs apply it part by part. */
+static inline double orf_cdivr(double x, double c)
+{
+    return (x * c) / (c * c);
+}
+
+/* x**e of the Fortran source with e a whole-number compile-time constant
+ * (cpu_cpml4.f90:25-30: nn, order and ma are parameters): the compiler folds
+ * it into the integer power, multiplied out by repeated squaring, as the
+ * reference compiled with amdflang shows (tests/golden/ref_fortran_step.npz);
+ * pow() differs from it in the last place. */
+static double orf_pow(double x, double e)
+{
+    if (e != floor(e) || fabs(e) > 64.0)
+        return pow(x, e);
+    long k = (long)fabs(e);
+    double r = 1.0, sq = x;
+    int any = 0;
+    while (k) {
+        if (k & 1) {
+            r = any ? r * sq : sq;
+            any = 1;
+        }
+        k >>= 1;
+        if (k)
+            sq = sq * sq;
+    }
+    return e < 0 ? 1.0 / r : r;
+}
 
 /* ver0040/cpu_cpml4.f90:8-248: ramps on the 1-based index i = 1..ncpml,
  * sigma_max without a per-axis factor, high side = mirror of the low side */
@@ -763,12 +801,12 @@ void orf_init_cpml(or_state *s)
         for (int i = 1; i <= nc; ++i) {
             const int r = i - 1;
             const double den = (double)nc - 1.0;
-            q[OR_P_ESIG][r] = smax * pow(((double)nc - (double)i) / den, p->nn + p->order);
-            q[OR_P_MSIG][r] = smax * pow(((double)nc - (double)i - 0.5) / den, p->nn + p->order);
-            q[OR_P_EKAP][r] = 1.0 + (p->kappa_max - 1.0) * pow(((double)nc - (double)i) / den, p->nn);
-            q[OR_P_MKAP][r] = 1.0 + (p->kappa_max - 1.0) * pow(((double)nc - (double)i - 0.5) / den, p->nn);
-            q[OR_P_AE][r] = p->alpha_max * pow(((double)i) / den, p->ma);
-            q[OR_P_AM][r] = p->alpha_max * pow(((double)i + 0.5) / den, p->ma);
+            q[OR_P_ESIG][r] = smax * orf_pow(((double)nc - (double)i) / den, p->nn + p->order);
+            q[OR_P_MSIG][r] = smax * orf_pow(((double)nc - (double)i - 0.5) / den, p->nn + p->order);
+            q[OR_P_EKAP][r] = 1.0 + (p->kappa_max - 1.0) * orf_pow(((double)nc - (double)i) / den, p->nn);
+            q[OR_P_MKAP][r] = 1.0 + (p->kappa_max - 1.0) * orf_pow(((double)nc - (double)i - 0.5) / den, p->nn);
+            q[OR_P_AE][r] = p->alpha_max * orf_pow(((double)i) / den, p->ma);
+            q[OR_P_AM][r] = p->alpha_max * orf_pow(((double)i + 0.5) / den, p->ma);
             q[OR_P_BE][r] = exp(-(q[OR_P_ESIG][r] / q[OR_P_EKAP][r] + q[OR_P_AE][r]) * p->dt);
             q[OR_P_BH][r] = exp(-(q[OR_P_MSIG][r] / q[OR_P_MKAP][r] + q[OR_P_AM][r]) * p->dt);
             q[OR_P_CE][r] = q[OR_P_ESIG][r] * (q[OR_P_BE][r] - 1.0) /
@@ -809,13 +847,26 @@ void orf_media_coeff(or_state *s, const double *mu)
             }
 }
 
+/* ver0040/cpu_waveform.f90:20-26: out[n] = s(istep) with istep = n + 1,
+ * s = -(2 beta (istep dt - t0) sqrt(beta/pi)) exp(-beta (istep dt - t0)**2),
+ * t0 = pi/fmax, beta = pi fmax**2, pi the module's 3.14159265358979d0 */
+void orf_waveform(int nstep, double dt, double fmax, double pi, double *out)
+{
+    const double t0 = pi / fmax;
+    const double beta = pi * (fmax * fmax);
+    for (int n = 0; n < nstep; ++n) {
+        const double u = (double)(n + 1) * dt - t0;
+        out[n] = -(2.0 * beta * u * sqrt(beta / pi)) * exp(-beta * (u * u));
+    }
+}
+
 /* ver0040/cpu_waveform.f90:46-51: Hz(src) -= signal*dt/myu(src)/dx/dy/dz, each point in turn */
 void orf_source_hz(or_state *s, int npts, const int *cells, const double *mu, double sval)
 {
     const or_params *p = &s->p;
     for (int q = 0; q < npts; ++q) {
         const long c = cells[q];
-        s->vol[OR_HZ][c] = s->vol[OR_HZ][c] - sval * p->dt / mu[c] / p->dx / p->dy / p->dz;
+        s->vol[OR_HZ][c] = s->vol[OR_HZ][c] - orf_cdivr(orf_cdivr(orf_cdivr(orf_cdivr(sval * p->dt, mu[c]), p->dx), p->dy), p->dz);
     }
 }
 
@@ -834,24 +885,24 @@ void orf_e_field_cpml4(or_state *s)
             for (int i = 0; i < ix; ++i) {
                 const long c = FIDX(i, j, k);
                 Ex[c] = s->vol[OR_CA_X][c] * Ex[c] +
-                        s->vol[OR_CB_X][c] * ((c1 * Hz[c] - c1 * Hz[c - sy] + c2 * Hz[c + sy] - c2 * Hz[c - 2 * sy]) / kedy[j] -
-                                              (c1 * Hy[c] - c1 * Hy[c - sz] + c2 * Hy[c + sz] - c2 * Hy[c - 2 * sz]) / kedz[k]);
+                        s->vol[OR_CB_X][c] * (orf_cdivr((c1 * Hz[c] - c1 * Hz[c - sy] + c2 * Hz[c + sy] - c2 * Hz[c - 2 * sy]), kedy[j]) -
+                                              orf_cdivr((c1 * Hy[c] - c1 * Hy[c - sz] + c2 * Hy[c + sz] - c2 * Hy[c - 2 * sz]), kedz[k]));
             }
     for (int k = 2; k < iz - 1; ++k)
         for (int j = 0; j < iy; ++j)
             for (int i = 2; i < ix - 1; ++i) {
                 const long c = FIDX(i, j, k);
                 Ey[c] = s->vol[OR_CA_Y][c] * Ey[c] +
-                        s->vol[OR_CB_Y][c] * ((c1 * Hx[c] - c1 * Hx[c - sz] + c2 * Hx[c + sz] - c2 * Hx[c - 2 * sz]) / kedz[k] -
-                                              (c1 * Hz[c] - c1 * Hz[c - 1] + c2 * Hz[c + 1] - c2 * Hz[c - 2]) / kedx[i]);
+                        s->vol[OR_CB_Y][c] * (orf_cdivr((c1 * Hx[c] - c1 * Hx[c - sz] + c2 * Hx[c + sz] - c2 * Hx[c - 2 * sz]), kedz[k]) -
+                                              orf_cdivr((c1 * Hz[c] - c1 * Hz[c - 1] + c2 * Hz[c + 1] - c2 * Hz[c - 2]), kedx[i]));
             }
     for (int k = 0; k < iz; ++k)
         for (int j = 2; j < iy - 1; ++j)
             for (int i = 2; i < ix - 1; ++i) {
                 const long c = FIDX(i, j, k);
                 Ez[c] = s->vol[OR_CA_Z][c] * Ez[c] +
-                        s->vol[OR_CB_Z][c] * ((c1 * Hy[c] - c1 * Hy[c - 1] + c2 * Hy[c + 1] - c2 * Hy[c - 2]) / kedx[i] -
-                                              (c1 * Hx[c] - c1 * Hx[c - sy] + c2 * Hx[c + sy] - c2 * Hx[c - 2 * sy]) / kedy[j]);
+                        s->vol[OR_CB_Z][c] * (orf_cdivr((c1 * Hy[c] - c1 * Hy[c - 1] + c2 * Hy[c + 1] - c2 * Hy[c - 2]), kedx[i]) -
+                                              orf_cdivr((c1 * Hx[c] - c1 * Hx[c - sy] + c2 * Hx[c + sy] - c2 * Hx[c - 2 * sy]), kedy[j]));
             }
 }
 
@@ -876,8 +927,8 @@ void orf_cpml_e4(or_state *s)
             for (int j = 0; j < iy; ++j)
                 for (int i = xr[side][0]; i < xr[side][1]; ++i) {
                     const long c = FIDX(i, j, k);
-                    pEzx[c] = bex[i] * pEzx[c] + cex[i] * (c1 * Hy[c] - c1 * Hy[c - 1] + c2 * Hy[c + 1] - c2 * Hy[c - 2]) / p->dx;
-                    pEyx[c] = bex[i] * pEyx[c] + cex[i] * (c1 * Hz[c] - c1 * Hz[c - 1] + c2 * Hz[c + 1] - c2 * Hz[c - 2]) / p->dx;
+                    pEzx[c] = bex[i] * pEzx[c] + orf_cdivr(cex[i] * (c1 * Hy[c] - c1 * Hy[c - 1] + c2 * Hy[c + 1] - c2 * Hy[c - 2]), p->dx);
+                    pEyx[c] = bex[i] * pEyx[c] + orf_cdivr(cex[i] * (c1 * Hz[c] - c1 * Hz[c - 1] + c2 * Hz[c + 1] - c2 * Hz[c - 2]), p->dx);
                     Ez[c] = Ez[c] + s->vol[OR_CB_Z][c] * pEzx[c];
                     Ey[c] = Ey[c] - s->vol[OR_CB_Y][c] * pEyx[c];
                 }
@@ -887,8 +938,8 @@ void orf_cpml_e4(or_state *s)
             for (int j = yr[side][0]; j < yr[side][1]; ++j)
                 for (int i = 0; i < ix; ++i) {
                     const long c = FIDX(i, j, k);
-                    pExy[c] = bey[j] * pExy[c] + cey[j] * (c1 * Hz[c] - c1 * Hz[c - sy] + c2 * Hz[c + sy] - c2 * Hz[c - 2 * sy]) / p->dy;
-                    pEzy[c] = bey[j] * pEzy[c] + cey[j] * (c1 * Hx[c] - c1 * Hx[c - sy] + c2 * Hx[c + sy] - c2 * Hx[c - 2 * sy]) / p->dy;
+                    pExy[c] = bey[j] * pExy[c] + orf_cdivr(cey[j] * (c1 * Hz[c] - c1 * Hz[c - sy] + c2 * Hz[c + sy] - c2 * Hz[c - 2 * sy]), p->dy);
+                    pEzy[c] = bey[j] * pEzy[c] + orf_cdivr(cey[j] * (c1 * Hx[c] - c1 * Hx[c - sy] + c2 * Hx[c + sy] - c2 * Hx[c - 2 * sy]), p->dy);
                     Ex[c] = Ex[c] + s->vol[OR_CB_X][c] * pExy[c];
                     Ez[c] = Ez[c] - s->vol[OR_CB_Z][c] * pEzy[c];
                 }
@@ -898,8 +949,8 @@ void orf_cpml_e4(or_state *s)
             for (int j = 0; j < iy; ++j)
                 for (int i = 0; i < ix; ++i) {
                     const long c = FIDX(i, j, k);
-                    pEyz[c] = bez[k] * pEyz[c] + cez[k] * (c1 * Hx[c] - c1 * Hx[c - sz] + c2 * Hx[c + sz] - c2 * Hx[c - 2 * sz]) / p->dz;
-                    pExz[c] = bez[k] * pExz[c] + cez[k] * (c1 * Hy[c] - c1 * Hy[c - sz] + c2 * Hy[c + sz] - c2 * Hy[c - 2 * sz]) / p->dz;
+                    pEyz[c] = bez[k] * pEyz[c] + orf_cdivr(cez[k] * (c1 * Hx[c] - c1 * Hx[c - sz] + c2 * Hx[c + sz] - c2 * Hx[c - 2 * sz]), p->dz);
+                    pExz[c] = bez[k] * pExz[c] + orf_cdivr(cez[k] * (c1 * Hy[c] - c1 * Hy[c - sz] + c2 * Hy[c + sz] - c2 * Hy[c - 2 * sz]), p->dz);
                     Ey[c] = Ey[c] + s->vol[OR_CB_Y][c] * pEyz[c];
                     Ex[c] = Ex[c] - s->vol[OR_CB_X][c] * pExz[c];
                 }
@@ -920,24 +971,24 @@ void orf_h_field_cpml4(or_state *s)
             for (int i = 0; i < ix; ++i) {
                 const long c = FIDX(i, j, k);
                 Hx[c] = s->vol[OR_DA_X][c] * Hx[c] -
-                        s->vol[OR_DB_X][c] * ((c1 * Ez[c + sy] - c1 * Ez[c] + c2 * Ez[c + 2 * sy] - c2 * Ez[c - sy]) / khdy[j] -
-                                              (c1 * Ey[c + sz] - c1 * Ey[c] + c2 * Ey[c + 2 * sz] - c2 * Ey[c - sz]) / khdz[k]);
+                        s->vol[OR_DB_X][c] * (orf_cdivr((c1 * Ez[c + sy] - c1 * Ez[c] + c2 * Ez[c + 2 * sy] - c2 * Ez[c - sy]), khdy[j]) -
+                                              orf_cdivr((c1 * Ey[c + sz] - c1 * Ey[c] + c2 * Ey[c + 2 * sz] - c2 * Ey[c - sz]), khdz[k]));
             }
     for (int k = 1; k < iz - 2; ++k)
         for (int j = 0; j < iy; ++j)
             for (int i = 1; i < ix - 2; ++i) {
                 const long c = FIDX(i, j, k);
                 Hy[c] = s->vol[OR_DA_Y][c] * Hy[c] -
-                        s->vol[OR_DB_Y][c] * ((c1 * Ex[c + sz] - c1 * Ex[c] + c2 * Ex[c + 2 * sz] - c2 * Ex[c - sz]) / khdz[k] -
-                                              (c1 * Ez[c + 1] - c1 * Ez[c] + c2 * Ez[c + 2] - c2 * Ez[c - 1]) / khdx[i]);
+                        s->vol[OR_DB_Y][c] * (orf_cdivr((c1 * Ex[c + sz] - c1 * Ex[c] + c2 * Ex[c + 2 * sz] - c2 * Ex[c - sz]), khdz[k]) -
+                                              orf_cdivr((c1 * Ez[c + 1] - c1 * Ez[c] + c2 * Ez[c + 2] - c2 * Ez[c - 1]), khdx[i]));
             }
     for (int k = 0; k < iz; ++k)
         for (int j = 1; j < iy - 2; ++j)
             for (int i = 1; i < ix - 2; ++i) {
                 const long c = FIDX(i, j, k);
                 Hz[c] = s->vol[OR_DA_Z][c] * Hz[c] -
-                        s->vol[OR_DB_Z][c] * ((c1 * Ey[c + 1] - c1 * Ey[c] + c2 * Ey[c + 2] - c2 * Ey[c - 1]) / khdx[i] -
-                                              (c1 * Ex[c + sy] - c1 * Ex[c] + c2 * Ex[c + 2 * sy] - c2 * Ex[c - sy]) / khdy[j]);
+                        s->vol[OR_DB_Z][c] * (orf_cdivr((c1 * Ey[c + 1] - c1 * Ey[c] + c2 * Ey[c + 2] - c2 * Ey[c - 1]), khdx[i]) -
+                                              orf_cdivr((c1 * Ex[c + sy] - c1 * Ex[c] + c2 * Ex[c + 2 * sy] - c2 * Ex[c - sy]), khdy[j]));
             }
 }
 
@@ -962,8 +1013,8 @@ void orf_cpml_h4(or_state *s)
             for (int j = 0; j < iy; ++j)
                 for (int i = xr[side][0]; i < xr[side][1]; ++i) {
                     const long c = FIDX(i, j, k);
-                    pHzx[c] = bhx[i] * pHzx[c] + chx[i] * (c1 * Ey[c + 1] - c1 * Ey[c] + c2 * Ey[c + 2] - c2 * Ey[c - 1]) / p->dx;
-                    pHyx[c] = bhx[i] * pHyx[c] + chx[i] * (c1 * Ez[c + 1] - c2 * Ez[c] + c2 * Ez[c + 2] - c2 * Ez[c - 1]) / p->dx;
+                    pHzx[c] = bhx[i] * pHzx[c] + orf_cdivr(chx[i] * (c1 * Ey[c + 1] - c1 * Ey[c] + c2 * Ey[c + 2] - c2 * Ey[c - 1]), p->dx);
+                    pHyx[c] = bhx[i] * pHyx[c] + orf_cdivr(chx[i] * (c1 * Ez[c + 1] - c2 * Ez[c] + c2 * Ez[c + 2] - c2 * Ez[c - 1]), p->dx);
                     Hz[c] = Hz[c] - s->vol[OR_DB_Z][c] * pHzx[c];
                     Hy[c] = Hy[c] + s->vol[OR_DB_Y][c] * pHyx[c];
                 }
@@ -973,8 +1024,8 @@ void orf_cpml_h4(or_state *s)
             for (int j = yr[side][0]; j < yr[side][1]; ++j)
                 for (int i = 0; i < ix; ++i) {
                     const long c = FIDX(i, j, k);
-                    pHxy[c] = bhy[j] * pHxy[c] + chy[j] * (c1 * Ez[c + sy] - c1 * Ez[c] + c2 * Ez[c + 2 * sy] - c2 * Ez[c - sy]) / p->dy;
-                    pHzy[c] = bhy[j] * pHzy[c] + chy[j] * (c1 * Ex[c + sy] - c1 * Ex[c] + c2 * Ex[c + 2 * sy] - c2 * Ex[c - sy]) / p->dy;
+                    pHxy[c] = bhy[j] * pHxy[c] + orf_cdivr(chy[j] * (c1 * Ez[c + sy] - c1 * Ez[c] + c2 * Ez[c + 2 * sy] - c2 * Ez[c - sy]), p->dy);
+                    pHzy[c] = bhy[j] * pHzy[c] + orf_cdivr(chy[j] * (c1 * Ex[c + sy] - c1 * Ex[c] + c2 * Ex[c + 2 * sy] - c2 * Ex[c - sy]), p->dy);
                     Hx[c] = Hx[c] - s->vol[OR_DB_X][c] * pHxy[c];
                     Hz[c] = Hz[c] + s->vol[OR_DB_Z][c] * pHzy[c];
                 }
@@ -984,8 +1035,8 @@ void orf_cpml_h4(or_state *s)
             for (int j = 0; j < iy; ++j)
                 for (int i = 0; i < ix; ++i) {
                     const long c = FIDX(i, j, k);
-                    pHyz[c] = bhz[k] * pHyz[c] + chz[k] * (c1 * Ex[c + sz] - c1 * Ex[c] + c2 * Ex[c + 2 * sz] - c2 * Ex[c - sz]) / p->dz;
-                    pHxz[c] = bhz[k] * pHxz[c] + chz[k] * (c1 * Ey[c + sz] - c1 * Ey[c] + c2 * Ey[c + 2 * sz] - c2 * Ey[c - sz]) / p->dz;
+                    pHyz[c] = bhz[k] * pHyz[c] + orf_cdivr(chz[k] * (c1 * Ex[c + sz] - c1 * Ex[c] + c2 * Ex[c + 2 * sz] - c2 * Ex[c - sz]), p->dz);
+                    pHxz[c] = bhz[k] * pHxz[c] + orf_cdivr(chz[k] * (c1 * Ey[c + sz] - c1 * Ey[c] + c2 * Ey[c + 2 * sz] - c2 * Ey[c - sz]), p->dz);
                     Hy[c] = Hy[c] - s->vol[OR_DB_Y][c] * pHyz[c];
                     Hx[c] = Hx[c] + s->vol[OR_DB_X][c] * pHxz[c];
                 }

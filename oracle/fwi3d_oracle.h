@@ -115,6 +115,7 @@ void or_update_para2(int ix, int iy, int iz, int nc, int iseabed, const double *
  * Source and receiver cells are 0-based flat indices. */
 void orf_init_cpml(or_state *s);
 void orf_media_coeff(or_state *s, const double *mu);
+void orf_waveform(int nstep, double dt, double fmax, double pi, double *out);
 void orf_source_hz(or_state *s, int npts, const int *cells, const double *mu, double sval);
 void orf_e_field_cpml4(or_state *s);
 void orf_cpml_e4(or_state *s);

@@ -100,6 +100,10 @@ def test_shot_bytes_accounts_ring_and_accumulators():
     assert plain > 8 * 6 * n
     # 15 extra ring slots of E (3 comps) + 4 complex accumulators x 3 comps + weights
     assert with_dft - plain == 8 * 3 * n * 15 + 16 * 3 * 4 * n + 16 * 4 * 100
+    # H sums add an H ring of the same 16 slots and 4 x 3 complex H accumulators
+    with_h = shot_bytes(g, it=100, nf_vol=4, ring=16, volume_dft_h=True)
+    assert with_h - with_dft == 8 * 3 * n * 16 + 16 * 3 * 4 * n
+    assert shot_bytes(g, it=100, volume_dft_h=True) == plain
 
 
 def test_null_arguments_are_errors():

@@ -83,3 +83,45 @@ def test_ten_shot_batch_traces_equal_alone(family, monkeypatch):
         for c in range(6):
             np.testing.assert_array_equal(shot.field(c), f[c], err_msg=f"shot {q} field {c}")
     b.close()
+
+
+@pytest.mark.parametrize("family", ["lds", "auto"])
+def test_ring_one_traces_many_receivers(family, monkeypatch):
+    """Shots without a volume DFT keep E in one slot updated in place; their
+    receiver gathers must come after each step's E launch, never inside the
+    next one.  256 receivers (a gather loop of 6 x 256 samples per block)
+    against the oracle stepped one step at a time, alone, cut into pieces, and
+    as three ring-1 shots in a batch beside a ring-8 shot."""
+    from mittet_main_amd import Batch
+    if family == "auto":
+        monkeypatch.delenv("MITTET_FDTD_LDS", raising=False)
+    it = 40
+    m, shot, S, sig, signal, src, rec, om, dt = setup_pair(seed=61, it=it, nrec=256, omegas=())
+    print(f"ring-1 traces: {family} family, 256 receivers, it {it}", flush=True)
+    assert shot.nf == 0   # no volume DFT: one E slot
+    want = oracle_traces(S, signal, src, rec, it)
+    shot.run()
+    np.testing.assert_array_equal(shot.traces(), want)
+    shot.set_zero_eh()
+    for p in (7, 1, 19, 12):
+        shot.step_n(p)
+    np.testing.assert_array_equal(shot.traces(), want)
+
+    specs = [dict(seed=61, nrec=256, omegas=()), dict(seed=62, nrec=200, omegas=(), ix=28),
+             dict(seed=63, nrec=256, omegas=(), iz=24), dict(seed=64, nrec=64, ring=8)]
+    alone, keep = [], []
+    for sp in specs:
+        m2, sh2, S2, _, sg2, src2, rec2, *_ = setup_pair(it=it, **sp)
+        alone.append(oracle_traces(S2, sg2, src2, rec2, it))
+        keep.append((m2, sh2))
+    b = Batch(4, chunk=8)
+    for _, sh2 in keep:
+        b.add(sh2)
+    for _ in range(5):
+        b.step(8)
+    b.sync()
+    for q, (_, sh2) in enumerate(keep):
+        b.remove(sh2)
+        assert sh2.step == it - 1
+        np.testing.assert_array_equal(sh2.traces(), alone[q], err_msg=f"batch shot {q}")
+    b.close()
