@@ -85,44 +85,101 @@ def kernel_bytes(g, nf, ring):
     }
 
 
-def dft_every_step_bytes(g, nf, nrec):
-    """Bytes of one step when the volume DFT accumulators are read and written
-    every step (no ring): N(6*16+8) + sum_axes 2*ncpml*n^2*64 + N*3*Nf*32 +
-    Nrec*3*Nf*32."""
-    N = g.ncells
-    n2 = g.ix * g.iy
-    return N * (6 * 16 + 8) + 3 * 2 * g.ncpml * n2 * 64 + N * 3 * nf * 32 + nrec * 3 * nf * 32
+def host_info():
+    """CPU model, sockets, physical cores and logical CPUs of this host
+    (from /proc/cpuinfo, as lscpu reports them), and the CPUs this process
+    may use."""
+    model, phys, logical = "unknown", set(), 0
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if cur:
+                        phys.add((cur.get("physical id", "0"), cur.get("core id", str(logical))))
+                        logical += 1
+                        cur = {}
+                    continue
+                k, v = (t.strip() for t in line.split(":", 1))
+                cur[k] = v
+                if k == "model name":
+                    model = v
+        if cur:
+            phys.add((cur.get("physical id", "0"), cur.get("core id", str(logical))))
+            logical += 1
+    except OSError:
+        pass
+    sockets = len({p for p, _ in phys}) or 1
+    return {"cpu_model": model, "sockets": sockets, "physical_cores": len(phys) or None,
+            "logical_cpus": logical or os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0))}
 
 
 def cpu_baseline(wl, seconds):
-    """The oracle (reference loop structure, OpenMP) on the same workload: one
-    untimed warm-up step, then whole steps until `seconds` of wall time or the
-    workload's last step (it - 2), whichever comes first."""
+    """The oracle (the reference's loop structure, OpenMP C) on the same
+    workload, on this host, in about `seconds` of wall time:
+      * STREAM triad GB/s with the same threads;
+      * three untimed warm-up steps, then three consecutive windows of W steps
+        with the running volume DFT and three without, all threads (the
+        OpenMP default, OMP_NUM_THREADS on the GPU host), medians reported;
+      * one window with the DFT and one without on 1 thread.
+    W fills about seconds/8 per window at the rate of warm-up steps 1-2 (at
+    least 2 steps), capped so that all windows fit in the workload."""
+    import statistics
     import oracle
     g = wl.grid
+    host = host_info()
+    threads = oracle.num_threads()
+    triad = oracle.stream_triad(40_000_000, 5)
     S = oracle.OracleSim(g.ix, g.iy, g.iz, g.ncpml, g.dx, g.dy, g.dz, g.dt, g.omega0, g.cmax)
     S.set_sigma(wl.sigma)
     S.init_cpml()
     S.media_coeff_3d()
     w = oracle.dft_weights(wl.it, g.dt, g.omega0, wl.omegas)
     acc = np.zeros((3, len(wl.omegas), g.ncells), dtype=np.complex128)
+    acc.fill(0.0)                          # fault the pages in before any timing
     traces = np.zeros((3, len(wl.receivers), wl.it))
     src1 = tuple(c + 1 for c in wl.src)
-    S.forward(0, wl.signal, src1, 1, rec1=wl.receivers + 1, it=wl.it, weights=w, acc=acc,
-              traces=traces)                      # warm-up step
-    steps = 0
-    t0 = time.perf_counter()
-    while (steps < 2 or time.perf_counter() - t0 < seconds) and 1 + steps <= wl.it - 2:
-        S.forward(0, wl.signal, src1, 1, rec1=wl.receivers + 1, it=wl.it, step0=1 + steps,
-                  weights=w, acc=acc, traces=traces)
-        steps += 1
-    dt = time.perf_counter() - t0
-    return {"value": g.ncells * steps / dt / 1e9, "unit": "Gcell-updates/s",
-            "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"{steps} steps of the {wl.name} workload ({g.ix}^3 cells incl. CPML, "
-                      f"{len(wl.omegas)}-frequency running volume DFT, {len(wl.receivers)} "
-                      f"receivers) in the reference's loop structure, after 1 warm-up step; "
-                      f"{dt:.2f} s"}
+    step = [0]
+
+    def window(n, dft):
+        t0 = time.perf_counter()
+        S.forward(0, wl.signal, src1, n, rec1=wl.receivers + 1, it=wl.it, step0=step[0],
+                  weights=w if dft else None, acc=acc if dft else None, traces=traces)
+        step[0] += n
+        return time.perf_counter() - t0
+
+    left = wl.it - 4                      # steps 3 .. it-2 follow the warm-up steps 0-2
+    if left < 8:
+        raise ValueError(f"{wl.name}: {wl.it} samples are too few for 8 timed windows")
+    window(1, True)
+    t_step = window(2, True) / 2.0        # rate probe, part of the warm-up
+    W = max(2, int(seconds / 8.0 / max(t_step, 1e-6)))
+    one = max(1, int(seconds / 8.0 / max(t_step * threads, 1e-6)))
+    one = min(one, W, max(1, left // 8))
+    W = max(1, min(W, (left - 2 * one) // 6))
+    rate = lambda n, t: g.ncells * n / t / 1e9
+    with_dft = [rate(W, window(W, True)) for _ in range(3)]
+    plain = [rate(W, window(W, False)) for _ in range(3)]
+    oracle.set_num_threads(1)
+    try:
+        one_dft = rate(one, window(one, True))
+        one_plain = rate(one, window(one, False))
+    finally:
+        oracle.set_num_threads(threads)
+    value = statistics.median(with_dft)
+    return {"value": round(value, 5), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"steps 3-{step[0] - 2 * one - 1} of the {wl.name} workload ({g.ix}^3 cells incl. "
+                      f"CPML, {len(wl.receivers)} receivers) in the reference's loop structure: "
+                      f"median of 3 windows of {W} steps with the {len(wl.omegas)}-frequency running "
+                      f"volume DFT on {threads} threads, after 3 warm-up steps",
+            "plain_step_value": round(statistics.median(plain), 5),
+            "windows_with_dft": [round(v, 5) for v in with_dft],
+            "windows_plain": [round(v, 5) for v in plain],
+            "one_thread": {"steps": one, "with_dft": round(one_dft, 5),
+                           "plain_step": round(one_plain, 5)},
+            "last_step": step[0] - 1,
+            "stream_triad_GBps": round(triad, 1), **host}
 
 
 def serve_mode(args):
@@ -290,8 +347,8 @@ def main():
                     help="forward: also sum H over the volume (H ring, fwi3d_enable_volume_dft_h)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps timed per kernel after the timed region (default: one ring)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="wall time of the CPU-oracle sample (whole steps, at least 2)")
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="wall time of the CPU-oracle sample (8 windows of whole steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (counter calibration)")
@@ -354,7 +411,6 @@ def main():
     achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
     ms_step = wall / args.steps * 1e3
     value = g.ncells * args.steps / wall / 1e9
-    sb = dft_every_step_bytes(g, nf, len(wl.receivers))
     result = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -381,14 +437,12 @@ def main():
                      "traffic": measured_traffic(wl.name, args.ring, dom),
                      "bytes_per_launch": kb[dom], "ms_per_launch": round(dom_ms, 5)},
         "kernel_ms": {k: round(v, 5) for k, v in prof.items()},
-        "step_vs_dft_every_step_bytes": {"bytes_per_step": sb,
-                                 "equivalent_GBps": round(sb / (ms_step * 1e-3) / 1e9, 1)},
         "interior_gcells": round((g.ix - 2 * g.ncpml) ** 3 * args.steps / wall / 1e9, 4),
     }
     shot.close()
     model.close()
     if not args.no_cpu_baseline:
-        log(f"[bench] cpu baseline: {args.cpu_seconds:.0f} s of oracle steps")
+        log(f"[bench] cpu baseline: about {args.cpu_seconds:.0f} s of oracle steps")
         result["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result), flush=True)

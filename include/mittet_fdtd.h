@@ -94,6 +94,27 @@ double fwi3d_lattice_time_3d(double minval, double omega0, double mu0,
                              double dx, double dy, double dz, double factor,
                              double *cmax_out);
 
+/* satan_i/fwi3d_cpu_function_v1.4.c:1392-1405 (fmax_check): the highest
+ * frequency the grid resolves, fmax = cmin/glim/max(dx, dy, dz) with
+ * cmin = sqrt(2*omega0/mu0/maxval) (fwi3d_cpu_lattice_v1.4.c:36).  Returns
+ * FWI3D_EINVAL when fmax < fmax_w (the reference prints and exits).
+ * fmax_out: fmax, may be NULL.  The reference's maxin3 (function.c:1407)
+ * returns the largest spacing as an int, which truncates spacings below one
+ * metre to 0; the spacing stays a double here.  Its time_check
+ * (function.c:1374-1389) multiplies by the global nx, which nothing assigns,
+ * so it never fires; it has no counterpart here. */
+int fwi3d_fmax_check(double maxval, double omega0, double mu0, double glim,
+                     double dx, double dy, double dz, double fmax_w, double *fmax_out);
+/* Largest stable dt of the leapfrog on a model: 1/(c_max sqrt(1/dx^2 + 1/dy^2
+ * + 1/dz^2)) times 6/7 (variant 0, the C reference's courant*6/7,
+ * fwi3d_cpu_lattice_v1.4.c:38-39) or 1/1.16667 (variant 1,
+ * ver0040/cpu_parameter_wafe.f90:118), c_max = sqrt(2*omega0/min(mu*sigma)).
+ * mu: per-cell permeability or NULL (mu0).  fwi3d_media_coeff_3d and
+ * fwi3d_model_set_mu refuse (FWI3D_EINVAL) a model whose limit is below the
+ * grid's dt, since such a run diverges.  -1 on a bad argument. */
+double fwi3d_stable_dt(const fwi3d_grid_params *p, int variant, const double *sigma,
+                       const double *mu);
+
 /* Creates the grid, computes the CPML profiles (replaces init_cpml,
  * satan_i/fwi3d_cpu_cpmlabc_v1.4.c:7-175).  stream: hipStream_t or NULL. */
 int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **out);
@@ -123,7 +144,9 @@ int fwi3d_model_get_profile(const fwi3d_model *m, int axis, int which, double *o
 /* Uploads sigma (ix*iy*iz, S/m, > 0) and derives eps' = sigma/2/omega0 on the
  * device.  Replaces media_coeff_3d (satan_i/fwi3d_cpu_function_v1.4.c:325-435):
  * the 12 coefficient volumes are recomputed in-kernel from eps' and the 1-D
- * profiles instead of being stored. */
+ * profiles instead of being stored.  FWI3D_EINVAL, leaving the model as it
+ * was, when the grid's dt exceeds the model's stability limit
+ * (fwi3d_stable_dt). */
 int fwi3d_media_coeff_3d(fwi3d_model *m, const double *sigma);
 
 /* ---- shots --------------------------------------------------------------- */
@@ -165,6 +188,12 @@ int fwi3d_shot_sync(fwi3d_shot *s);
  * (its timing is a clock() around main, rec2rec.c:18,65). */
 int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms);
 int fwi3d_shot_step(const fwi3d_shot *s); /* steps done so far */
+/* Divergence flag, with no counterpart in the reference (which runs on
+ * whatever its fields do): 1 once a receiver sample or the source cell's E or
+ * H has turned NaN or infinite since fwi3d_set_zero_eh (checked at every
+ * receiver gather and at the end of every chunk of steps), else 0; negative
+ * on error.  Waits for the shot's (or its batch's) stream. */
+int fwi3d_shot_diverged(fwi3d_shot *s);
 /* Diagnostic with no counterpart in the reference: copies `bytes` bytes between
  * two device buffers `reps` times with a plain streaming kernel (k_diag_copy),
  * so hardware byte counters can be calibrated against a known transfer.

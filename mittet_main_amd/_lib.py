@@ -46,6 +46,8 @@ _SIGS = {
     "fwi3d_last_error": (C.c_char_p, []),
     "fwi3d_version": (C.c_char_p, []),
     "fwi3d_lattice_time_3d": (C.c_double, [C.c_double] * 7 + [C.POINTER(C.c_double)]),
+    "fwi3d_fmax_check": (C.c_int, [C.c_double] * 8 + [C.POINTER(C.c_double)]),
+    "fwi3d_stable_dt": (C.c_double, [C.POINTER(GridParams), C.c_int, C.c_void_p, C.c_void_p]),
     "fwi3d_model_create": (C.c_int, [C.POINTER(GridParams), C.c_void_p, C.POINTER(C.c_void_p)]),
     "fwi3d_model_destroy": (C.c_int, [C.c_void_p]),
     "fwi3d_model_get_profile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
@@ -66,6 +68,7 @@ _SIGS = {
     "fwi3d_shot_sync": (C.c_int, [C.c_void_p]),
     "fwi3d_profile_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fwi3d_shot_step": (C.c_int, [C.c_void_p]),
+    "fwi3d_shot_diverged": (C.c_int, [C.c_void_p]),
     "fwi3d_diag_copy": (C.c_int, [C.c_longlong, C.c_int, C.c_void_p]),
     "fwi3d_get_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fwi3d_set_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),

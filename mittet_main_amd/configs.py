@@ -178,3 +178,26 @@ def gradient_128(seed: int = 4, it: int = 0, nshots: int = 8) -> GradientWorkloa
     rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, k_top)], axis=1).astype(np.int32)
     return GradientWorkload("gradient_128cube_fp64", grid, sig, sig0, it, src, rec,
                             list(np.geomspace(0.25, 2.0, 4)), firstderiv_gauss(it, dt, FMAX_W))
+
+
+def fwi_384(seed: int = 5, it: int = 1000, nshots: int = 2) -> GradientWorkload:
+    """The served-FWI grid: 384^3 interior + 16-cell CPML (416^3 cells); true
+    model = layered_box_sigma, start = smoothed_background (same seed);
+    x-dipoles 2 cells below the top interface at seeded positions; 64
+    receivers on an 8x8 grid on the interface; 8 frequencies log-spaced in
+    0.1-4 Hz.  `it` is the time-sample count (bounded: a full-length run at
+    this size takes minutes per shot)."""
+    n_int, nc = 384, 16
+    sig, k_top = layered_box_sigma(n_int, nc, seed)
+    sig0 = smoothed_background(n_int, nc, seed)
+    n = n_int + 2 * nc
+    dt, cmax = dt_for(50.0, 50.0, 50.0)
+    grid = Grid(n, n, n, nc, 50.0, 50.0, 50.0, dt, OMEGA0, cmax)
+    rng = np.random.default_rng(seed + 1000)
+    xy = rng.integers(nc + 8, n - nc - 8, size=(nshots, 2))
+    src = np.concatenate([xy, np.full((nshots, 1), k_top + 2)], axis=1).astype(np.int32)
+    g = np.linspace(nc + 4, n - nc - 5, 8).round().astype(int)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, k_top)], axis=1).astype(np.int32)
+    return GradientWorkload("fwi_384cube_fp64", grid, sig, sig0, it, src, rec,
+                            list(np.geomspace(0.1, 4.0, 8)), firstderiv_gauss(it, dt, FMAX_W))

@@ -195,6 +195,35 @@ void cpml_axis(const fwi3d_grid_params &p, int a, std::vector<double> &out, int 
     }
 }
 
+// Largest stable dt of the 4th-order leapfrog on the model: c_max =
+// sqrt(2 omega0 / min(mu sigma)) over the cells, dt_lim = 1/(c_max sqrt(sum
+// 1/dl^2)) times 6/7 (satan_i/fwi3d_cpu_lattice_v1.4.c:36-39, the C
+// reference's courant*6/7 before its 0.999 safety factor) or 1/1.16667
+// (ver0040/cpu_parameter_wafe.f90:118, Fortran, before 0.999).  A step above
+// it grows without bound.  mu: per-cell permeability (null: mu0 everywhere).
+double stable_dt(const fwi3d_grid_params &p, int variant, const double *sigma, const double *mu,
+                 size_t n)
+{
+    double m = HUGE_VAL;
+    for (size_t c = 0; c < n; ++c)
+        m = std::min(m, (mu ? mu[c] : p.mu0) * sigma[c]);
+    const double cmax = std::sqrt(2.0 * p.omega0 / m);
+    const double inv = std::sqrt(1.0 / (p.dx * p.dx) + 1.0 / (p.dy * p.dy) + 1.0 / (p.dz * p.dz));
+    return variant == 1 ? 1.0 / cmax / inv / 1.16667 : 1.0 / cmax / inv * 6.0 / 7.0;
+}
+
+int check_stable(const fwi3d_grid_params &p, int variant, const double *sigma, const double *mu,
+                 size_t n)
+{
+    const double lim = stable_dt(p, variant, sigma, mu, n);
+    if (p.dt > lim * (1.0 + 1e-12))
+        return fail(FWI3D_EINVAL,
+                    "dt %.9g exceeds the stability limit %.9g of this model (its lowest "
+                    "conductivity-permeability product sets the fastest fictitious wave)",
+                    p.dt, lim);
+    return FWI3D_OK;
+}
+
 // host rows (FWI3D_P_*) -> device rows (mt::PR_*)
 void pack_device_profile(const std::vector<double> &hp, int n, std::vector<double> &dp)
 {
@@ -311,7 +340,7 @@ struct fwi3d_shot {
     double *d_amp = nullptr, *d_traces = nullptr, *d_w = nullptr, *d_acc = nullptr;
     double *d_acc_h = nullptr;   // H volume spectra (fwi3d_enable_volume_dft_h)
     double *d_hring = nullptr;   // H after each step of a chunk, for the H sums
-    int *d_rec = nullptr, *d_step = nullptr, *d_pre = nullptr;
+    int *d_rec = nullptr, *d_step = nullptr, *d_pre = nullptr, *d_flag = nullptr;
     ShotDev desc{};
     ShotDev *d_desc = nullptr;
     bool desc_dirty = true;
@@ -517,6 +546,30 @@ double fwi3d_lattice_time_3d(double minval, double omega0, double mu0, double dx
     return courant * 6.0 / 7.0 * factor;
 }
 
+int fwi3d_fmax_check(double maxval, double omega0, double mu0, double glim, double dx, double dy,
+                     double dz, double fmax_w, double *fmax_out)
+{
+    if (!(maxval > 0) || !(omega0 > 0) || !(mu0 > 0) || !(glim > 0) || !(dx > 0) || !(dy > 0) ||
+        !(dz > 0))
+        return fail(FWI3D_EINVAL, "fmax_check: every argument must be > 0");
+    const double cmin = std::sqrt(2.0 * omega0 / mu0 / maxval);
+    const double fmax = cmin / glim / std::max(std::max(dx, dy), dz);
+    if (fmax_out)
+        *fmax_out = fmax;
+    if (fmax < fmax_w)
+        return fail(FWI3D_EINVAL, "fmax %.6g Hz of the grid is below the waveform's %.6g Hz", fmax,
+                    fmax_w);
+    return FWI3D_OK;
+}
+
+double fwi3d_stable_dt(const fwi3d_grid_params *p, int variant, const double *sigma,
+                       const double *mu)
+{
+    if (!p || !sigma || (variant != 0 && variant != 1))
+        return -1.0;
+    return stable_dt(*p, variant, sigma, mu, (size_t)p->ix * p->iy * p->iz);
+}
+
 int fwi3d_model_create(const fwi3d_grid_params *p, void *stream, fwi3d_model **out)
 {
     return fwi3d_model_create_variant(p, 0, stream, out);
@@ -598,6 +651,9 @@ int fwi3d_model_set_mu(fwi3d_model *m, const double *mu)
     for (size_t c = 0; c < N; ++c)
         if (!(mu[c] > 0) || !std::isfinite(mu[c]))
             return fail(FWI3D_EINVAL, "mu[%zu] must be finite and > 0", c);
+    int rc;
+    if (m->sig.size() == N && (rc = check_stable(m->p, m->variant, m->sig.data(), mu, N)))
+        return rc;
     m->mu.assign(mu, mu + N);
     HIP_TRY(hipStreamSynchronize(m->stream));
     if (hcopy(m->d_mu, mu, sizeof(double) * N) != hipSuccess)
@@ -642,6 +698,9 @@ int fwi3d_media_coeff_3d(fwi3d_model *m, const double *sigma)
         if (!(sigma[c] > 0) || !std::isfinite(sigma[c]))
             return fail(FWI3D_EINVAL, "sigma[%zu] = %g: conductivity must be finite and > 0", c,
                         sigma[c]);
+    int rc = check_stable(m->p, m->variant, sigma, m->variant == 1 ? m->mu.data() : nullptr, N);
+    if (rc)
+        return rc;
     m->sig.assign(sigma, sigma + N);
     HIP_TRY(hipMemcpyAsync(m->d_sig, sigma, sizeof(double) * N, hipMemcpyHostToDevice, m->stream));
     HIP_TRY(mt::launch_eps(m->d_sig, m->d_eps, m->d_cbi, (int)N, m->p.omega0, m->p.dt, m->stream));
@@ -758,7 +817,8 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
               dalloc(&s->d_amp, (size_t)sp->it * (magnetic ? 4 : 1)) == hipSuccess &&
               dalloc(&s->d_traces, std::max<size_t>(1, 6 * (size_t)sp->nrec * sp->it)) == hipSuccess &&
               dalloc(&s->d_rec, std::max<size_t>(1, (size_t)sp->nrec)) == hipSuccess &&
-              dalloc(&s->d_step, 1) == hipSuccess && dalloc(&s->d_pre, 2) == hipSuccess &&
+              dalloc(&s->d_step, 1) == hipSuccess && dalloc(&s->d_flag, 1) == hipSuccess &&
+              dalloc(&s->d_pre, 2) == hipSuccess &&
               dalloc(&s->d_desc, 1) == hipSuccess;
     if (ok && s->nf)
         ok = dalloc(&s->d_w, 2 * (size_t)s->nf * sp->it) == hipSuccess &&
@@ -855,6 +915,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.acc_h = nullptr;
     d.hring = nullptr;
     d.step = s->d_step;
+    d.flag = s->d_flag;
     d.variant = m->variant;
     d.mu = m->d_mu;
     d.nsrc_h = (int)s->src_h.size();
@@ -960,6 +1021,7 @@ int fwi3d_shot_destroy(fwi3d_shot *s)
     dfree(s->d_hring);
     dfree(s->d_rec);
     dfree(s->d_step);
+    dfree(s->d_flag);
     dfree(s->d_pre);
     dfree(s->d_shell);
     dfree(s->d_desc);
@@ -988,6 +1050,7 @@ int fwi3d_set_zero_eh(fwi3d_shot *s)
         HIP_TRY(mt::launch_fill(s->d_acc_h, 2 * 3 * (size_t)s->nf * s->desc.astride, 0.0, s->st));
     static const int kZero = 0;
     HIP_TRY(hipMemcpyAsync(s->d_step, &kZero, sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIP_TRY(hipMemcpyAsync(s->d_flag, &kZero, sizeof(int), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     s->step = 0;
     if (s->desc.cur != 0) {
@@ -1218,6 +1281,17 @@ int fwi3d_shot_sync(fwi3d_shot *s)
 }
 
 int fwi3d_shot_step(const fwi3d_shot *s) { return s ? s->step : -1; }
+
+int fwi3d_shot_diverged(fwi3d_shot *s)
+{
+    if (!s)
+        return fail(FWI3D_EINVAL, "null shot");
+    hipStream_t st = s->batch ? s->batch->st : s->st;
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, s->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return flag ? 1 : 0;
+}
 
 int fwi3d_get_field(fwi3d_shot *s, int which, double *out)
 {

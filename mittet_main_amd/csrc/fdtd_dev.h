@@ -79,6 +79,8 @@ struct ShotDev {
     double *acc_h;            // H volume spectra, same layout (null: off)
     double *hring;            // with acc_h: H after each step, ring slots like e
     int *step;                // absolute step of local index j = 0 of the current chunk
+    int *flag;                // set to 1 once a gathered sample or the source cell's field
+                              // turns NaN or infinite (divergence probe)
     int variant;              // 0: C v1.4 semantics; 1: Fortran ver0040 (fdtd_fortran.hip)
     int nsrc_h;               // variant 1: magnetic (Hz) source points; amp[q * (it - 1) + n]
     int src_h[4];             // their cells

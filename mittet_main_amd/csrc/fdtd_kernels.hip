@@ -395,6 +395,8 @@ __device__ __forceinline__ void rec_store(const ShotDev &S, int n, int so)
         const double v = comp < 3 ? S.e[(size_t)so * S.sstride + (size_t)comp * S.n + cell]
                                   : S.h[(size_t)(comp - 3) * S.n + cell];
         S.traces[((size_t)comp * S.nrec + r) * S.it + n] = v;
+        if (!__builtin_isfinite(v))
+            *S.flag = 1;
     }
 }
 
@@ -654,11 +656,25 @@ hipError_t launch_hstore(const ShotDev *d, int ns, const int *pre_cells, int nbl
     return hipGetLastError();
 }
 
+// Step counters forward by L; then the divergence probe: E (the slot the
+// chunk ended in) and H at the source cell (the first Hz point of a
+// Fortran-variant shot) must be finite.
 __global__ void k_advance(const ShotDev *__restrict__ D, int ns, int L)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < ns)
-        *D[s].step += L;
+    if (s >= ns)
+        return;
+    const ShotDev &S = D[s];
+    *S.step += L;
+    const int c = S.src_cell >= 0 ? S.src_cell : (S.nsrc_h > 0 ? S.src_h[0] : -1);
+    if (c < 0)
+        return;
+    const double *E = S.e + (size_t)((S.cur + L) % S.ring) * S.sstride;
+    double v = 0.0;
+    for (int d = 0; d < 3; ++d)
+        v += __builtin_fabs(E[(size_t)d * S.n + c]) + __builtin_fabs(S.h[(size_t)d * S.n + c]);
+    if (!__builtin_isfinite(v))
+        *S.flag = 1;
 }
 
 // One wavefront per (row, frequency): sum_n x[n] * dt*exp(-a n dt)*exp(i a n dt),

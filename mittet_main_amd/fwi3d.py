@@ -298,6 +298,13 @@ class Shot:
     def step(self) -> int:
         return _lib.load().fwi3d_shot_step(self._h)
 
+    @property
+    def diverged(self) -> bool:
+        """A receiver sample or the source cell's field turned NaN or infinite."""
+        rc = _lib.load().fwi3d_shot_diverged(self._h)
+        check(min(rc, 0))
+        return rc == 1
+
     # ---- readback
     def field(self, which: int) -> np.ndarray:
         out = np.zeros(self.model.grid.shape)

@@ -104,6 +104,9 @@ def lib():
         L.or_fft_backward_real.argtypes = [i, dp, dp]
         L.or_laplace_to_freq.argtypes = [i, d, d, i, dp, dp, dp, dp, dp, dp, dp]
         L.or_num_threads.restype = i
+        L.or_set_num_threads.argtypes = [i]
+        L.or_stream_triad.restype = d
+        L.or_stream_triad.argtypes = [C.c_long, i]
         for nm in ("orf_init_cpml", "orf_e_field_cpml4", "orf_cpml_e4", "orf_h_field_cpml4",
                    "orf_cpml_h4"):
             getattr(L, nm).argtypes = [vp]
@@ -125,6 +128,15 @@ def lib():
 
 def num_threads() -> int:
     return lib().or_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    lib().or_set_num_threads(int(n))
+
+
+def stream_triad(n: int = 40_000_000, reps: int = 5) -> float:
+    """Best STREAM-triad GB/s of the host with the current thread count."""
+    return lib().or_stream_triad(int(n), int(reps))
 
 
 def lattice_time_3d(minval, omega0, dx, dy, dz, factor=DT_FACTOR_REF, mu0=MU0_C):

@@ -47,6 +47,55 @@ int or_num_threads(void)
 #endif
 }
 
+/* thread count of the following parallel regions (the CPU baseline's 1-thread
+ * and all-core runs) */
+void or_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+/* STREAM triad a = b + s*c over n doubles per array with the current thread
+ * count, first touch by the same threads; best of `reps` passes in GB/s
+ * (24 bytes moved per element, as STREAM counts them).  Host-memory bandwidth
+ * beside the CPU baseline; no counterpart in the reference. */
+double or_stream_triad(long n, int reps)
+{
+    double *a = (double *)malloc(sizeof(double) * n);
+    double *b = (double *)malloc(sizeof(double) * n);
+    double *c = (double *)malloc(sizeof(double) * n);
+    if (!a || !b || !c) {
+        free(a);
+        free(b);
+        free(c);
+        return -1.0;
+    }
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < n; ++q) {
+        a[q] = 0.0;
+        b[q] = 1.0;
+        c[q] = 2.0;
+    }
+    double best = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < n; ++q)
+            a[q] = b[q] + 3.0 * c[q];
+        const double t = omp_get_wtime() - t0;
+        const double gbs = 24.0 * (double)n / t / 1e9;
+        if (gbs > best)
+            best = gbs;
+    }
+    free(a);
+    free(b);
+    free(c);
+    return best;
+}
+
 /* satan_i/fwi3d_cpu_lattice_v1.4.c:35-39: cmax from the minimum conductivity,
  * Courant number of the 3-D grid, dt = courant*6/7*factor (factor 0.999f in
  * the reference; 6/7 = 1/(|c1|+|c2|) is the 4th-order stability margin). */

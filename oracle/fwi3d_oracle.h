@@ -55,6 +55,8 @@ void or_destroy(or_state *s);
 double *or_array(or_state *s, int id);
 double *or_profile(or_state *s, int axis, int id);
 int or_num_threads(void);
+void or_set_num_threads(int n);
+double or_stream_triad(long n, int reps);
 
 void or_init_cpml(or_state *s);
 void or_media_coeff_3d(or_state *s);

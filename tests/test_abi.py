@@ -112,3 +112,45 @@ def test_null_arguments_are_errors():
     assert L.fwi3d_get_traces(None, None) == -1
     assert L.fwi3d_batch_step(None, 1) == -1
     assert L.fwi3d_model_destroy(None) == 0
+
+
+def test_fmax_check_follows_the_reference_rule():
+    """fmax = cmin/Glim/max(dx,dy,dz), cmin = sqrt(2 w0/mu0/maxval)
+    (function.c:1392-1405, lattice.c:36): the 64^3 workload (sigma_max 3.3,
+    Glim 10.4 of posiinf.c:40, 50 m) resolves 3.35 Hz, so a 3 Hz waveform
+    passes and a 4 Hz one is refused."""
+    L = _lib.load()
+    fm = C.c_double(0.0)
+    assert L.fwi3d_fmax_check(3.3, 2 * np.pi, MU0, 10.4, 50.0, 40.0, 50.0, 3.0, C.byref(fm)) == 0
+    cmin = np.sqrt(2 * 2 * np.pi / MU0 / 3.3)
+    assert fm.value == cmin / 10.4 / 50.0
+    assert abs(fm.value - 3.35) < 0.01
+    assert L.fwi3d_fmax_check(3.3, 2 * np.pi, MU0, 10.4, 50.0, 40.0, 50.0, 4.0, None) == -1
+    assert b"fmax" in L.fwi3d_last_error()
+    # sub-metre spacings stay doubles (the reference's int maxin3 would divide by 0)
+    assert L.fwi3d_fmax_check(3.2, 2 * np.pi, MU0, 10.4, 4.5e-3, 4.5e-3, 4.5e-3, 2e3, C.byref(fm)) == 0
+    assert np.isfinite(fm.value) and fm.value > 2e3
+    assert L.fwi3d_fmax_check(-1.0, 2 * np.pi, MU0, 10.4, 50.0, 50.0, 50.0, 1.0, None) == -1
+
+
+def test_stable_dt_is_the_lattice_limit():
+    """Variant 0: the reference's courant*6/7 at the lowest conductivity
+    (lattice_time_3d with factor 1); variant 1: 1/1.16667 of the Fortran dt
+    rule, with per-cell mu entering through min(mu*sigma)."""
+    L = _lib.load()
+    sig = np.exp(np.random.default_rng(0).uniform(np.log(0.02), np.log(3.0), (12, 11, 10)))
+    g = Grid(10, 11, 12, 2, 50.0, 40.0, 30.0, 1e-3, 2 * np.pi, 1.0)
+    gp = g.to_c()
+    d0 = L.fwi3d_stable_dt(C.byref(gp), 0, sig.ctypes.data_as(C.c_void_p), None)
+    np.testing.assert_allclose(
+        d0, lattice_time_3d(float(sig.min()), 2 * np.pi, 50.0, 40.0, 30.0, factor=1.0)[0],
+        rtol=1e-15)
+    mu = np.full(sig.shape, MU0)
+    c = np.unravel_index(np.argmin(sig), sig.shape)
+    mu[c] = 0.25 * MU0                     # 2x the wave speed at the slowest cell
+    d1 = L.fwi3d_stable_dt(C.byref(gp), 1, sig.ctypes.data_as(C.c_void_p),
+                           mu.ctypes.data_as(C.c_void_p))
+    cmax = np.sqrt(2 * 2 * np.pi / (0.25 * MU0 * sig.min()))
+    inv = np.sqrt(1 / 50.0 ** 2 + 1 / 40.0 ** 2 + 1 / 30.0 ** 2)
+    np.testing.assert_allclose(d1, 1.0 / cmax / inv / 1.16667, rtol=1e-15)
+    assert L.fwi3d_stable_dt(C.byref(gp), 2, sig.ctypes.data_as(C.c_void_p), None) == -1.0

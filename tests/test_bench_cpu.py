@@ -21,11 +21,15 @@ def tiny(it=6):
 
 
 def test_cpu_baseline_stops_at_the_last_step():
-    wl = tiny(it=6)
-    out = bench.cpu_baseline(wl, 5.0)      # far longer than 4 steps of a 16^3 grid take
-    steps = int(out["sample"].split()[0])
-    assert steps == wl.it - 2              # timed steps 1 .. it-2 after the warm-up step 0
-    assert out["kind"] == "port" and out["value"] > 0
+    wl = tiny(it=14)
+    out = bench.cpu_baseline(wl, 5.0)      # far longer than 12 steps of a 16^3 grid take
+    assert out["last_step"] <= wl.it - 2   # timed steps 1 .. it-2 after the warm-up step 0
+    assert len(out["windows_with_dft"]) == 3 and len(out["windows_plain"]) == 3
+    assert out["kind"] == "port" and out["value"] > 0 and out["plain_step_value"] > 0
+    assert out["one_thread"]["with_dft"] > 0 and out["stream_triad_GBps"] > 0
+    assert out["sockets"] >= 1 and out["cores"] >= 1
+    with pytest.raises(ValueError):
+        bench.cpu_baseline(tiny(it=8), 1.0)
 
 
 def test_oracle_refuses_steps_past_the_arrays():
