@@ -102,7 +102,10 @@ def test_compat_iteration_matches_oracle(dims):
     rng = np.random.default_rng(12)
     sig_true = np.exp(rng.uniform(np.log(0.2), np.log(2.0), (iz, iy, ix)))
     sig0 = np.full((iz, iy, ix), 0.6)
-    dt, cmax = dt_for(50.0, 45.0, 40.0, sig_min=0.2)
+    # dt from the update's lower clamp, as the reference takes it from minval
+    # (minmax.dat): every model the iteration can produce is then stable
+    minval, maxval = 0.05, 5.0
+    dt, cmax = dt_for(50.0, 45.0, 40.0, sig_min=minval)
     g = Grid(ix, iy, iz, nc, 50.0, 45.0, 40.0, dt, OM0, cmax)
     signal_x = firstderiv_gauss(it, dt, 6.0)
     signal_true = firstderiv_gauss(it, dt, 2.0)
@@ -110,7 +113,7 @@ def test_compat_iteration_matches_oracle(dims):
     receivers = np.array([[7, 7, 5], [14, 12, 5]])
     assert sources.max() < min(dims) and receivers.max() < min(dims)
     F = CompatFWI(g, sources, receivers, it, signal_x, signal_true, start_f, upper_f,
-                  minval=0.05, maxval=5.0, iseabed=iseabed)
+                  minval=minval, maxval=maxval, iseabed=iseabed)
     true_model = F.test_model
     true_model.media_coeff_3d(sig_true)
     eobs = [F.forward_receivers(true_model, s) for s in sources]
@@ -118,7 +121,7 @@ def test_compat_iteration_matches_oracle(dims):
 
     # ---- oracle: the same iteration with the reference's loops
     want = oracle_iteration(g, sig0, sources, receivers, it, signal_x, signal_true, start_f,
-                            upper_f, iseabed, 0.05, 5.0, eobs, F.omegas)
+                            upper_f, iseabed, minval, maxval, eobs, F.omegas)
     check_iteration(res, want, sig0)
     # the model now holds the updated conductivity: same data as a fresh upload
     after = F.forward_receivers(F.model, sources[0])
