@@ -179,11 +179,13 @@ int fwi3d_step_n(fwi3d_shot *s, int nsteps);
 int fwi3d_run(fwi3d_shot *s);
 int fwi3d_shot_sync(fwi3d_shot *s);
 /* Runs nsteps fused steps with HIP events around every launch on the shot's
- * stream and returns the mean duration in ms of each kernel kind (ms holds 7
+ * stream and returns the mean duration in ms of each kernel kind (ms holds 9
  * doubles): ms[0] E+CPML-E+source, ms[1] H+CPML-H, ms[2] receiver gather,
  * ms[3] volume-DFT flush (per flush launch), ms[4] whole step (per step, every
  * launch included), ms[5] H ring store and ms[6] H-sum flush (shots with
- * fwi3d_enable_volume_dft_h; else 0).
+ * fwi3d_enable_volume_dft_h; else 0), ms[7] the fused E+H launch over the
+ * CPML-free interior and ms[8] 1.0 when the shot runs it (then ms[0] and
+ * ms[1] cover the CPML shell only), else 0.
  * Advances the shot like fwi3d_step_n.  No counterpart in the reference
  * (its timing is a clock() around main, rec2rec.c:18,65). */
 int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms);

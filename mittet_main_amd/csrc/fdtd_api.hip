@@ -306,6 +306,32 @@ int lds_chunk(int ltx, int lty, int nz)
 // 216^3 18.1 / 16.5.  MITTET_FDTD_LDS=1 takes it on any grid.
 constexpr long long kLdsMinCells = 2500000;
 
+// Whether a C-variant shot takes the LDS family (family 3): size, no z-chunk
+// or x-pair override, and a CPML-free interior to march.
+bool takes_lds(const fwi3d_grid_params &p, int zchunk)
+{
+    const long long N = (long long)p.ix * p.iy * p.iz;
+    const int xlo = (p.ncpml + 15) / 16 * 16;
+    const bool pair = p.ix % 2 == 0 && env_pad("MITTET_FDTD_PAIR", 0);
+    const bool lds_size = N >= kLdsMinCells || env_pad("MITTET_FDTD_LDS", 0);
+    return zchunk == 0 && !pair && lds_size && p.ncpml >= 2 && xlo < p.ix - p.ncpml &&
+           p.ncpml < p.iy - p.ncpml && p.ncpml < p.iz - p.ncpml && !env_pad("MITTET_FDTD_NOLDS", 0);
+}
+
+// With MITTET_FDTD_FUSE=1 at shot creation, family-3 shots with E in a ring
+// of more than one slot run the fused E+H interior kernel (k_ehfield_l),
+// which reads E and H of the chunk's planes k0 - 3 .. k1 + 2 (so ncpml >= 3)
+// and indexes H planes with 32-bit offsets.  It is opt-in: at 288^3 it moves
+// 104 B per interior cell instead of 152 but takes 0.60 ms against 0.49 ms
+// for k_efield_l + k_hfield_l (register-bound at two blocks per CU, two
+// barriers per plane and a 1.42x E halo recomputation).
+bool takes_fused(const fwi3d_grid_params &p, int zchunk, int ring)
+{
+    const long long N = (long long)p.ix * p.iy * p.iz;
+    return takes_lds(p, zchunk) && ring > 1 && p.ncpml >= 3 && 3 * N < (1LL << 31) &&
+           env_pad("MITTET_FDTD_FUSE", 0);
+}
+
 constexpr long long kSlotPad = 0;
 constexpr long long kAccPad = 0;
 
@@ -337,6 +363,8 @@ struct fwi3d_shot {
     std::vector<double> signal, omega;
     std::vector<int> rec;
     double *d_e = nullptr, *d_h = nullptr, *d_psi[3] = {nullptr, nullptr, nullptr};
+    double *d_h2 = nullptr;   // second H buffer of the fused E+H step (null: not fused)
+    bool fused = false;       // steps run the fused E+H interior kernel
     double *d_amp = nullptr, *d_traces = nullptr, *d_w = nullptr, *d_acc = nullptr;
     double *d_acc_h = nullptr;   // H volume spectra (fwi3d_enable_volume_dft_h)
     double *d_hring = nullptr;   // H after each step of a chunk, for the H sums
@@ -442,6 +470,7 @@ struct Fam {
     const int *pre_l;
     int nblocks_l;
     bool rings;   // every shot keeps E in a ring of more than one slot
+    bool fused;   // family 3, every shot with a second H buffer: the fused E+H step
 };
 
 // Families whose E half-step has a per-cell launch that can carry the
@@ -456,7 +485,9 @@ bool rec_folds(const Fam &f)
 }
 
 // rec_prev: the E launch of step j also gathers step j - 1's receivers
-// (rec_folds(f) must hold)
+// (rec_folds(f) must hold).  Fused family 3: the E half-step is the shell E
+// launch plus the fused E+H interior launch, the H half-step the shell H
+// launch alone.
 int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t st,
               bool rec_prev = false)
 {
@@ -476,12 +507,16 @@ int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t
         e = h ? mt::launch_hfield_p(d, ns, f.pre, f.nblocks, j, 1, st)
               : mt::launch_efield_p(d, ns, f.pre, f.nblocks, j, 1, st);
     else
-        e = h ? mt::launch_hfield_f(d, ns, f.pre, f.nblocks, j, 1, st)
-              : mt::launch_efield_f(d, ns, f.pre, f.nblocks, j, 1, st, rec_prev);
+        e = h ? mt::launch_hfield_f(d, ns, f.pre, f.nblocks, j, 1, st, f.fused)
+              : mt::launch_efield_f(d, ns, f.pre, f.nblocks, j, 1, st, rec_prev, f.fused);
     HIP_TRY(e);
-    if (f.kind == 3)
+    if (f.kind == 3 && f.fused) {
+        if (!h)
+            HIP_TRY(mt::launch_ehfield_l(d, ns, f.pre_l, f.nblocks_l, j, st));
+    } else if (f.kind == 3) {
         HIP_TRY(h ? mt::launch_hfield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st)
                   : mt::launch_efield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st));
+    }
     return FWI3D_OK;
 }
 
@@ -491,20 +526,21 @@ int launch_chunk(const ShotDev *d, int ns, const Fam &f, const int *pre_cells, i
 {
     int rc;
     const bool fold = any_rec && rec_folds(f);   // step j's gather rides on step j + 1's E launch
+    const int hf = f.fused ? 1 : 0;
     for (int j = 0; j < L; ++j) {
         if ((rc = launch_eh(f, d, ns, j, false, st, fold && j > 0)) ||
             (rc = launch_eh(f, d, ns, j, true, st)))
             return rc;
         if (any_rec && !(fold && j + 1 < L))
-            HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st));
+            HIP_TRY(mt::launch_receivers(d, ns, j, -1, 0, st, hf));
         if (any_hdft)
-            HIP_TRY(mt::launch_hstore(d, ns, pre_cells, ncellblocks, j, st));
+            HIP_TRY(mt::launch_hstore(d, ns, pre_cells, ncellblocks, j, st, hf));
     }
     if (any_dft)
         HIP_TRY(mt::launch_flush(d, ns, pre_cells, ncellblocks, L, st));
     if (any_hdft)
         HIP_TRY(mt::launch_flush(d, ns, pre_cells, ncellblocks, L, st, true));
-    HIP_TRY(mt::launch_advance(d, ns, L, st));
+    HIP_TRY(mt::launch_advance(d, ns, L, st, hf));
     return FWI3D_OK;
 }
 
@@ -717,6 +753,8 @@ long long fwi3d_shot_bytes(const fwi3d_grid_params *p, const fwi3d_shot_params *
     const long long nc2 = 2LL * p->ncpml;
     const int ring = sp->nf_vol > 0 ? (sp->ring > 0 ? sp->ring : kDefaultChunk) : 1;
     long long b = 8LL * (3LL * N * ring + 3LL * N);                          // E ring, H
+    if (takes_fused(*p, sp->zchunk, ring))
+        b += 8LL * 3 * N;                                                     // second H
     b += 8LL * 4 * ((long long)p->iz * p->iy * nc2 + (long long)p->iz * nc2 * p->ix +
                     nc2 * p->iy * p->ix);                                     // psi slabs
     b += 8LL * sp->it + 8LL * 6 * sp->nrec * (long long)sp->it + 4LL * sp->nrec;
@@ -903,6 +941,8 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     d.ring = ring;
     d.cur = 0;
     d.h = s->d_h;
+    d.h2 = nullptr;
+    d.hpar = 0;
     d.src_comp = s->src_comp;
     d.src_cell = s->src_cell;
     d.amp = s->d_amp;
@@ -938,9 +978,7 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
     // family alone (two launches per step instead of four) is faster.
     d.shell = nullptr;
     const int xlo = (p.ncpml + 15) / 16 * 16;
-    const bool lds_size = (long long)N >= kLdsMinCells || env_pad("MITTET_FDTD_LDS", 0);
-    if (s->kind == 0 && lds_size && p.ncpml >= 2 && xlo < p.ix - p.ncpml &&
-        p.ncpml < p.iy - p.ncpml && p.ncpml < p.iz - p.ncpml && !env_pad("MITTET_FDTD_NOLDS", 0)) {
+    if (s->kind == 0 && takes_lds(p, sp->zchunk)) {
         const int lo[3] = {xlo, p.ncpml, p.ncpml};
         const int hi[3] = {p.ix - p.ncpml, p.iy - p.ncpml, p.iz - p.ncpml};
         for (int a = 0; a < 3; ++a) {
@@ -966,6 +1004,23 @@ int fwi3d_shot_create(fwi3d_model *m, const fwi3d_shot_params *sp, fwi3d_shot **
         d.shell = s->d_shell;
         s->kind = 3;
         s->nblocks_l = d.lb_pad;
+        // the fused kernel takes one ked / khd per axis over the region (kappa = 1)
+        bool flat = true;
+        for (int a = 0; a < 3; ++a) {
+            const int na = axis_len(p, a);
+            const double *ked = &m->prof[a][(size_t)FWI3D_P_KED * na];
+            const double *khd = &m->prof[a][(size_t)FWI3D_P_KHD * na];
+            for (int q = lo[a]; q < hi[a]; ++q)
+                flat = flat && ked[q] == ked[lo[a]] && khd[q] == khd[lo[a]];
+        }
+        if (flat && takes_fused(p, sp->zchunk, ring)) {
+            if (dalloc(&s->d_h2, 3 * N) != hipSuccess) {
+                fwi3d_shot_destroy(s);
+                return fail(FWI3D_ENOMEM, "second H buffer (%zu cells)", N);
+            }
+            s->fused = true;
+            d.h2 = s->d_h2;
+        }
     }
     s->nblocks = s->kind == 2 ? d.zb_pad : d.fb_pad;
     s->ncellblocks = tiles((int)N, mt::FLUSH_CELLS);
@@ -1011,6 +1066,7 @@ int fwi3d_shot_destroy(fwi3d_shot *s)
         (void)hipGraphExecDestroy(s->graph);
     dfree(s->d_e);
     dfree(s->d_h);
+    dfree(s->d_h2);
     for (auto &q : s->d_psi)
         dfree(q);
     dfree(s->d_amp);
@@ -1041,6 +1097,8 @@ int fwi3d_set_zero_eh(fwi3d_shot *s)
     const size_t N = (size_t)p.ix * p.iy * p.iz;
     HIP_TRY(mt::launch_fill(s->d_e, (size_t)s->desc.sstride * s->ring, 0.0, s->st));
     HIP_TRY(mt::launch_fill(s->d_h, 3 * N, 0.0, s->st));
+    if (s->d_h2)
+        HIP_TRY(mt::launch_fill(s->d_h2, 3 * N, 0.0, s->st));
     for (int a = 0; a < 3; ++a)
         HIP_TRY(mt::launch_fill(s->d_psi[a], mt::PSI_N * (size_t)s->desc.slab[a], 0.0, s->st));
     HIP_TRY(mt::launch_fill(s->d_traces, std::max<size_t>(1, 6 * (size_t)s->nrec * s->it), 0.0, s->st));
@@ -1053,8 +1111,9 @@ int fwi3d_set_zero_eh(fwi3d_shot *s)
     HIP_TRY(hipMemcpyAsync(s->d_flag, &kZero, sizeof(int), hipMemcpyHostToDevice, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     s->step = 0;
-    if (s->desc.cur != 0) {
+    if (s->desc.cur != 0 || s->desc.hpar != 0) {
         s->desc.cur = 0;
+        s->desc.hpar = 0;
         s->desc_dirty = true;
     }
     return FWI3D_OK;
@@ -1112,7 +1171,25 @@ int fwi3d_output(fwi3d_shot *s, int step) { return single_op(s, 5, 1, step); }
 
 static Fam shot_fam(const fwi3d_shot *s)
 {
-    return {s->kind, s->d_pre, s->nblocks, s->d_pre, s->nblocks_l, s->ring > 1};
+    return {s->kind, s->d_pre, s->nblocks, s->d_pre, s->nblocks_l, s->ring > 1,
+            s->kind == 3 && s->fused};
+}
+
+// After `steps` steps of a fused schedule the current H is in the other buffer
+// when `steps` is odd, and E sits `steps` ring slots further on.  Returns
+// whether the device descriptor changed.
+static bool advance_desc(fwi3d_shot *s, int steps, bool fused)
+{
+    bool moved = false;
+    if (s->ring > 1 && steps % s->ring) {
+        s->desc.cur = (s->desc.cur + steps) % s->ring;
+        moved = true;
+    }
+    if (fused && (steps & 1)) {
+        s->desc.hpar ^= 1;
+        moved = true;
+    }
+    return moved;
 }
 
 int fwi3d_step_n(fwi3d_shot *s, int nsteps)
@@ -1142,11 +1219,8 @@ int fwi3d_step_n(fwi3d_shot *s, int nsteps)
         }
         s->step += L;
         left -= L;
-        if (s->ring > 1 && L % s->ring) {
-            s->desc.cur = (s->desc.cur + L) % s->ring;
-            if ((rc = upload_desc(s)))
-                return rc;
-        }
+        if (advance_desc(s, L, shot_fam(s).fused) && (rc = upload_desc(s)))
+            return rc;
     }
     return FWI3D_OK;
 }
@@ -1164,8 +1238,13 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
     hipEvent_t ev[2];
     HIP_TRY(hipEventCreate(&ev[0]));
     HIP_TRY(hipEventCreate(&ev[1]));
-    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const Fam f = shot_fam(s);
+    const int hf = f.fused ? 1 : 0;
+    // the E half-step's launches without the fused interior launch, timed apart
+    Fam fe = f;
+    fe.nblocks_l = f.fused ? 0 : f.nblocks_l;
     auto timed = [&](int slot, auto &&launch) -> int {
         HIP_TRY(hipEventRecord(ev[0], s->st));
         HIP_TRY(launch());
@@ -1183,19 +1262,21 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
         const int L = std::min(chunk, left);
         for (int j = 0; j < L && !rc; ++j) {
             rc = timed(0, [&] {
-                return launch_eh(shot_fam(s), s->d_desc, 1, j, false, s->st) ? hipErrorUnknown
-                                                                             : hipSuccess;
+                return launch_eh(fe, s->d_desc, 1, j, false, s->st) ? hipErrorUnknown : hipSuccess;
             });
+            if (!rc && f.fused)
+                rc = timed(7, [&] {
+                    return mt::launch_ehfield_l(s->d_desc, 1, s->d_pre, s->nblocks_l, j, s->st);
+                });
             if (!rc)
                 rc = timed(1, [&] {
-                    return launch_eh(shot_fam(s), s->d_desc, 1, j, true, s->st) ? hipErrorUnknown
-                                                                                : hipSuccess;
+                    return launch_eh(f, s->d_desc, 1, j, true, s->st) ? hipErrorUnknown : hipSuccess;
                 });
             if (!rc && s->nrec)
-                rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st); });
+                rc = timed(2, [&] { return mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st, hf); });
             if (!rc && s->d_acc_h)
                 rc = timed(5, [&] {
-                    return mt::launch_hstore(s->d_desc, 1, s->d_pre, s->ncellblocks, j, s->st);
+                    return mt::launch_hstore(s->d_desc, 1, s->d_pre, s->ncellblocks, j, s->st, hf);
                 });
         }
         if (!rc && s->nf)
@@ -1205,26 +1286,25 @@ int fwi3d_profile_steps(fwi3d_shot *s, int nsteps, double *ms)
                 return mt::launch_flush(s->d_desc, 1, s->d_pre, s->ncellblocks, L, s->st, true);
             });
         if (!rc)
-            rc = timed(4, [&] { return mt::launch_advance(s->d_desc, 1, L, s->st); });
+            rc = timed(4, [&] { return mt::launch_advance(s->d_desc, 1, L, s->st, hf); });
         if (rc)
             break;
         s->step += L;
         left -= L;
-        if (s->ring > 1 && L % s->ring) {
-            s->desc.cur = (s->desc.cur + L) % s->ring;
+        if (advance_desc(s, L, f.fused))
             rc = upload_desc(s);
-        }
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     if (rc)
         return rc;
-    for (int q = 0; q < 7; ++q)
+    for (int q = 0; q < 8; ++q)
         ms[q] = cnt[q] ? acc[q] / cnt[q] : 0.0;
     double tot = 0.0;
-    for (int q = 0; q < 7; ++q)
+    for (int q = 0; q < 8; ++q)
         tot += acc[q];
     ms[4] = steps_run ? tot / steps_run : 0.0;
+    ms[8] = f.fused ? 1.0 : 0.0;
     return FWI3D_OK;
 }
 
@@ -1299,7 +1379,7 @@ int fwi3d_get_field(fwi3d_shot *s, int which, double *out)
         return fail(FWI3D_EINVAL, "bad field request");
     const size_t N = (size_t)s->desc.n;
     const double *src = which < 3 ? s->d_e + (size_t)s->desc.cur * s->desc.sstride + (size_t)which * N
-                                  : s->d_h + (size_t)(which - 3) * N;
+                                  : mt::hsel(s->desc, 0, 0) + (size_t)(which - 3) * N;
     HIP_TRY(hipMemcpyAsync(out, src, sizeof(double) * N, hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     return FWI3D_OK;
@@ -1311,8 +1391,14 @@ int fwi3d_set_field(fwi3d_shot *s, int which, const double *in)
         return fail(FWI3D_EINVAL, "bad field request");
     const size_t N = (size_t)s->desc.n;
     double *dst = which < 3 ? s->d_e + (size_t)s->desc.cur * s->desc.sstride + (size_t)which * N
-                            : s->d_h + (size_t)(which - 3) * N;
+                            : mt::hsel(s->desc, 0, 0) + (size_t)(which - 3) * N;
     HIP_TRY(hipMemcpyAsync(dst, in, sizeof(double) * N, hipMemcpyHostToDevice, s->st));
+    // both H buffers of a fused shot: cells no step updates must agree in them
+    if (which >= 3 && s->d_h2) {
+        double *other = (dst == s->d_h + (size_t)(which - 3) * N ? s->d_h2 : s->d_h) +
+                        (size_t)(which - 3) * N;
+        HIP_TRY(hipMemcpyAsync(other, in, sizeof(double) * N, hipMemcpyHostToDevice, s->st));
+    }
     HIP_TRY(hipStreamSynchronize(s->st));
     return FWI3D_OK;
 }
@@ -1769,7 +1855,7 @@ static mt::StateSegs state_segs(const fwi3d_grad *g)
     mt::StateSegs v{};
     v.live[0] = s->d_e;
     v.count[0] = 3 * N;
-    v.live[1] = s->d_h;
+    v.live[1] = mt::hsel(s->desc, 0, 0);
     v.count[1] = 3 * N;
     for (int a = 0; a < 3; ++a) {
         v.live[2 + a] = s->d_psi[a];
@@ -2673,10 +2759,12 @@ static int batch_upload(fwi3d_batch *b)
 
 static Fam batch_fam(const fwi3d_batch *b)
 {
-    bool rings = true;
-    for (const auto *s : b->shots)
+    bool rings = true, fused = b->kind == 3;
+    for (const auto *s : b->shots) {
         rings = rings && s->ring > 1;
-    return {b->kind, b->d_pre, b->nblocks, b->d_pre_l, b->nblocks_l, rings};
+        fused = fused && s->kind == 3 && s->fused;
+    }
+    return {b->kind, b->d_pre, b->nblocks, b->d_pre_l, b->nblocks_l, rings, fused};
 }
 
 int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
@@ -2711,13 +2799,11 @@ int fwi3d_batch_step(fwi3d_batch *b, int nsteps)
         }
         left -= L;
         bool moved = false;
+        const bool fused = batch_fam(b).fused;
         for (auto *s : b->shots) {
             const int done = std::min(L, s->it - 1 - s->step);
             s->step += done;
-            if (s->ring > 1 && done % s->ring) {
-                s->desc.cur = (s->desc.cur + done) % s->ring;
-                moved = true;
-            }
+            moved = advance_desc(s, done, fused) || moved;
             s->desc_dirty = true;
         }
         if (moved && (rc = batch_upload(b)))

@@ -64,7 +64,11 @@ struct ShotDev {
     long long sstride;        // doubles between ring slots (3n + padding)
     long long astride;        // complex elements between accumulator rows (n + padding)
     int ring, cur;            // ring slots; slot that holds E at chunk start
-    double *h;                // H [3][n], updated in place
+    double *h;                // H [3][n]: updated in place, except by the fused E+H
+                              // launches (see hsel)
+    double *h2;               // second H buffer of shots that run the fused E+H interior
+                              // kernel (null: none)
+    int hpar;                 // 1: the current H lives in h2
     double *psi[3];           // [PSI_N][slab[a]]
     long long slab[3];
     int src_comp;
@@ -88,6 +92,17 @@ struct ShotDev {
 };
 
 enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
+
+// H before local step j of a chunk.  In a launch sequence that updates H in
+// place (hf = 0) that is always the current buffer.  The fused E+H launches
+// (hf = 1) read H from one buffer and write the other, because the halo of a
+// fused tile reads old H that neighbouring tiles overwrite: step j then reads
+// hsel(S, j, 1) and writes hsel(S, j + 1, 1).  Cells no step updates hold the
+// same value in both buffers.
+__host__ __device__ inline double *hsel(const ShotDev &S, int j, int hf)
+{
+    return ((S.hpar + (hf ? j : 0)) & 1) ? S.h2 : S.h;
+}
 
 // Arguments of one adjoint step (fdtd_adjoint.hip), passed by value.
 struct AdjArgs {
@@ -134,11 +149,13 @@ hipError_t launch_efield_z(const ShotDev *d, int ns, const int *pre, int nblocks
                            int sign, hipStream_t st);
 hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
-// rec_prev (j > 0): ns more blocks gather the receivers of local step j - 1
+// rec_prev (j > 0): ns more blocks gather the receivers of local step j - 1.
+// hf: part of a fused E+H step (H read from hsel(S, j, 1), written to
+// hsel(S, j + 1, 1)); 0: H updated in place.
 hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
-                           int sign, hipStream_t st, bool rec_prev = false);
+                           int sign, hipStream_t st, bool rec_prev = false, int hf = 0);
 hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
-                           int sign, hipStream_t st);
+                           int sign, hipStream_t st, int hf = 0);
 hipError_t launch_efield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_hfield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
@@ -147,6 +164,11 @@ hipError_t launch_efield_l(const ShotDev *d, int ns, const int *pre, int nblocks
                            int sign, hipStream_t st);
 hipError_t launch_hfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
+// Fused E+H step over the LDS region (same blocks as launch_efield_l): after
+// launch_efield_f(hf = 1), before launch_hfield_f(hf = 1); shots with h2 and a
+// ring of more than one slot
+hipError_t launch_ehfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                            hipStream_t st);
 // Fortran-variant step (fdtd_fortran.hip): Hz source points before the E
 // update, then per-cell E + CPML-E and H + CPML-H over 32 x 8 tiles
 hipError_t launch_source_hz(const ShotDev *d, int ns, int j, hipStream_t st);
@@ -155,14 +177,14 @@ hipError_t launch_efield_v(const ShotDev *d, int ns, const int *pre, int nblocks
 hipError_t launch_hfield_v(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            hipStream_t st);
 hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
-                            hipStream_t st);
+                            hipStream_t st, int hf = 0);
 // h: the H ring into acc_h (shots with H sums) instead of the E ring into acc
 hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int L,
                         hipStream_t st, bool h = false);
-hipError_t launch_advance(const ShotDev *d, int ns, int L, hipStream_t st);
+hipError_t launch_advance(const ShotDev *d, int ns, int L, hipStream_t st, int hf = 0);
 // H after local step j's H update into its H ring slot (shots with hring)
 hipError_t launch_hstore(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
-                         hipStream_t st);
+                         hipStream_t st, int hf = 0);
 hipError_t launch_receiver_dft(const double *traces, int ncomp_rows, int it, double dt,
                                double omega0, int nfreq, const double *omegas,
                                double *out, hipStream_t st);

@@ -72,8 +72,8 @@ __device__ __forceinline__ int slab_pos(int r, int n, int nc)
 
 // One cell of the E update + CPML-E corrections + source, general path.
 template <int MODE>
-__device__ __forceinline__ void efield_cell(const ShotDev &S, int i, int jj, int k, int n, int si,
-                                            int so, int sign)
+__device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, int i, int jj, int k,
+                                            int n, int si, int so, int sign)
 {
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc, N = S.n;
     const int sy = ix, sz = ix * iy;
@@ -81,9 +81,9 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, int i, int jj, int
 
     const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
     GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
-    const GLOBAL double *__restrict__ Hx = gc(S.h);
-    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
-    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * (size_t)N);
+    const GLOBAL double *__restrict__ Hx = gc(H);
+    const GLOBAL double *__restrict__ Hy = gc(H + N);
+    const GLOBAL double *__restrict__ Hz = gc(H + 2 * (size_t)N);
     double ex = Ein[c], ey = Ein[N + c], ez = Ein[2 * (size_t)N + c];
 
     const double c1 = S.c1, c2 = S.c2, dt = S.dt;
@@ -223,12 +223,14 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
         si = (S.cur + j) % S.ring;
         so = (S.cur + j + 1) % S.ring;
     }
-    efield_cell<MODE>(S, i, jj, k, n, si, so, sign);
+    efield_cell<MODE>(S, hsel(S, 0, 0), i, jj, k, n, si, so, sign);
 }
 
 // One cell of the H update + CPML-H corrections, general path.
+// H is read from Hin and written to Hout (the same buffer when updated in place).
 template <int MODE>
-__device__ __forceinline__ void hfield_cell(const ShotDev &S, int i, int jj, int k, int so, int sign)
+__device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin, double *Hout, int i,
+                                            int jj, int k, int so, int sign)
 {
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc, N = S.n;
     const int sy = ix, sz = ix * iy;
@@ -237,9 +239,6 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, int i, int jj, int
     const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
     const GLOBAL double *__restrict__ Ey = gc(Ex + N);
     const GLOBAL double *__restrict__ Ez = gc(Ex + 2 * (size_t)N);
-    GLOBAL double *__restrict__ Hx = gm(S.h);
-    GLOBAL double *__restrict__ Hy = gm(S.h + N);
-    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * (size_t)N);
 
     const double c1 = S.c1, c2 = S.c2, dt = S.dt;
     const GLOBAL double *__restrict__ px = gc(S.prof[0]);
@@ -293,7 +292,7 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, int i, int jj, int
     const double dbx = dt / S.mu0 / (1.0 + smx);
     const double dby = dt / S.mu0 / (1.0 + smy);
     const double dbz = dt / S.mu0 / (1.0 + smz);
-    double hx = Hx[c], hy = Hy[c], hz = Hz[c];
+    double hx = Hin[c], hy = Hin[N + c], hz = Hin[2 * (size_t)N + c];
 
     if (upx) {
         const double da = (1.0 - smx) / (1.0 + smx);
@@ -346,9 +345,10 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, int i, int jj, int
         hx += dbx * p0;
         hy -= dby * p1;
     }
-    Hx[c] = hx;
-    Hy[c] = hy;
-    Hz[c] = hz;
+    GLOBAL double *__restrict__ Ho = gm(Hout);
+    Ho[c] = hx;
+    Ho[N + c] = hy;
+    Ho[2 * (size_t)N + c] = hz;
 }
 
 
@@ -372,7 +372,8 @@ __global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D
     const int i = bx * TX + threadIdx.x, jj = by * TY + threadIdx.y;
     if (i >= S.ix || jj >= S.iy)
         return;
-    hfield_cell<MODE>(S, i, jj, k, so, sign);
+    double *H = hsel(S, 0, 0);
+    hfield_cell<MODE>(S, H, H, i, jj, k, so, sign);
 }
 
 // read_source_3d as a stand-alone operator (in place on the current slot)
@@ -385,23 +386,25 @@ __global__ void k_source(const ShotDev *__restrict__ D, int n)
     E[S.src_cell] -= gc(S.amp)[n];
 }
 
-// Trace samples of step n: E from ring slot so and H, one block per shot.
-__device__ __forceinline__ void rec_store(const ShotDev &S, int n, int so)
+// Trace samples of step n: E from ring slot so and H from buffer H, one block
+// per shot.
+__device__ __forceinline__ void rec_store(const ShotDev &S, int n, int so, const double *H)
 {
     const int tot = 6 * S.nrec;
     for (int q = threadIdx.x; q < tot; q += blockDim.x) {
         const int comp = q / S.nrec, r = q - comp * S.nrec;
         const int cell = S.rec_cell[r];
         const double v = comp < 3 ? S.e[(size_t)so * S.sstride + (size_t)comp * S.n + cell]
-                                  : S.h[(size_t)(comp - 3) * S.n + cell];
+                                  : H[(size_t)(comp - 3) * S.n + cell];
         S.traces[((size_t)comp * S.nrec + r) * S.it + n] = v;
         if (!__builtin_isfinite(v))
             *S.flag = 1;
     }
 }
 
-// Receiver gather: E (from the slot written this step) and H after the step.
-__global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int inplace)
+// Receiver gather: E (from the slot written this step) and H after the step
+// (hf: the step was a fused E+H launch sequence, see hsel).
+__global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int inplace, int hf)
 {
     const ShotDev &S = D[blockIdx.x];
     int n = nover, so = S.cur;
@@ -412,7 +415,7 @@ __global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int
     }
     if (!inplace)
         so = (S.cur + j + 1) % S.ring;
-    rec_store(S, n, so);
+    rec_store(S, n, so, inplace ? hsel(S, 0, 0) : hsel(S, j + 1, hf));
 }
 
 // Volume running DFT over the last L ring slots: acc += E[n] * w[n] in step
@@ -629,7 +632,7 @@ __global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__
 // sums the chunk's slots with the E sums' weights, each accumulator in
 // ascending step order, so the sums equal per-step running sums bit for bit.
 __global__ __launch_bounds__(256) void k_hstore(const ShotDev *__restrict__ D, int ns,
-                                                const int *__restrict__ pre, int j)
+                                                const int *__restrict__ pre, int j, int hf)
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(256) void k_hstore(const ShotDev *__restrict__ D, i
         return;
     const size_t N = (size_t)S.n;
     GLOBAL double *__restrict__ dst = gm(S.hring + (size_t)((S.cur + j + 1) % S.ring) * S.sstride);
-    const GLOBAL double *__restrict__ H = gc(S.h);
+    const GLOBAL double *__restrict__ H = gc(hsel(S, j + 1, hf));
     const int base = (blockIdx.x - pre[s]) * FLUSH_CELLS;
     for (int c = base + threadIdx.x; c < base + FLUSH_CELLS && c < S.n; c += blockDim.x) {
         dst[c] = H[c];
@@ -648,18 +651,18 @@ __global__ __launch_bounds__(256) void k_hstore(const ShotDev *__restrict__ D, i
 }
 
 hipError_t launch_hstore(const ShotDev *d, int ns, const int *pre_cells, int nblocks, int j,
-                         hipStream_t st)
+                         hipStream_t st, int hf)
 {
     if (nblocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_hstore, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, j);
+    hipLaunchKernelGGL(k_hstore, dim3(nblocks), dim3(256), 0, st, d, ns, pre_cells, j, hf);
     return hipGetLastError();
 }
 
 // Step counters forward by L; then the divergence probe: E (the slot the
 // chunk ended in) and H at the source cell (the first Hz point of a
 // Fortran-variant shot) must be finite.
-__global__ void k_advance(const ShotDev *__restrict__ D, int ns, int L)
+__global__ void k_advance(const ShotDev *__restrict__ D, int ns, int L, int hf)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns)
@@ -670,9 +673,10 @@ __global__ void k_advance(const ShotDev *__restrict__ D, int ns, int L)
     if (c < 0)
         return;
     const double *E = S.e + (size_t)((S.cur + L) % S.ring) * S.sstride;
+    const double *H = hsel(S, L, hf);
     double v = 0.0;
     for (int d = 0; d < 3; ++d)
-        v += __builtin_fabs(E[(size_t)d * S.n + c]) + __builtin_fabs(S.h[(size_t)d * S.n + c]);
+        v += __builtin_fabs(E[(size_t)d * S.n + c]) + __builtin_fabs(H[(size_t)d * S.n + c]);
     if (!__builtin_isfinite(v))
         *S.flag = 1;
 }
@@ -789,9 +793,10 @@ __global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_efield_z(const ShotDev *
     const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
     const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
     GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
-    const GLOBAL double *__restrict__ Hx = gc(S.h);
-    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
-    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * N);
+    const double *Hc = hsel(S, 0, 0);
+    const GLOBAL double *__restrict__ Hx = gc(Hc);
+    const GLOBAL double *__restrict__ Hy = gc(Hc + N);
+    const GLOBAL double *__restrict__ Hz = gc(Hc + 2 * N);
     const double c1 = S.c1, c2 = S.c2, dt = S.dt;
     const GLOBAL double *__restrict__ px = gc(S.prof[0]);
     const GLOBAL double *__restrict__ py = gc(S.prof[1]);
@@ -944,9 +949,10 @@ __global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_hfield_z(const ShotDev *
     const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
     const GLOBAL double *__restrict__ Ey = gc(Ex + N);
     const GLOBAL double *__restrict__ Ez = gc(Ex + 2 * N);
-    GLOBAL double *__restrict__ Hx = gm(S.h);
-    GLOBAL double *__restrict__ Hy = gm(S.h + N);
-    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * N);
+    double *Hc = hsel(S, 0, 0);
+    GLOBAL double *__restrict__ Hx = gm(Hc);
+    GLOBAL double *__restrict__ Hy = gm(Hc + N);
+    GLOBAL double *__restrict__ Hz = gm(Hc + 2 * N);
     const double c1 = S.c1, c2 = S.c2, dt = S.dt;
     const double dbi = dt / S.mu0 / 1.0;   // db where the magnetic CPML sigma is zero
     const GLOBAL double *__restrict__ px = gc(S.prof[0]);
@@ -1138,17 +1144,18 @@ __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, 
 // step j - 1, block s for shot s: its E slot is the one this step reads and H
 // is not written until the H half-step, so the samples are the ones a receiver
 // launch after step j - 1 takes, with one graph node less per step.  Leading,
-// they overlap the tile blocks instead of trailing the launch.
+// they overlap the tile blocks instead of trailing the launch.  hf: the step
+// is a fused E+H launch sequence (H before step j is hsel(S, j, 1)).
 __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__ D, int ns,
                                                      const int *__restrict__ pre, int j, int sign,
-                                                     int recn)
+                                                     int recn, int hf)
 {
     if ((int)blockIdx.x < recn) {
         if ((int)blockIdx.x < ns) {
             const ShotDev &R = D[blockIdx.x];
             const int n = *R.step + j - 1;
             if (n < R.it - 1)
-                rec_store(R, n, (R.cur + j) % R.ring);
+                rec_store(R, n, (R.cur + j) % R.ring, hsel(R, j, hf));
         }
         return;
     }
@@ -1164,8 +1171,9 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
     const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
     const bool inner = i >= nc && i < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
+    const double *Hc = hsel(S, j, hf);
     if (!__all(inner)) {
-        efield_cell<M_UPD | M_PSI | M_SRC>(S, i, jj, k, n, si, so, sign);
+        efield_cell<M_UPD | M_PSI | M_SRC>(S, Hc, i, jj, k, n, si, so, sign);
         return;
     }
     const size_t N = (size_t)S.n;
@@ -1173,9 +1181,9 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
     const int c = (k * iy + jj) * ix + i;
     const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
     GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
-    const GLOBAL double *__restrict__ Hx = gc(S.h);
-    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
-    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * N);
+    const GLOBAL double *__restrict__ Hx = gc(Hc);
+    const GLOBAL double *__restrict__ Hy = gc(Hc + N);
+    const GLOBAL double *__restrict__ Hz = gc(Hc + 2 * N);
     const double ex0 = Ein[c], ey0 = Ein[N + c], ez0 = Ein[2 * N + c], cb = gc(S.cbi)[c];
     const double z0 = Hz[c], zym1 = Hz[c - sy], zyp1 = Hz[c + sy], zym2 = Hz[c - 2 * sy];
     const double zxm1 = Hz[c - 1], zxp1 = Hz[c + 1], zxm2 = Hz[c - 2];
@@ -1214,7 +1222,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
 }
 
 __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__ D, int ns,
-                                                     const int *__restrict__ pre, int j, int sign)
+                                                     const int *__restrict__ pre, int j, int sign,
+                                                     int hf)
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
@@ -1226,8 +1235,9 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc;
     const int so = (S.cur + j + 1) % S.ring;
     const bool inner = i >= nc && i < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
+    double *Hi = hsel(S, j, hf), *Ho = hsel(S, j + 1, hf);
     if (!__all(inner)) {
-        hfield_cell<M_UPD | M_PSI>(S, i, jj, k, so, sign);
+        hfield_cell<M_UPD | M_PSI>(S, Hi, Ho, i, jj, k, so, sign);
         return;
     }
     const size_t N = (size_t)S.n;
@@ -1236,10 +1246,13 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
     const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
     const GLOBAL double *__restrict__ Ey = gc(Ex + N);
     const GLOBAL double *__restrict__ Ez = gc(Ex + 2 * N);
-    GLOBAL double *__restrict__ Hx = gm(S.h);
-    GLOBAL double *__restrict__ Hy = gm(S.h + N);
-    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * N);
-    const double hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
+    const GLOBAL double *__restrict__ Hxi = gc(Hi);
+    const GLOBAL double *__restrict__ Hyi = gc(Hi + N);
+    const GLOBAL double *__restrict__ Hzi = gc(Hi + 2 * N);
+    GLOBAL double *__restrict__ Hx = gm(Ho);
+    GLOBAL double *__restrict__ Hy = gm(Ho + N);
+    GLOBAL double *__restrict__ Hz = gm(Ho + 2 * N);
+    const double hx0 = Hxi[c], hy0 = Hyi[c], hz0 = Hzi[c];
     const double z0 = Ez[c], zyp1 = Ez[c + sy], zyp2 = Ez[c + 2 * sy], zym1 = Ez[c - sy];
     const double zxp1 = Ez[c + 1], zxp2 = Ez[c + 2], zxm1 = Ez[c - 1];
     const double y0 = Ey[c], yzp1 = Ey[c + sz], yzp2 = Ey[c + 2 * sz], yzm1 = Ey[c - sz];
@@ -1266,22 +1279,22 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
 }
 
 hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
-                           hipStream_t st, bool rec_prev)
+                           hipStream_t st, bool rec_prev, int hf)
 {
     if (nblocks <= 0)
         return rec_prev ? hipErrorInvalidValue : hipSuccess;
     const int recn = rec_prev && j > 0 ? (ns + 7) & ~7 : 0;
     hipLaunchKernelGGL(k_efield_f, dim3(recn + nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign,
-                       recn);
+                       recn, hf);
     return hipGetLastError();
 }
 
 hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
-                           hipStream_t st)
+                           hipStream_t st, int hf)
 {
     if (nblocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_hfield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign);
+    hipLaunchKernelGGL(k_hfield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign, hf);
     return hipGetLastError();
 }
 
@@ -1380,7 +1393,8 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
     const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
     const GLOBAL double *Ein = gc(S.e + (size_t)si * S.sstride);
     GLOBAL double *Eout = gm(S.e + (size_t)so * S.sstride);
-    const GLOBAL double *Hx = gc(S.h), *Hy = gc(S.h + N), *Hz = gc(S.h + 2 * N);
+    const double *Hc = hsel(S, 0, 0);
+    const GLOBAL double *Hx = gc(Hc), *Hy = gc(Hc + N), *Hz = gc(Hc + 2 * N);
     const GLOBAL double *CB = gc(S.cbi);
     const double kedx = gc(S.prof[0])[PR_KED * ix + ci];
     const double kedy = gc(S.prof[1])[PR_KED * iy + cj];
@@ -1496,7 +1510,8 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
     const int so = (S.cur + j + 1) % S.ring;
     const GLOBAL double *Ex = gc(S.e + (size_t)so * S.sstride);
     const GLOBAL double *Ey = Ex + N, *Ez = Ex + 2 * N;
-    GLOBAL double *Hx = gm(S.h), *Hy = gm(S.h + N), *Hz = gm(S.h + 2 * N);
+    double *Hc = hsel(S, 0, 0);
+    GLOBAL double *Hx = gm(Hc), *Hy = gm(Hc + N), *Hz = gm(Hc + 2 * N);
     const double khdx = gc(S.prof[0])[PR_KHD * ix + ci];
     const double khdy = gc(S.prof[1])[PR_KHD * iy + cj];
     const double c1 = S.c1, c2 = S.c2;
@@ -1594,6 +1609,335 @@ hipError_t launch_hfield_l(const ShotDev *d, int ns, const int *pre, int nblocks
 
 
 // ------------------------------------------------------------------------
+// Fused E+H z-march over the LDS region: one launch per step in place of
+// k_efield_l + k_hfield_l.  Plane p's E is computed, then H of plane p - 2,
+// the plane whose forward z stencil E(p) completes.  The H update of a tile
+// needs new E one cell before and two cells after it along x and y, in cells
+// that neighbouring blocks own: the block recomputes E there from old H (halo
+// items), or reads it from the E slot where the cell lies outside the region
+// (the shell launch before this one has written it).  The same holds for the
+// planes k0 - 1, k1 and k1 + 1 around a z-chunk.  Old H is read from
+// hsel(S, j, 1) and new H written to hsel(S, j + 1, 1), and E goes from ring
+// slot si to slot so, so no block overwrites a value another block still
+// reads.  Per step this moves E, H and dt/eps' in and E and H out (104 B per
+// cell) instead of the 152 B of the two launches.
+//
+// Work items: lane t owns the tile cell (t % LX, t / LX) and, for t < FB_N,
+// one halo cell: first the rows -1, LY, LY + 1 of the tile's columns (their
+// Ez and Ex are needed), then the columns -1, LX, LX + 1 of its rows (Ez and
+// Ey).  The arithmetic is the per-cell fast path term for term, so every value
+// is bitwise the two-launch schedule's.  On the region kappa = 1, so ked and
+// khd are one value per axis (the host checks this before it selects the
+// kernel).  LDS: old H of plane p, and new E of planes p, p - 1 and p - 2; two
+// barriers per plane.
+
+namespace {
+
+constexpr int FB_N = 3 * LX + 3 * LY;
+// old-H tiles of a plane, as offsets (u, v) from the tile origin
+constexpr int FZW = LX + 6, FZH = LY + 6;   // Hz: u in [-3, LX + 3), v in [-3, LY + 3)
+constexpr int FYW = LX + 6, FYH = LY + 3;   // Hy: u in [-3, LX + 3), v in [-1, LY + 2)
+constexpr int FXW = LX + 3, FXH = LY + 6;   // Hx: u in [-1, LX + 2), v in [-3, LY + 3)
+constexpr int FZN = FZW * FZH, FYN = FYW * FYH, FXN = FXW * FXH;
+constexpr int FHN = FZN + FYN + FXN;
+constexpr int FHR = (FHN + LNT - 1) / LNT;
+// new-E tiles of a plane
+constexpr int GZW = LX + 3, GZH = LY + 3;   // Ez: u in [-1, LX + 2), v in [-1, LY + 2)
+constexpr int GYW = LX + 3, GYH = LY;       // Ey: u in [-1, LX + 2), v in [0, LY)
+constexpr int GXW = LX, GXH = LY + 3;       // Ex: u in [0, LX), v in [-1, LY + 2)
+constexpr int GZN = GZW * GZH, GYN = GYW * GYH, GXN = GXW * GXH;
+constexpr int GEN = GZN + GYN + GXN;
+static_assert(FB_N <= LNT, "halo items exceed the block");
+
+__device__ __forceinline__ int fz_at(int u, int v) { return (v + 3) * FZW + u + 3; }
+__device__ __forceinline__ int fy_at(int u, int v) { return FZN + (v + 1) * FYW + u + 3; }
+__device__ __forceinline__ int fx_at(int u, int v) { return FZN + FYN + (v + 3) * FXW + u + 1; }
+__device__ __forceinline__ int gz_at(int u, int v) { return (v + 1) * GZW + u + 1; }
+__device__ __forceinline__ int gy_at(int u, int v) { return GZN + v * GYW + u + 1; }
+__device__ __forceinline__ int gx_at(int u, int v) { return GZN + GYN + (v + 1) * GXW + u; }
+
+}  // namespace
+
+__global__ __launch_bounds__(LNT, 4) void k_ehfield_l(const ShotDev *__restrict__ D, int ns,
+                                                      const int *__restrict__ pre, int j)
+{
+    __shared__ double sh[FHN], se[3][GEN];
+    const int s = find_shot(pre, ns, blockIdx.x);
+    const ShotDev &S = D[s];
+    const int n = *S.step + j;
+    if (n >= S.it - 1)
+        return;
+    int x0, y0, k0, k1;
+    if (!lcol(S, blockIdx.x - pre[s], x0, y0, k0, k1))
+        return;
+    const int ix = S.ix, iy = S.iy, iz = S.iz;
+    const size_t N = (size_t)S.n, pl = (size_t)ix * iy;
+    const int lo2 = S.lo[2], hi2 = S.hi[2];
+    const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
+    const GLOBAL double *Ein = gc(S.e + (size_t)si * S.sstride);
+    GLOBAL double *Eout = gm(S.e + (size_t)so * S.sstride);
+    const GLOBAL double *Hin = gc(hsel(S, j, 1));
+    GLOBAL double *Hout = gm(hsel(S, j + 1, 1));
+    const GLOBAL double *CB = gc(S.cbi);
+    const double c1 = S.c1, c2 = S.c2;
+    const double db = S.dt / S.mu0 / 1.0;
+    const double kedx = gc(S.prof[0])[PR_KED * ix + S.lo[0]];
+    const double kedy = gc(S.prof[1])[PR_KED * iy + S.lo[1]];
+    const double kedz = gc(S.prof[2])[PR_KED * iz + lo2];
+    const double khdx = gc(S.prof[0])[PR_KHD * ix + S.lo[0]];
+    const double khdy = gc(S.prof[1])[PR_KHD * iy + S.lo[1]];
+    const double khdz = gc(S.prof[2])[PR_KHD * iz + lo2];
+    const int t = threadIdx.x;
+
+    // old-H tile elements this lane loads: offset within a plane (component
+    // included; the host keeps 3 N below 2^31), -1 outside the grid
+    int ho[FHR];
+#pragma unroll
+    for (int q = 0; q < FHR; ++q) {
+        int e = t + q * LNT, comp, u, v;
+        ho[q] = -1;
+        if (e >= FHN)
+            continue;
+        if (e < FZN) {
+            comp = 2, u = e % FZW - 3, v = e / FZW - 3;
+        } else if (e < FZN + FYN) {
+            e -= FZN;
+            comp = 1, u = e % FYW - 3, v = e / FYW - 1;
+        } else {
+            e -= FZN + FYN;
+            comp = 0, u = e % FXW - 1, v = e / FXW - 3;
+        }
+        const int gx = x0 + u, gy = y0 + v;
+        if (gx >= 0 && gx < ix && gy >= 0 && gy < iy)
+            ho[q] = comp * S.n + gy * ix + gx;
+    }
+
+    // tile item (A): computes and owns E and H where it lies in the region
+    const int uA = t % LX, vA = t / LX;
+    const bool ingA = x0 + uA < ix && y0 + vA < iy;
+    const bool own = x0 + uA < S.hi[0] && y0 + vA < S.hi[1];
+    const int colA = ingA ? (y0 + vA) * ix + x0 + uA : 0;
+    // halo item (B): yh on the halo rows (Ez, Ex), else on the halo columns (Ez, Ey)
+    const bool hasB = t < FB_N, yh = t < 3 * LX;
+    int uB, vB;
+    if (yh) {
+        const int r = t / LX;
+        uB = t % LX;
+        vB = r == 0 ? -1 : LY - 1 + r;
+    } else {
+        const int m = t - 3 * LX, c3 = m % 3;
+        uB = c3 == 0 ? -1 : LX - 1 + c3;
+        vB = m / 3;
+    }
+    const int gxB = x0 + uB, gyB = y0 + vB;
+    const bool ingB = hasB && gxB >= 0 && gxB < ix && gyB >= 0 && gyB < iy;
+    const bool inrB = ingB && gxB >= S.lo[0] && gxB < S.hi[0] && gyB >= S.lo[1] && gyB < S.hi[1];
+    const int colB = ingB ? gyB * ix + gxB : 0;
+    const int dB = yh ? 0 : 1;                     // B's second component
+    const size_t qoB = yh ? N : 0;                 // B's z queue: Hy on the rows, Hx on the columns
+    const int e2B = yh ? gx_at(uB, vB) : gy_at(uB, vB);
+
+    const int pa = k0 - 1, pb = k1 + 1;
+    // own-column old H at planes p - 2, p - 1 and p + 1 (plane p comes from LDS)
+    double qyA[3], qxA[3], qB[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int pz = pa - 2 + (r < 2 ? r : 3);
+        const bool okz = pz >= 0 && pz < iz;
+        qyA[r] = ingA && okz ? Hin[N + (size_t)pz * pl + colA] : 0.0;
+        qxA[r] = ingA && okz ? Hin[(size_t)pz * pl + colA] : 0.0;
+        qB[r] = ingB && okz ? Hin[qoB + (size_t)pz * pl + colB] : 0.0;
+    }
+    // E(p) inputs: old E and dt/eps' where the item computes, else the new E
+    // the shell launch wrote
+    double eA[3], eBz, eB2, cbA, cbB;
+    double rh[FHR];
+    {
+        const bool z = pa >= lo2 && pa < hi2;
+        const bool cA = own && z, cB = inrB && z;
+        const size_t cAi = (size_t)pa * pl + colA, cBi = (size_t)pa * pl + colB;
+        const GLOBAL double *sA = cA ? Ein : Eout, *sB = cB ? Ein : Eout;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            eA[d] = ingA ? sA[d * N + cAi] : 0.0;
+        cbA = cA ? CB[cAi] : 0.0;
+        eBz = ingB ? sB[2 * N + cBi] : 0.0;
+        eB2 = ingB ? sB[dB * N + cBi] : 0.0;
+        cbB = cB ? CB[cBi] : 0.0;
+#pragma unroll
+        for (int q = 0; q < FHR; ++q)
+            rh[q] = ho[q] >= 0 ? Hin[(size_t)pa * pl + ho[q]] : 0.0;
+    }
+    double d3x = 0.0, d3y = 0.0;              // new Ex, Ey of plane p - 3 at the tile cell
+
+    for (int p = pa; p <= pb; ++p) {
+        const bool zc = p >= lo2 && p < hi2;   // E computed (not read) at plane p
+#pragma unroll
+        for (int q = 0; q < FHR; ++q) {
+            const int e = t + q * LNT;
+            if (e < FHN)
+                sh[e] = rh[q];
+        }
+        __syncthreads();   // sh holds plane p; se[p % 3] is free
+
+        double *eb = se[p % 3];
+        // E(p) at the tile cell
+        const double yA = sh[fy_at(uA, vA)], xA = sh[fx_at(uA, vA)];
+        double ex = eA[0], ey = eA[1], ez = eA[2];
+        if (zc && own) {
+            const double *Z = sh + fz_at(uA, vA);
+            const double z0 = Z[0], zym1 = Z[-FZW], zyp1 = Z[FZW], zym2 = Z[-2 * FZW];
+            const double zxm1 = Z[-1], zxp1 = Z[1], zxm2 = Z[-2];
+            const double *Y = sh + fy_at(uA, vA);
+            const double yxm1 = Y[-1], yxp1 = Y[1], yxm2 = Y[-2];
+            const double *X = sh + fx_at(uA, vA);
+            const double xym1 = X[-FXW], xyp1 = X[FXW], xym2 = X[-2 * FXW];
+            const double hzy = c1 * z0 - c1 * zym1 + c2 * zyp1 - c2 * zym2;
+            const double hyz = c1 * yA - c1 * qyA[1] + c2 * qyA[2] - c2 * qyA[0];
+            const double hxz = c1 * xA - c1 * qxA[1] + c2 * qxA[2] - c2 * qxA[0];
+            const double hzx = c1 * z0 - c1 * zxm1 + c2 * zxp1 - c2 * zxm2;
+            const double hyx = c1 * yA - c1 * yxm1 + c2 * yxp1 - c2 * yxm2;
+            const double hxy = c1 * xA - c1 * xym1 + c2 * xyp1 - c2 * xym2;
+            const double cux = hzy / kedy - hyz / kedz;
+            const double cuy = hxz / kedz - hzx / kedx;
+            const double cuz = hyx / kedx - hxy / kedy;
+            ex = eA[0] + cbA * cux;
+            ey = eA[1] + cbA * cuy;
+            ez = eA[2] + cbA * cuz;
+            if ((long long)((size_t)p * pl + colA) == S.src_cell) {
+                const double a = gc(S.amp)[n];
+                if (S.src_comp == 0)
+                    ex -= a;
+                else if (S.src_comp == 1)
+                    ey -= a;
+                else
+                    ez -= a;
+            }
+        }
+        eb[gz_at(uA, vA)] = ez;
+        eb[gy_at(uA, vA)] = ey;
+        eb[gx_at(uA, vA)] = ex;
+        if (own && p >= k0 && p < k1) {
+            const size_t c = (size_t)p * pl + colA;
+            Eout[c] = ex;
+            Eout[N + c] = ey;
+            Eout[2 * N + c] = ez;
+        }
+        // E(p) at the halo cell
+        double ownB = 0.0;
+        if (hasB) {
+            ownB = sh[yh ? fy_at(uB, vB) : fx_at(uB, vB)];
+            double ezb = eBz, e2b = eB2;
+            if (zc && inrB) {
+                const double *Y = sh + fy_at(uB, vB);
+                const double *X = sh + fx_at(uB, vB);
+                const double *Z = sh + fz_at(uB, vB);
+                const double hyx = c1 * Y[0] - c1 * Y[-1] + c2 * Y[1] - c2 * Y[-2];
+                const double hxy = c1 * X[0] - c1 * X[-FXW] + c2 * X[FXW] - c2 * X[-2 * FXW];
+                const double cuz = hyx / kedx - hxy / kedy;
+                ezb = eBz + cbB * cuz;
+                if (yh) {   // Ex
+                    const double hzy = c1 * Z[0] - c1 * Z[-FZW] + c2 * Z[FZW] - c2 * Z[-2 * FZW];
+                    const double hyz = c1 * ownB - c1 * qB[1] + c2 * qB[2] - c2 * qB[0];
+                    const double cux = hzy / kedy - hyz / kedz;
+                    e2b = eB2 + cbB * cux;
+                } else {    // Ey
+                    const double hxz = c1 * ownB - c1 * qB[1] + c2 * qB[2] - c2 * qB[0];
+                    const double hzx = c1 * Z[0] - c1 * Z[-1] + c2 * Z[1] - c2 * Z[-2];
+                    const double cuy = hxz / kedz - hzx / kedx;
+                    e2b = eB2 + cbB * cuy;
+                }
+                if ((long long)((size_t)p * pl + colB) == S.src_cell) {
+                    const double a = gc(S.amp)[n];
+                    if (S.src_comp == 2)
+                        ezb -= a;
+                    else if (S.src_comp == dB)
+                        e2b -= a;
+                }
+            }
+            eb[gz_at(uB, vB)] = ezb;
+            eb[e2B] = e2b;
+        }
+        __syncthreads();   // se[p % 3] holds plane p; sh is free
+
+        // queues forward one plane; the old-H tile of plane p + 1 is fetched
+        // while H of plane p - 2 is computed, the rest after it
+        qyA[0] = qyA[1];
+        qyA[1] = yA;
+        qxA[0] = qxA[1];
+        qxA[1] = xA;
+        qB[0] = qB[1];
+        qB[1] = ownB;
+        const bool more = p < pb;
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < FHR; ++q)
+                rh[q] = ho[q] >= 0 ? Hin[(size_t)(p + 1) * pl + ho[q]] : 0.0;
+        }
+
+        // H of plane q = p - 2 at the tile cell: E(q) from se[q % 3], E(q + 1)
+        // from se[(q + 1) % 3], E(q + 2) = this plane's, E(q - 1) from d3
+        if (own && p - 2 >= k0) {
+            const size_t c = (size_t)(p - 2) * pl + colA;
+            const double hA0 = Hin[c], hA1 = Hin[N + c], hA2 = Hin[2 * N + c];
+            const double *E0 = se[(p - 2) % 3], *E1 = se[(p - 1) % 3];
+            const double *Z = E0 + gz_at(uA, vA);
+            const double z0 = Z[0], zyp1 = Z[GZW], zyp2 = Z[2 * GZW], zym1 = Z[-GZW];
+            const double zxp1 = Z[1], zxp2 = Z[2], zxm1 = Z[-1];
+            const double *Y = E0 + gy_at(uA, vA);
+            const double y0v = Y[0], yxp1 = Y[1], yxp2 = Y[2], yxm1 = Y[-1];
+            const double *X = E0 + gx_at(uA, vA);
+            const double x0v = X[0], xyp1 = X[GXW], xyp2 = X[2 * GXW], xym1 = X[-GXW];
+            const double yzp1 = E1[gy_at(uA, vA)], xzp1 = E1[gx_at(uA, vA)];
+            const double ezy = c1 * zyp1 - c1 * z0 + c2 * zyp2 - c2 * zym1;
+            const double eyz = c1 * yzp1 - c1 * y0v + c2 * ey - c2 * d3y;
+            const double exz = c1 * xzp1 - c1 * x0v + c2 * ex - c2 * d3x;
+            const double ezx = c1 * zxp1 - c1 * z0 + c2 * zxp2 - c2 * zxm1;
+            const double eyx = c1 * yxp1 - c1 * y0v + c2 * yxp2 - c2 * yxm1;
+            const double exy = c1 * xyp1 - c1 * x0v + c2 * xyp2 - c2 * xym1;
+            const double cux = ezy / khdy - eyz / khdz;
+            const double cuy = exz / khdz - ezx / khdx;
+            const double cuz = eyx / khdx - exy / khdy;
+            Hout[c] = hA0 - db * cux;
+            Hout[N + c] = hA1 - db * cuy;
+            Hout[2 * N + c] = hA2 - db * cuz;
+        }
+        if (more) {
+            const int pn = p + 1, pz = p + 2;
+            const bool z = pn >= lo2 && pn < hi2;
+            const bool cA = own && z, cB = inrB && z;
+            const size_t cAi = (size_t)pn * pl + colA, cBi = (size_t)pn * pl + colB;
+            const GLOBAL double *sA = cA ? Ein : Eout, *sB = cB ? Ein : Eout;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                eA[d] = ingA ? sA[d * N + cAi] : 0.0;
+            cbA = cA ? CB[cAi] : 0.0;
+            eBz = ingB ? sB[2 * N + cBi] : 0.0;
+            eB2 = ingB ? sB[dB * N + cBi] : 0.0;
+            cbB = cB ? CB[cBi] : 0.0;
+            const bool okz = pz < iz;
+            qyA[2] = ingA && okz ? Hin[N + (size_t)pz * pl + colA] : 0.0;
+            qxA[2] = ingA && okz ? Hin[(size_t)pz * pl + colA] : 0.0;
+            qB[2] = ingB && okz ? Hin[qoB + (size_t)pz * pl + colB] : 0.0;
+        }
+        if (p - 2 >= pa) {
+            d3x = se[(p - 2) % 3][gx_at(uA, vA)];
+            d3y = se[(p - 2) % 3][gy_at(uA, vA)];
+        }
+    }
+}
+
+hipError_t launch_ehfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
+                            hipStream_t st)
+{
+    if (nblocks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_ehfield_l, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j);
+    return hipGetLastError();
+}
+
+
+// ------------------------------------------------------------------------
 // Pair variant of the fused per-cell kernels (even ix): every lane updates the
 // x-pair (i, i+1) with 16-byte loads, halving the load instructions of the
 // stencil.  Tiles are 32 x 16 cells (16 x 16 lanes); same arithmetic, same
@@ -1644,8 +1988,8 @@ __global__ __launch_bounds__(256) void k_efield_p(const ShotDev *__restrict__ D,
     const bool inner =
         i >= nc && i + 1 < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
     if (!__all(inner)) {
-        efield_cell<M_UPD | M_PSI | M_SRC>(S, i, jj, k, n, si, so, sign);
-        efield_cell<M_UPD | M_PSI | M_SRC>(S, i + 1, jj, k, n, si, so, sign);
+        efield_cell<M_UPD | M_PSI | M_SRC>(S, hsel(S, 0, 0), i, jj, k, n, si, so, sign);
+        efield_cell<M_UPD | M_PSI | M_SRC>(S, hsel(S, 0, 0), i + 1, jj, k, n, si, so, sign);
         return;
     }
     const size_t N = (size_t)S.n;
@@ -1653,9 +1997,10 @@ __global__ __launch_bounds__(256) void k_efield_p(const ShotDev *__restrict__ D,
     const size_t c = ((size_t)k * iy + jj) * ix + i;
     const GLOBAL double *__restrict__ Ein = gc(S.e + (size_t)si * S.sstride);
     GLOBAL double *__restrict__ Eout = gm(S.e + (size_t)so * S.sstride);
-    const GLOBAL double *__restrict__ Hx = gc(S.h);
-    const GLOBAL double *__restrict__ Hy = gc(S.h + N);
-    const GLOBAL double *__restrict__ Hz = gc(S.h + 2 * N);
+    const double *Hc = hsel(S, 0, 0);
+    const GLOBAL double *__restrict__ Hx = gc(Hc);
+    const GLOBAL double *__restrict__ Hy = gc(Hc + N);
+    const GLOBAL double *__restrict__ Hz = gc(Hc + 2 * N);
     const double2 ex0 = ld2(Ein, c), ey0 = ld2(Ein, N + c), ez0 = ld2(Ein, 2 * N + c);
     const double2 cb = ld2(gc(S.cbi), c);
     const double2 z0 = ld2(Hz, c), zym1 = ld2(Hz, c - sy), zyp1 = ld2(Hz, c + sy),
@@ -1727,8 +2072,9 @@ __global__ __launch_bounds__(256) void k_hfield_p(const ShotDev *__restrict__ D,
     const bool inner =
         i >= nc && i + 1 < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
     if (!__all(inner)) {
-        hfield_cell<M_UPD | M_PSI>(S, i, jj, k, so, sign);
-        hfield_cell<M_UPD | M_PSI>(S, i + 1, jj, k, so, sign);
+        double *H = hsel(S, 0, 0);
+        hfield_cell<M_UPD | M_PSI>(S, H, H, i, jj, k, so, sign);
+        hfield_cell<M_UPD | M_PSI>(S, H, H, i + 1, jj, k, so, sign);
         return;
     }
     const size_t N = (size_t)S.n;
@@ -1737,9 +2083,10 @@ __global__ __launch_bounds__(256) void k_hfield_p(const ShotDev *__restrict__ D,
     const GLOBAL double *__restrict__ Ex = gc(S.e + (size_t)so * S.sstride);
     const GLOBAL double *__restrict__ Ey = Ex + N;
     const GLOBAL double *__restrict__ Ez = Ex + 2 * N;
-    GLOBAL double *__restrict__ Hx = gm(S.h);
-    GLOBAL double *__restrict__ Hy = gm(S.h + N);
-    GLOBAL double *__restrict__ Hz = gm(S.h + 2 * N);
+    double *Hc = hsel(S, 0, 0);
+    GLOBAL double *__restrict__ Hx = gm(Hc);
+    GLOBAL double *__restrict__ Hy = gm(Hc + N);
+    GLOBAL double *__restrict__ Hz = gm(Hc + 2 * N);
     const double2 hx0 = ld2(Hx, c), hy0 = ld2(Hy, c), hz0 = ld2(Hz, c);
     const double2 z0 = ld2(Ez, c), zyp1 = ld2(Ez, c + sy), zyp2 = ld2(Ez, c + 2 * sy),
                   zym1 = ld2(Ez, c - sy), zxm = ld2(Ez, c - 2), zxp = ld2(Ez, c + 2);
@@ -1842,11 +2189,11 @@ hipError_t launch_source(const ShotDev *d, int n, hipStream_t st)
 }
 
 hipError_t launch_receivers(const ShotDev *d, int ns, int j, int nover, int inplace,
-                            hipStream_t st)
+                            hipStream_t st, int hf)
 {
     if (ns <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_receivers, dim3(ns), dim3(256), 0, st, d, j, nover, inplace);
+    hipLaunchKernelGGL(k_receivers, dim3(ns), dim3(256), 0, st, d, j, nover, inplace, hf);
     return hipGetLastError();
 }
 
@@ -1864,9 +2211,9 @@ hipError_t launch_flush(const ShotDev *d, int ns, const int *pre_cells, int nblo
     return hipGetLastError();
 }
 
-hipError_t launch_advance(const ShotDev *d, int ns, int L, hipStream_t st)
+hipError_t launch_advance(const ShotDev *d, int ns, int L, hipStream_t st, int hf)
 {
-    hipLaunchKernelGGL(k_advance, dim3((ns + 63) / 64), dim3(64), 0, st, d, ns, L);
+    hipLaunchKernelGGL(k_advance, dim3((ns + 63) / 64), dim3(64), 0, st, d, ns, L, hf);
     return hipGetLastError();
 }
 

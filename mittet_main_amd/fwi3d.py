@@ -289,10 +289,10 @@ class Shot:
 
     def profile_steps(self, nsteps: int) -> dict:
         """Mean kernel durations (ms) over nsteps event-bracketed fused steps."""
-        ms = np.zeros(7)
+        ms = np.zeros(9)
         check(_lib.load().fwi3d_profile_steps(self._h, nsteps, _dp(ms)))
-        return dict(zip(("efield", "hfield", "receivers", "flush", "step", "hstore", "hflush"),
-                        ms.tolist()))
+        return dict(zip(("efield", "hfield", "receivers", "flush", "step", "hstore", "hflush",
+                         "ehfield", "fused"), ms.tolist()))
 
     @property
     def step(self) -> int:

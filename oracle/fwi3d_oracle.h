@@ -8,6 +8,13 @@
  * Layout: x fastest, ijk = k*ix*iy + j*ix + i (satan_i/fwi3d_cpu_function_v1.4.c:1200).
  * All arithmetic is FP64 and follows the reference's evaluation order so that a
  * build with -ffp-contract=off reproduces the reference bit for bit.
+ *
+ * Parity unpinned: the C update ranges along each component's own axis,
+ * [1, n-1) for E and H (or_e_field_cpml42 / or_h_field_cpml42,
+ * function.c:1201-1231, 1255-1285).  function.c cannot be built here (it
+ * includes fftw3.h) and no closed form fixes them; everything else is pinned
+ * against the reference's own cpmlabc.c, lattice.c, posiinf.c and ver0040
+ * Fortran builds, or by closed forms and invariants (see tests/).
  */
 #ifndef FWI3D_ORACLE_H
 #define FWI3D_ORACLE_H

@@ -92,14 +92,20 @@ def test_model_create_rejects_bad_grids(bad):
     assert not h.value
 
 
-def test_shot_bytes_accounts_ring_and_accumulators():
+@pytest.mark.parametrize("fuse", ["0", "1"])
+def test_shot_bytes_accounts_ring_and_accumulators(fuse, monkeypatch):
+    monkeypatch.setenv("MITTET_FDTD_FUSE", fuse)
     g = grid()
     n = 40 ** 3
     plain = shot_bytes(g, it=100)
     with_dft = shot_bytes(g, it=100, nf_vol=4, ring=16)
     assert plain > 8 * 6 * n
-    # 15 extra ring slots of E (3 comps) + 4 complex accumulators x 3 comps + weights
-    assert with_dft - plain == 8 * 3 * n * 15 + 16 * 3 * 4 * n + 16 * 4 * 100
+    # 15 extra ring slots of E (3 comps) + 4 complex accumulators x 3 comps + weights;
+    # with MITTET_FDTD_FUSE=1 a ring of more than one slot on the LDS family
+    # (forced on every grid in these tests) runs the fused E+H step, which
+    # holds a second H (3 comps)
+    h2 = 8 * 3 * n if fuse == "1" else 0
+    assert with_dft - plain == 8 * 3 * n * 15 + 16 * 3 * 4 * n + 16 * 4 * 100 + h2
     # H sums add an H ring of the same 16 slots and 4 x 3 complex H accumulators
     with_h = shot_bytes(g, it=100, nf_vol=4, ring=16, volume_dft_h=True)
     assert with_h - with_dft == 8 * 3 * n * 16 + 16 * 3 * 4 * n
