@@ -239,6 +239,18 @@ long long fwi3d_volume_dft_h_bytes(const fwi3d_grid_params *p, const fwi3d_shot_
  * reference's bin k = 0, where it divides by zero) they are NaN. */
 int fwi3d_receiver_dft(fwi3d_shot *s, int nfreq, const double *omegas, int h_factor,
                        double *out);
+/* No counterpart in the reference: fwi3d_receiver_dft for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  The same kernel and the same results,
+ * with the frequencies and the spectra staged in a fixed device array of
+ * FWI3D_NOALLOC_SCRATCH_DOUBLES doubles that each GPU holds until the process
+ * ends, instead of buffers allocated per call.  A request needs
+ * 32 * ceil(nfreq / 32) + 12 * nrec * nfreq doubles (nrec: the shot's
+ * receivers); a larger one is refused with FWI3D_EINVAL.  Calls that overlap
+ * on one GPU share that array and race: the caller serialises them. */
+#define FWI3D_NOALLOC_SCRATCH_DOUBLES (1 << 19)
+int fwi3d_receiver_dft_noalloc(fwi3d_shot *s, int nfreq, const double *omegas, int h_factor,
+                               double *out);
 /* f_to_d_e / f_to_d_h on already-accumulated spectra (host arrays, complex):
  * E passes through; H is multiplied by sqrt(-2*omega0/(i*omega)). */
 int fwi3d_f_to_d_h(double omega0, int nfreq, const double *omegas, long long count,

@@ -77,6 +77,8 @@ _SIGS = {
     "fwi3d_enable_volume_dft_h": (C.c_int, [C.c_void_p]),
     "fwi3d_get_volume_dft_h": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fwi3d_receiver_dft": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    "fwi3d_receiver_dft_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                             C.c_void_p]),
     "fwi3d_f_to_d_h": (C.c_int, [C.c_double, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p]),
     "fwi3d_source_spectrum": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_int,
                                         C.c_void_p, C.c_void_p]),

@@ -1476,6 +1476,34 @@ int fwi3d_get_volume_dft_h(fwi3d_shot *s, double *out)
     return FWI3D_OK;
 }
 
+// Scratch of fwi3d_receiver_dft_noalloc: a fixed device array instead of
+// per-call buffers.  The name is a host-side stand-in, not a device address:
+// host code takes the address from hipGetSymbolAddress, once per device with
+// that device current, and uses only that pointer.
+__device__ __align__(256) double scratch[FWI3D_NOALLOC_SCRATCH_DOUBLES];
+static constexpr int kScratchDevices = 64;
+static double *g_scratch[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+static int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return fail(FWI3D_EDEVICE, "device %d beyond the %d the scratch table holds", dev,
+                    kScratchDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FWI3D_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return FWI3D_OK;
+}
+
 static int device_rows_dft(hipStream_t st, const double *d_rows, int nrows, int it, double dt,
                            double omega0, int nfreq, const double *omegas, double *out_host)
 {
@@ -1510,16 +1538,9 @@ static std::complex<double> h_factor(double omega0, double om)
     return std::sqrt(std::complex<double>(-2.0 * omega0, 0.0) / std::complex<double>(0.0, 1.0) / om);
 }
 
-int fwi3d_receiver_dft(fwi3d_shot *s, int nfreq, const double *omegas, int hfac, double *out)
+// the f_to_d_h factor on the magnetic rows of receiver spectra (host)
+static void h_factor_rows(const fwi3d_shot *s, int nfreq, const double *omegas, double *out)
 {
-    if (!s || !omegas || !out || nfreq <= 0)
-        return fail(FWI3D_EINVAL, "bad receiver DFT request");
-    if (s->nrec == 0)
-        return FWI3D_OK;
-    int rc = device_rows_dft(s->st, s->d_traces, 6 * s->nrec, s->it, s->m->p.dt, s->m->p.omega0,
-                             nfreq, omegas, out);
-    if (rc || !hfac)
-        return rc;
     for (int f = 0; f < nfreq; ++f) {
         const std::complex<double> k = h_factor(s->m->p.omega0, omegas[f]);
         for (int row = 3 * s->nrec; row < 6 * s->nrec; ++row) {
@@ -1530,6 +1551,50 @@ int fwi3d_receiver_dft(fwi3d_shot *s, int nfreq, const double *omegas, int hfac,
             out[2 * ((size_t)row * nfreq + f) + 1] = v.imag();
         }
     }
+}
+
+int fwi3d_receiver_dft(fwi3d_shot *s, int nfreq, const double *omegas, int hfac, double *out)
+{
+    if (!s || !omegas || !out || nfreq <= 0)
+        return fail(FWI3D_EINVAL, "bad receiver DFT request");
+    if (s->nrec == 0)
+        return FWI3D_OK;
+    int rc = device_rows_dft(s->st, s->d_traces, 6 * s->nrec, s->it, s->m->p.dt, s->m->p.omega0,
+                             nfreq, omegas, out);
+    if (rc || !hfac)
+        return rc;
+    h_factor_rows(s, nfreq, omegas, out);
+    return FWI3D_OK;
+}
+
+int fwi3d_receiver_dft_noalloc(fwi3d_shot *s, int nfreq, const double *omegas, int hfac,
+                               double *out)
+{
+    if (!s || !omegas || !out || nfreq <= 0)
+        return fail(FWI3D_EINVAL, "bad receiver DFT request");
+    if (s->nrec == 0)
+        return FWI3D_OK;
+    const size_t rows = 6 * (size_t)s->nrec;
+    const size_t off = ((size_t)nfreq + 31) / 32 * 32;   // spectra start 256-byte aligned
+    const size_t need = off + 2 * rows * (size_t)nfreq;
+    if (need > (size_t)FWI3D_NOALLOC_SCRATCH_DOUBLES)
+        return fail(FWI3D_EINVAL,
+                    "receiver DFT of %zu rows x %d frequencies needs %zu scratch doubles; the "
+                    "allocation-free call holds %d",
+                    rows, nfreq, need, FWI3D_NOALLOC_SCRATCH_DOUBLES);
+    double *d = nullptr;
+    int rc = noalloc_scratch(&d);
+    if (rc)
+        return rc;
+    // every output element is written by the kernel: nothing needs clearing
+    HIP_TRY(hipMemcpyAsync(d, omegas, sizeof(double) * nfreq, hipMemcpyHostToDevice, s->st));
+    HIP_TRY(mt::launch_receiver_dft(s->d_traces, (int)rows, s->it, s->m->p.dt, s->m->p.omega0,
+                                    nfreq, d, d + off, s->st));
+    HIP_TRY(hipMemcpyAsync(out, d + off, sizeof(double) * 2 * rows * nfreq, hipMemcpyDeviceToHost,
+                           s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    if (hfac)
+        h_factor_rows(s, nfreq, omegas, out);
     return FWI3D_OK;
 }
 
