@@ -221,24 +221,33 @@ def serve_mode(args):
     whole stream, and with the reference-named per-step operators on the
     smallest requests."""
     from mittet_main_amd import serving
-    reqs = serving.request_stream(args.seed, args.requests, max_shots=args.max_shots)
+    reqs = serving.request_stream(args.seed, args.requests, max_shots=args.max_shots,
+                                  steps_per_cell=args.steps_per_cell)
     log(f"[serve] {len(reqs)} requests, {sum(r.nshots for r in reqs)} shots, "
-        f"(n_int, shots): {[(r.n_int, r.nshots) for r in reqs]}")
+        f"(n_int, shots): {[(r.n_int, r.nshots) for r in reqs]}, Nt = {args.steps_per_cell} x n_int")
     done = lambda r: log(f"[serve]   request {r.rid} done, latency {r.latency:.2f} s")
-    srv = serving.Server(budget_bytes=200e9, chunk=32)
-    served = srv.run(reqs, realtime=True, on_done=done)
-    srv.close()
-    log(f"[serve] served: {served['shots_per_s']:.3f} shots/s, p50 {served['p50_s']:.2f} s, "
-        f"p99 {served['p99_s']:.2f} s")
+    runs = {}
+    for launch in args.launch.split(","):
+        srv = serving.Server(budget_bytes=200e9, chunk=32, launch=launch)
+        runs[launch] = srv.run(reqs, realtime=True, on_done=done)
+        srv.close()
+        r = runs[launch]
+        log(f"[serve] served ({launch}): {r['shots_per_s']:.3f} shots/s, p50 {r['p50_s']:.2f} s, "
+            f"p99 {r['p99_s']:.2f} s")
+    served = runs[args.launch.split(",")[0]]
     seq = serving.run_sequential(reqs, mode="fused", realtime=True, on_done=done)
     log(f"[serve] sequential fused: {seq['shots_per_s']:.3f} shots/s, p50 {seq['p50_s']:.2f} s")
     strip = lambda d: {k: (round(v, 4) if isinstance(v, float) else v)
                        for k, v in d.items() if k != "results"}
     out = {"mode": "serve", "metric": "served shots/s, p50/p95/p99 latency",
            "requests": len(reqs), "shots": served["shots"], "seed": args.seed,
+           "steps_per_cell": args.steps_per_cell, "launch": args.launch.split(",")[0],
            "served": strip(served), "sequential_fused": strip(seq),
            "throughput_vs_sequential": round(served["shots_per_s"] / seq["shots_per_s"], 3),
            "p50_vs_sequential": round(served["p50_s"] / seq["p50_s"], 3)}
+    for launch, r in runs.items():
+        out[f"served_{launch}"] = strip(r)
+        out[f"throughput_vs_sequential_{launch}"] = round(r["shots_per_s"] / seq["shots_per_s"], 3)
     if args.ops_requests:
         sub = sorted(reqs, key=serving.request_work)[:args.ops_requests]
         ops = serving.run_sequential(sub, mode="ops", realtime=False)
@@ -248,10 +257,60 @@ def serve_mode(args):
             "ops_shots_per_s": round(ops["shots_per_s"], 4),
             "fused_shots_per_s": round(fus["shots_per_s"], 4)}
         log(f"[serve] subset: ops {ops['shots_per_s']:.3f} vs fused {fus['shots_per_s']:.3f} shots/s")
-    same = all(np.array_equal(a, b) for rid, r in seq["results"].items()
-               for a, b in zip(r.spectra, served["results"][rid].spectra))
+    same = all(np.array_equal(a, b) for sv in runs.values() for rid, r in seq["results"].items()
+               for a, b in zip(r.spectra, sv["results"][rid].spectra))
     out["served_equals_alone_bitwise"] = bool(same)
     print(json.dumps(out), flush=True)
+
+
+def merge_mode(args):
+    """Merged multi-shot launches against the same shots back to back: k shots
+    of one grid (96^3 interior + 12-cell CPML, seed 7, 1024 steps, one receiver,
+    no volume DFT) advanced by one batch (every kernel launch covers all k)
+    and run one after another (fwi3d_run), for k = 1, 2, 3, 4, 8.  Each
+    configuration runs twice; the second run is timed."""
+    from mittet_main_amd import Batch, Model, configs
+    from mittet_main_amd.fwi3d import Grid, firstderiv_gauss
+    n_int, nc, it = 96, 12, 1024
+    sig, _ = configs.layered_box_sigma(n_int, nc, 7)
+    n = n_int + 2 * nc
+    dt, cmax = configs.dt_for(50.0, 50.0, 50.0)
+    g = Grid(n, n, n, nc, 50.0, 50.0, 50.0, dt, configs.OMEGA0, cmax)
+    signal = firstderiv_gauss(it, dt, configs.FMAX_W)
+    rec = np.array([[nc + 5, nc + 5, nc + n_int // 3]], dtype=np.int32)
+
+    def rate(k, merged):
+        m = Model(g)
+        m.media_coeff_3d(sig)
+        src = np.random.default_rng(3).integers(nc + 2, nc + n_int - 3, size=(k, 3))
+        shots = [m.shot(it, signal, tuple(int(v) for v in s), 0, rec) for s in src]
+        t0 = time.perf_counter()
+        if merged:
+            b = Batch(k + 1, chunk=32)
+            for sh in shots:
+                b.add(sh)
+            b.step(it)
+            b.sync()
+            for sh in shots:
+                b.remove(sh)
+            b.close()
+        else:
+            for sh in shots:
+                sh.run()
+                sh.sync()
+        t = time.perf_counter() - t0
+        m.close()
+        return g.ncells * (it - 1) * k / t / 1e9
+
+    rows = []
+    for k in (1, 2, 3, 4, 8):
+        alone = [rate(k, False) for _ in range(2)][-1]
+        merged = [rate(k, True) for _ in range(2)][-1]
+        rows.append({"shots": k, "fields_mb": round(k * g.ncells * 48 / 1e6), "alone_gcell_s": round(alone, 3),
+                     "merged_gcell_s": round(merged, 3), "merged_over_alone": round(merged / alone, 3)})
+        log(f"[merge] {k} shots of {n}^3 ({rows[-1]['fields_mb']} MB of E and H): alone {alone:.2f}, "
+            f"merged {merged:.2f} Gcell-updates/s")
+    print(json.dumps({"mode": "merge", "grid": n, "steps": it - 1, "rows": rows}), flush=True)
 
 
 def serve_fwi_mode(args):
@@ -358,11 +417,16 @@ def forward_step_counts(args, workload_steps):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", default="forward", choices=["forward", "serve", "gradient", "serve-fwi"])
+    ap.add_argument("--mode", default="forward",
+                    choices=["forward", "serve", "gradient", "serve-fwi", "merge"])
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--requests", type=int, default=4)
     ap.add_argument("--max-shots", type=int, default=16)
     ap.add_argument("--ops-requests", type=int, default=1)
+    ap.add_argument("--steps-per-cell", type=int, default=30,
+                    help="serve: Nt = this x interior size (30 as the served workload defines it)")
+    ap.add_argument("--launch", default="cache",
+                    help="serve: server launch modes to run, comma-separated (cache, interleave, merged)")
     ap.add_argument("--grad-nt", type=int, default=0, help="time steps (default 30 x 128)")
     ap.add_argument("--segment", type=int, default=0)
     ap.add_argument("--fwi-it", type=int, default=1000, help="serve-fwi: time samples per shot")
@@ -401,6 +465,8 @@ def main():
         return serve_mode(args)
     if args.mode == "serve-fwi":
         return serve_fwi_mode(args)
+    if args.mode == "merge":
+        return merge_mode(args)
     if args.mode == "gradient":
         return gradient_mode(args)
 

@@ -51,13 +51,19 @@ def test_request_work_orders_queue():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("policy,target_cells", [("swf", 24e6), ("swf", 1.0), ("fifo", 60e3)])
-def test_served_mix_equals_alone(policy, target_cells):
+@pytest.mark.parametrize("policy,target_cells,launch", [("swf", 24e6, "cache"),
+                                                         ("swf", 24e6, "merged"),
+                                                         ("swf", 1.0, "interleave"),
+                                                         ("fifo", 60e3, "interleave")])
+def test_served_mix_equals_alone(policy, target_cells, launch):
     reqs = small_stream()
     # budget for roughly two requests at a time: forces queueing and joins mid-run;
-    # target_cells 1 admits one shot at a time, 60e3 about two small grids
+    # target_cells 1 admits one shot at a time, 60e3 about two small grids; the
+    # shots in flight advance in groups that fit the cache (all of them, at these
+    # sizes), in one merged batch, or each in turn
     budget = 2.2 * max(serving.request_bytes(r, serving.build_model(r)[0]) for r in reqs)
-    srv = serving.Server(budget_bytes=budget, chunk=8, target_cells=target_cells, policy=policy)
+    srv = serving.Server(budget_bytes=budget, chunk=8, target_cells=target_cells, policy=policy,
+                         launch=launch)
     served = srv.run(reqs, realtime=False)
     srv.close()
     alone = serving.run_sequential(reqs, mode="fused", realtime=False)
