@@ -71,6 +71,17 @@ __device__ __forceinline__ int slab_pos(int r, int n, int nc)
 }
 
 // One cell of the E update + CPML-E corrections + source, general path.
+// Every load a cell needs is issued before the arithmetic that uses any of
+// them (the loads a flag does not ask for are skipped and their values left
+// zero), so a cell pays one memory round trip instead of one per branch.  The
+// arithmetic is the same expressions in the same order as one branch after the
+// other, so the result is unchanged to the bit.
+__device__ __forceinline__ double dm(double f0, double fm1, double fp1, double fm2, double c1,
+                                     double c2)
+{
+    return c1 * f0 - c1 * fm1 + c2 * fp1 - c2 * fm2;
+}
+
 template <int MODE>
 __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, int i, int jj, int k,
                                             int n, int si, int so, int sign)
@@ -84,7 +95,6 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
     const GLOBAL double *__restrict__ Hx = gc(H);
     const GLOBAL double *__restrict__ Hy = gc(H + N);
     const GLOBAL double *__restrict__ Hz = gc(H + 2 * (size_t)N);
-    double ex = Ein[c], ey = Ein[N + c], ez = Ein[2 * (size_t)N + c];
 
     const double c1 = S.c1, c2 = S.c2, dt = S.dt;
     const GLOBAL double *__restrict__ px = gc(S.prof[0]);
@@ -105,91 +115,168 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
     const bool fz = (MODE & M_PSI) && ((k >= 2 && k < nc) || (k >= iz - nc + 1 && k < iz - 2)) &&
                     i < ix - 1 && jj < iy - 1;
 
-    double hzy = 0.0, hyz = 0.0, hxz = 0.0, hzx = 0.0, hyx = 0.0, hxy = 0.0;
-    if (i >= 2 && i < ix - 1 && jj >= 2 && jj < iy - 1 && k >= 2 && k < iz - 1) {
-        // every backward stencil is in bounds: issue all loads up front
-        hzy = dminus(Hz, c, sy, c1, c2);
-        hyz = dminus(Hy, c, sz, c1, c2);
-        hxz = dminus(Hx, c, sz, c1, c2);
-        hzx = dminus(Hz, c, 1, c1, c2);
-        hyx = dminus(Hy, c, 1, c1, c2);
-        hxy = dminus(Hx, c, sy, c1, c2);
-    } else {
-        if (upx || fy) hzy = dminus(Hz, c, sy, c1, c2);
-        if (upx || fz) hyz = dminus(Hy, c, sz, c1, c2);
-        if (upy || fz) hxz = dminus(Hx, c, sz, c1, c2);
-        if (upy || fx) hzx = dminus(Hz, c, 1, c1, c2);
-        if (upz || fx) hyx = dminus(Hy, c, 1, c1, c2);
-        if (upz || fy) hxy = dminus(Hx, c, sy, c1, c2);
+    // ---- loads
+    double ex = Ein[c], ey = Ein[N + c], ez = Ein[2 * (size_t)N + c];
+    // every backward stencil in bounds: all six derivatives are loaded
+    const bool all = rx2 && ry2 && rz2;
+    const bool nzy = all || upx || fy, nyz = all || upx || fz, nxz = all || upy || fz;
+    const bool nzx = all || upy || fx, nyx = all || upz || fx, nxy = all || upz || fy;
+    double z0 = 0.0, zym1 = 0.0, zyp1 = 0.0, zym2 = 0.0, zxm1 = 0.0, zxp1 = 0.0, zxm2 = 0.0;
+    double y0 = 0.0, yzm1 = 0.0, yzp1 = 0.0, yzm2 = 0.0, yxm1 = 0.0, yxp1 = 0.0, yxm2 = 0.0;
+    double x0 = 0.0, xzm1 = 0.0, xzp1 = 0.0, xzm2 = 0.0, xym1 = 0.0, xyp1 = 0.0, xym2 = 0.0;
+    if (nzy || nzx)
+        z0 = Hz[c];
+    if (nzy) {
+        zym1 = Hz[c - sy];
+        zyp1 = Hz[c + sy];
+        zym2 = Hz[c - 2 * sy];
     }
-
+    if (nzx) {
+        zxm1 = Hz[c - 1];
+        zxp1 = Hz[c + 1];
+        zxm2 = Hz[c - 2];
+    }
+    if (nyz || nyx)
+        y0 = Hy[c];
+    if (nyz) {
+        yzm1 = Hy[c - sz];
+        yzp1 = Hy[c + sz];
+        yzm2 = Hy[c - 2 * sz];
+    }
+    if (nyx) {
+        yxm1 = Hy[c - 1];
+        yxp1 = Hy[c + 1];
+        yxm2 = Hy[c - 2];
+    }
+    if (nxz || nxy)
+        x0 = Hx[c];
+    if (nxz) {
+        xzm1 = Hx[c - sz];
+        xzp1 = Hx[c + sz];
+        xzm2 = Hx[c - 2 * sz];
+    }
+    if (nxy) {
+        xym1 = Hx[c - sy];
+        xyp1 = Hx[c + sy];
+        xym2 = Hx[c - 2 * sy];
+    }
     const double eps2 = gc(S.eps)[c];
-    const double sex = px[PR_ESIG * ix + i] * dt / (2.0 * eps2);
-    const double sey = py[PR_ESIG * iy + jj] * dt / (2.0 * eps2);
-    const double sez = pz[PR_ESIG * iz + k] * dt / (2.0 * eps2);
-    const double cbx = dt / eps2 / (1.0 + sex);
-    const double cby = dt / eps2 / (1.0 + sey);
-    const double cbz = dt / eps2 / (1.0 + sez);
+    const double sgx = px[PR_ESIG * ix + i], sgy = py[PR_ESIG * iy + jj], sgz = pz[PR_ESIG * iz + k];
+    double kx = 1.0, ky = 1.0, kz = 1.0;
+    if (MODE & M_UPD) {
+        kx = px[PR_KED * ix + i];
+        ky = py[PR_KED * iy + jj];
+        kz = pz[PR_KED * iz + k];
+    }
+    GLOBAL double *__restrict__ P0 = gm(S.psi[0]);
+    GLOBAL double *__restrict__ P1 = gm(S.psi[1]);
+    GLOBAL double *__restrict__ P2 = gm(S.psi[2]);
+    const size_t L0 = (size_t)S.slab[0], L1 = (size_t)S.slab[1], L2 = (size_t)S.slab[2];
+    size_t q0 = 0, q1 = 0, q2 = 0;
+    double bx = 0.0, cx = 0.0, ax0 = 0.0, ax1 = 0.0;
+    double by = 0.0, cy = 0.0, ay0 = 0.0, ay1 = 0.0;
+    double bz = 0.0, cz = 0.0, az0 = 0.0, az1 = 0.0;
+    if (fx) {
+        q0 = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
+        bx = px[PR_BE * ix + i];
+        cx = px[PR_CE * ix + i];
+        ax0 = P0[q0];
+        ax1 = P0[L0 + q0];
+    }
+    if (fy) {
+        q1 = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
+        by = py[PR_BE * iy + jj];
+        cy = py[PR_CE * iy + jj];
+        ay0 = P1[q1];
+        ay1 = P1[L1 + q1];
+    }
+    if (fz) {
+        q2 = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
+        bz = pz[PR_BE * iz + k];
+        cz = pz[PR_CE * iz + k];
+        az0 = P2[q2];
+        az1 = P2[L2 + q2];
+    }
+    const bool src = (MODE & M_SRC) && c == S.src_cell;
+    double amp = 0.0;
+    if (src)
+        amp = gc(S.amp)[n];
+
+    // ---- arithmetic
+    const double hzy = dm(z0, zym1, zyp1, zym2, c1, c2);
+    const double hyz = dm(y0, yzm1, yzp1, yzm2, c1, c2);
+    const double hxz = dm(x0, xzm1, xzp1, xzm2, c1, c2);
+    const double hzx = dm(z0, zxm1, zxp1, zxm2, c1, c2);
+    const double hyx = dm(y0, yxm1, yxp1, yxm2, c1, c2);
+    const double hxy = dm(x0, xym1, xyp1, xym2, c1, c2);
+
+    // cb = dt/eps'/(1 + s) and ca = (1 - s)/(1 + s) with s = sig dt/(2 eps');
+    // where an axis' CPML sigma is zero they are exactly dt/eps' and 1, so the
+    // divisions are skipped there (most shell cells lie in one axis' CPML)
+    const double dte = dt / eps2;
+    double cax = 1.0, cay = 1.0, caz = 1.0, cbx = dte, cby = dte, cbz = dte;
+    if (sgx != 0.0) {
+        const double se = sgx * dt / (2.0 * eps2);
+        cbx = dte / (1.0 + se);
+        cax = (1.0 - se) / (1.0 + se);
+    }
+    if (sgy != 0.0) {
+        const double se = sgy * dt / (2.0 * eps2);
+        cby = dte / (1.0 + se);
+        cay = (1.0 - se) / (1.0 + se);
+    }
+    if (sgz != 0.0) {
+        const double se = sgz * dt / (2.0 * eps2);
+        cbz = dte / (1.0 + se);
+        caz = (1.0 - se) / (1.0 + se);
+    }
 
     if (upx) {
-        const double ca = (1.0 - sex) / (1.0 + sex);
-        const double cu = hzy / py[PR_KED * iy + jj] - hyz / pz[PR_KED * iz + k];
-        ex = sign > 0 ? ca * ex + cbx * cu : ca * ex - cbx * cu;
+        const double cu = hzy / ky - hyz / kz;
+        ex = sign > 0 ? cax * ex + cbx * cu : cax * ex - cbx * cu;
     }
     if (upy) {
-        const double ca = (1.0 - sey) / (1.0 + sey);
-        const double cu = hxz / pz[PR_KED * iz + k] - hzx / px[PR_KED * ix + i];
-        ey = sign > 0 ? ca * ey + cby * cu : ca * ey - cby * cu;
+        const double cu = hxz / kz - hzx / kx;
+        ey = sign > 0 ? cay * ey + cby * cu : cay * ey - cby * cu;
     }
     if (upz) {
-        const double ca = (1.0 - sez) / (1.0 + sez);
-        const double cu = hyx / px[PR_KED * ix + i] - hxy / py[PR_KED * iy + jj];
-        ez = sign > 0 ? ca * ez + cbz * cu : ca * ez - cbz * cu;
+        const double cu = hyx / kx - hxy / ky;
+        ez = sign > 0 ? caz * ez + cbz * cu : caz * ez - cbz * cu;
     }
     if (fx) {
-        GLOBAL double *__restrict__ P = gm(S.psi[0]);
-        const size_t L = (size_t)S.slab[0];
-        const size_t q = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
-        const double be = px[PR_BE * ix + i], ce = px[PR_CE * ix + i], h = S.dl[0];
-        const double p0 = be * P[q] + ce * hyx / h;
-        const double p1 = be * P[L + q] + ce * hzx / h;
-        P[q] = p0;
-        P[L + q] = p1;
+        const double h = S.dl[0];
+        const double p0 = bx * ax0 + cx * hyx / h;
+        const double p1 = bx * ax1 + cx * hzx / h;
+        P0[q0] = p0;
+        P0[L0 + q0] = p1;
         ez += cbz * p0;
         ey -= cby * p1;
     }
     if (fy) {
-        GLOBAL double *__restrict__ P = gm(S.psi[1]);
-        const size_t L = (size_t)S.slab[1];
-        const size_t q = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
-        const double be = py[PR_BE * iy + jj], ce = py[PR_CE * iy + jj], h = S.dl[1];
-        const double p0 = be * P[q] + ce * hzy / h;
-        const double p1 = be * P[L + q] + ce * hxy / h;
-        P[q] = p0;
-        P[L + q] = p1;
+        const double h = S.dl[1];
+        const double p0 = by * ay0 + cy * hzy / h;
+        const double p1 = by * ay1 + cy * hxy / h;
+        P1[q1] = p0;
+        P1[L1 + q1] = p1;
         ex += cbx * p0;
         ez -= cbz * p1;
     }
     if (fz) {
-        GLOBAL double *__restrict__ P = gm(S.psi[2]);
-        const size_t L = (size_t)S.slab[2];
-        const size_t q = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
-        const double be = pz[PR_BE * iz + k], ce = pz[PR_CE * iz + k], h = S.dl[2];
-        const double p0 = be * P[q] + ce * hxz / h;
-        const double p1 = be * P[L + q] + ce * hyz / h;
-        P[q] = p0;
-        P[L + q] = p1;
+        const double h = S.dl[2];
+        const double p0 = bz * az0 + cz * hxz / h;
+        const double p1 = bz * az1 + cz * hyz / h;
+        P2[q2] = p0;
+        P2[L2 + q2] = p1;
         ey += cby * p0;
         ex -= cbx * p1;
     }
-    if ((MODE & M_SRC) && c == S.src_cell) {
-        const double a = gc(S.amp)[n];
+    if (src) {
         if (S.src_comp == 0)
-            ex -= a;
+            ex -= amp;
         else if (S.src_comp == 1)
-            ey -= a;
+            ey -= amp;
         else
-            ez -= a;
+            ez -= amp;
     }
     Eout[c] = ex;
     Eout[N + c] = ey;
@@ -226,8 +313,15 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
     efield_cell<MODE>(S, hsel(S, 0, 0), i, jj, k, n, si, so, sign);
 }
 
-// One cell of the H update + CPML-H corrections, general path.
-// H is read from Hin and written to Hout (the same buffer when updated in place).
+// One cell of the H update + CPML-H corrections, general path, with the loads
+// issued first as in efield_cell.  H is read from Hin and written to Hout (the
+// same buffer when updated in place).
+__device__ __forceinline__ double dp(double fp1, double f0, double fp2, double fm1, double c1,
+                                     double c2)
+{
+    return c1 * fp1 - c1 * f0 + c2 * fp2 - c2 * fm1;
+}
+
 template <int MODE>
 __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin, double *Hout, int i,
                                             int jj, int k, int so, int sign)
@@ -261,87 +355,155 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin,
     if (!(upx || upy || upz || gx || gy || gz))
         return;
 
-    double ezy = 0.0, eyz = 0.0, exz = 0.0, ezx = 0.0, eyx = 0.0, exy = 0.0;
-    if (i >= 1 && i < ix - 2 && jj >= 1 && jj < iy - 2 && k >= 1 && k < iz - 2) {
-        // every forward stencil is in bounds: issue all loads up front
-        ezy = dplus(Ez, c, sy, c1, c2);
-        eyz = dplus(Ey, c, sz, c1, c2);
-        exz = dplus(Ex, c, sz, c1, c2);
-        ezx = dplus(Ez, c, 1, c1, c2);
-        eyx = dplus(Ey, c, 1, c1, c2);
-        exy = dplus(Ex, c, sy, c1, c2);
-    } else {
-        if (upx || gy) ezy = dplus(Ez, c, sy, c1, c2);
-        if (upx || gz) eyz = dplus(Ey, c, sz, c1, c2);
-        if (upy || gz) exz = dplus(Ex, c, sz, c1, c2);
-        if (upy || gx) ezx = dplus(Ez, c, 1, c1, c2);
-        if (upz || gx) eyx = dplus(Ey, c, 1, c1, c2);
-        if (upz || gy) exy = dplus(Ex, c, sy, c1, c2);
+    // ---- loads
+    // every forward stencil in bounds: all six derivatives are loaded
+    const bool all = ax2 && ay2 && az2;
+    const bool nzy = all || upx || gy, nyz = all || upx || gz, nxz = all || upy || gz;
+    const bool nzx = all || upy || gx, nyx = all || upz || gx, nxy = all || upz || gy;
+    double z0 = 0.0, zyp1 = 0.0, zyp2 = 0.0, zym1 = 0.0, zxp1 = 0.0, zxp2 = 0.0, zxm1 = 0.0;
+    double y0 = 0.0, yzp1 = 0.0, yzp2 = 0.0, yzm1 = 0.0, yxp1 = 0.0, yxp2 = 0.0, yxm1 = 0.0;
+    double x0 = 0.0, xzp1 = 0.0, xzp2 = 0.0, xzm1 = 0.0, xyp1 = 0.0, xyp2 = 0.0, xym1 = 0.0;
+    if (nzy || nzx)
+        z0 = Ez[c];
+    if (nzy) {
+        zyp1 = Ez[c + sy];
+        zyp2 = Ez[c + 2 * sy];
+        zym1 = Ez[c - sy];
+    }
+    if (nzx) {
+        zxp1 = Ez[c + 1];
+        zxp2 = Ez[c + 2];
+        zxm1 = Ez[c - 1];
+    }
+    if (nyz || nyx)
+        y0 = Ey[c];
+    if (nyz) {
+        yzp1 = Ey[c + sz];
+        yzp2 = Ey[c + 2 * sz];
+        yzm1 = Ey[c - sz];
+    }
+    if (nyx) {
+        yxp1 = Ey[c + 1];
+        yxp2 = Ey[c + 2];
+        yxm1 = Ey[c - 1];
+    }
+    if (nxz || nxy)
+        x0 = Ex[c];
+    if (nxz) {
+        xzp1 = Ex[c + sz];
+        xzp2 = Ex[c + 2 * sz];
+        xzm1 = Ex[c - sz];
+    }
+    if (nxy) {
+        xyp1 = Ex[c + sy];
+        xyp2 = Ex[c + 2 * sy];
+        xym1 = Ex[c - sy];
+    }
+    const double msx = px[PR_MSIG * ix + i], msy = py[PR_MSIG * iy + jj], msz = pz[PR_MSIG * iz + k];
+    // eps' is only used where a magnetic CPML conductivity is nonzero; loading
+    // it beside the conductivities rather than after them saves a round trip
+    const double eps2 = gc(S.eps)[c];
+    double hx = Hin[c], hy = Hin[N + c], hz = Hin[2 * (size_t)N + c];
+    double kx = 1.0, ky = 1.0, kz = 1.0;
+    if (MODE & M_UPD) {
+        kx = px[PR_KHD * ix + i];
+        ky = py[PR_KHD * iy + jj];
+        kz = pz[PR_KHD * iz + k];
+    }
+    GLOBAL double *__restrict__ P0 = gm(S.psi[0]);
+    GLOBAL double *__restrict__ P1 = gm(S.psi[1]);
+    GLOBAL double *__restrict__ P2 = gm(S.psi[2]);
+    const size_t L0 = (size_t)S.slab[0], L1 = (size_t)S.slab[1], L2 = (size_t)S.slab[2];
+    size_t q0 = 0, q1 = 0, q2 = 0;
+    double bx = 0.0, cx = 0.0, ax0 = 0.0, ax1v = 0.0;
+    double by = 0.0, cy = 0.0, ay0 = 0.0, ay1v = 0.0;
+    double bz = 0.0, cz = 0.0, az0 = 0.0, az1v = 0.0;
+    if (gx) {
+        q0 = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
+        bx = px[PR_BH * ix + i];
+        cx = px[PR_CH * ix + i];
+        ax0 = P0[2 * L0 + q0];
+        ax1v = P0[3 * L0 + q0];
+    }
+    if (gy) {
+        q1 = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
+        by = py[PR_BH * iy + jj];
+        cy = py[PR_CH * iy + jj];
+        ay0 = P1[2 * L1 + q1];
+        ay1v = P1[3 * L1 + q1];
+    }
+    if (gz) {
+        q2 = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
+        bz = pz[PR_BH * iz + k];
+        cz = pz[PR_CH * iz + k];
+        az0 = P2[2 * L2 + q2];
+        az1v = P2[3 * L2 + q2];
     }
 
-    // da = 1 and db = dt/mu0 wherever the magnetic CPML conductivity is zero;
-    // eps' is only needed inside the shells.
-    const double msx = px[PR_MSIG * ix + i], msy = py[PR_MSIG * iy + jj], msz = pz[PR_MSIG * iz + k];
-    double smx = 0.0, smy = 0.0, smz = 0.0;
-    if (msx != 0.0 || msy != 0.0 || msz != 0.0) {
-        const double eps2 = gc(S.eps)[c];
-        smx = msx * dt / (2.0 * eps2);
-        smy = msy * dt / (2.0 * eps2);
-        smz = msz * dt / (2.0 * eps2);
+    // ---- arithmetic
+    const double ezy = dp(zyp1, z0, zyp2, zym1, c1, c2);
+    const double eyz = dp(yzp1, y0, yzp2, yzm1, c1, c2);
+    const double exz = dp(xzp1, x0, xzp2, xzm1, c1, c2);
+    const double ezx = dp(zxp1, z0, zxp2, zxm1, c1, c2);
+    const double eyx = dp(yxp1, y0, yxp2, yxm1, c1, c2);
+    const double exy = dp(xyp1, x0, xyp2, xym1, c1, c2);
+
+    // db = dt/mu0/(1 + s) and da = (1 - s)/(1 + s) with s = msig dt/(2 eps');
+    // exactly dt/mu0 and 1 where an axis' magnetic CPML sigma is zero
+    const double dtm = dt / S.mu0;
+    double dax = 1.0, day = 1.0, daz = 1.0, dbx = dtm, dby = dtm, dbz = dtm;
+    if (msx != 0.0) {
+        const double sm = msx * dt / (2.0 * eps2);
+        dbx = dtm / (1.0 + sm);
+        dax = (1.0 - sm) / (1.0 + sm);
     }
-    const double dbx = dt / S.mu0 / (1.0 + smx);
-    const double dby = dt / S.mu0 / (1.0 + smy);
-    const double dbz = dt / S.mu0 / (1.0 + smz);
-    double hx = Hin[c], hy = Hin[N + c], hz = Hin[2 * (size_t)N + c];
+    if (msy != 0.0) {
+        const double sm = msy * dt / (2.0 * eps2);
+        dby = dtm / (1.0 + sm);
+        day = (1.0 - sm) / (1.0 + sm);
+    }
+    if (msz != 0.0) {
+        const double sm = msz * dt / (2.0 * eps2);
+        dbz = dtm / (1.0 + sm);
+        daz = (1.0 - sm) / (1.0 + sm);
+    }
 
     if (upx) {
-        const double da = (1.0 - smx) / (1.0 + smx);
-        const double cu = ezy / py[PR_KHD * iy + jj] - eyz / pz[PR_KHD * iz + k];
-        hx = sign > 0 ? da * hx - dbx * cu : da * hx + dbx * cu;
+        const double cu = ezy / ky - eyz / kz;
+        hx = sign > 0 ? dax * hx - dbx * cu : dax * hx + dbx * cu;
     }
     if (upy) {
-        const double da = (1.0 - smy) / (1.0 + smy);
-        const double cu = exz / pz[PR_KHD * iz + k] - ezx / px[PR_KHD * ix + i];
-        hy = sign > 0 ? da * hy - dby * cu : da * hy + dby * cu;
+        const double cu = exz / kz - ezx / kx;
+        hy = sign > 0 ? day * hy - dby * cu : day * hy + dby * cu;
     }
     if (upz) {
-        const double da = (1.0 - smz) / (1.0 + smz);
-        const double cu = eyx / px[PR_KHD * ix + i] - exy / py[PR_KHD * iy + jj];
-        hz = sign > 0 ? da * hz - dbz * cu : da * hz + dbz * cu;
+        const double cu = eyx / kx - exy / ky;
+        hz = sign > 0 ? daz * hz - dbz * cu : daz * hz + dbz * cu;
     }
     if (gx) {
-        GLOBAL double *__restrict__ P = gm(S.psi[0]);
-        const size_t L = (size_t)S.slab[0];
-        const size_t q = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
-        const double bh = px[PR_BH * ix + i], ch = px[PR_CH * ix + i], h = S.dl[0];
-        const double p0 = bh * P[2 * L + q] + ch * ezx / h;
-        const double p1 = bh * P[3 * L + q] + ch * eyx / h;
-        P[2 * L + q] = p0;
-        P[3 * L + q] = p1;
+        const double h = S.dl[0];
+        const double p0 = bx * ax0 + cx * ezx / h;
+        const double p1 = bx * ax1v + cx * eyx / h;
+        P0[2 * L0 + q0] = p0;
+        P0[3 * L0 + q0] = p1;
         hy += dby * p0;
         hz -= dbz * p1;
     }
     if (gy) {
-        GLOBAL double *__restrict__ P = gm(S.psi[1]);
-        const size_t L = (size_t)S.slab[1];
-        const size_t q = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
-        const double bh = py[PR_BH * iy + jj], ch = py[PR_CH * iy + jj], h = S.dl[1];
-        const double p0 = bh * P[2 * L + q] + ch * exy / h;
-        const double p1 = bh * P[3 * L + q] + ch * ezy / h;
-        P[2 * L + q] = p0;
-        P[3 * L + q] = p1;
+        const double h = S.dl[1];
+        const double p0 = by * ay0 + cy * exy / h;
+        const double p1 = by * ay1v + cy * ezy / h;
+        P1[2 * L1 + q1] = p0;
+        P1[3 * L1 + q1] = p1;
         hz += dbz * p0;
         hx -= dbx * p1;
     }
     if (gz) {
-        GLOBAL double *__restrict__ P = gm(S.psi[2]);
-        const size_t L = (size_t)S.slab[2];
-        const size_t q = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
-        const double bh = pz[PR_BH * iz + k], ch = pz[PR_CH * iz + k], h = S.dl[2];
-        const double p0 = bh * P[2 * L + q] + ch * eyz / h;
-        const double p1 = bh * P[3 * L + q] + ch * exz / h;
-        P[2 * L + q] = p0;
-        P[3 * L + q] = p1;
+        const double h = S.dl[2];
+        const double p0 = bz * az0 + cz * eyz / h;
+        const double p1 = bz * az1v + cz * exz / h;
+        P2[2 * L2 + q2] = p0;
+        P2[3 * L2 + q2] = p1;
         hx += dbx * p0;
         hy -= dby * p1;
     }
