@@ -70,6 +70,23 @@ __device__ __forceinline__ int slab_pos(int r, int n, int nc)
     return r < nc ? r : r - (n - 2 * nc);
 }
 
+// Absolute step of local index 0 of the current chunk.  Every lane of a launch
+// reads the same word, which the previous chunk's k_advance wrote: a scalar
+// load through the constant address space serves it from the scalar cache
+// instead of a vector round trip to L2 per wave.
+__device__ __forceinline__ int chunk_step(const ShotDev &S)
+{
+    return *(const __attribute__((address_space(4))) int *)S.step;
+}
+
+// A wave-uniform double (a z-profile value of the current plane) through the
+// scalar cache.  A vector load issued after a plane's prefetch would make the
+// wait for it also wait for every prefetch load issued before it.
+__device__ __forceinline__ double uload(const double *p)
+{
+    return *(const __attribute__((address_space(4))) double *)p;
+}
+
 // One cell of the E update + CPML-E corrections + source, general path.
 // Every load a cell needs is issued before the arithmetic that uses any of
 // them (the loads a flag does not ask for are skipped and their values left
@@ -243,32 +260,30 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
         const double cu = hyx / kx - hxy / ky;
         ez = sign > 0 ? caz * ez + cbz * cu : caz * ez - cbz * cu;
     }
+    // The psi values are stored after all arithmetic: a store issued before a
+    // later face's branch would make that branch's wait for its loads also
+    // wait for the store to complete.
+    double px0 = 0.0, px1 = 0.0, py0 = 0.0, py1 = 0.0, pz0 = 0.0, pz1 = 0.0;
     if (fx) {
         const double h = S.dl[0];
-        const double p0 = bx * ax0 + cx * hyx / h;
-        const double p1 = bx * ax1 + cx * hzx / h;
-        P0[q0] = p0;
-        P0[L0 + q0] = p1;
-        ez += cbz * p0;
-        ey -= cby * p1;
+        px0 = bx * ax0 + cx * hyx / h;
+        px1 = bx * ax1 + cx * hzx / h;
+        ez += cbz * px0;
+        ey -= cby * px1;
     }
     if (fy) {
         const double h = S.dl[1];
-        const double p0 = by * ay0 + cy * hzy / h;
-        const double p1 = by * ay1 + cy * hxy / h;
-        P1[q1] = p0;
-        P1[L1 + q1] = p1;
-        ex += cbx * p0;
-        ez -= cbz * p1;
+        py0 = by * ay0 + cy * hzy / h;
+        py1 = by * ay1 + cy * hxy / h;
+        ex += cbx * py0;
+        ez -= cbz * py1;
     }
     if (fz) {
         const double h = S.dl[2];
-        const double p0 = bz * az0 + cz * hxz / h;
-        const double p1 = bz * az1 + cz * hyz / h;
-        P2[q2] = p0;
-        P2[L2 + q2] = p1;
-        ey += cby * p0;
-        ex -= cbx * p1;
+        pz0 = bz * az0 + cz * hxz / h;
+        pz1 = bz * az1 + cz * hyz / h;
+        ey += cby * pz0;
+        ex -= cbx * pz1;
     }
     if (src) {
         if (S.src_comp == 0)
@@ -277,6 +292,18 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
             ey -= amp;
         else
             ez -= amp;
+    }
+    if (fx) {
+        P0[q0] = px0;
+        P0[L0 + q0] = px1;
+    }
+    if (fy) {
+        P1[q1] = py0;
+        P1[L1 + q1] = py1;
+    }
+    if (fz) {
+        P2[q2] = pz0;
+        P2[L2 + q2] = pz1;
     }
     Eout[c] = ex;
     Eout[N + c] = ey;
@@ -293,7 +320,7 @@ __global__ __launch_bounds__(TX *TY) void k_efield(const ShotDev *__restrict__ D
     const ShotDev &S = D[s];
     int n = nover;
     if (n < 0) {
-        n = *S.step + j;
+        n = chunk_step(S) + j;
         if (n >= S.it - 1)
             return;  // this shot has finished its time loop
     }
@@ -480,32 +507,40 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin,
         const double cu = eyx / kx - exy / ky;
         hz = sign > 0 ? daz * hz - dbz * cu : daz * hz + dbz * cu;
     }
+    // psi stored after all arithmetic, as in efield_cell
+    double px0 = 0.0, px1 = 0.0, py0 = 0.0, py1 = 0.0, pz0 = 0.0, pz1 = 0.0;
     if (gx) {
         const double h = S.dl[0];
-        const double p0 = bx * ax0 + cx * ezx / h;
-        const double p1 = bx * ax1v + cx * eyx / h;
-        P0[2 * L0 + q0] = p0;
-        P0[3 * L0 + q0] = p1;
-        hy += dby * p0;
-        hz -= dbz * p1;
+        px0 = bx * ax0 + cx * ezx / h;
+        px1 = bx * ax1v + cx * eyx / h;
+        hy += dby * px0;
+        hz -= dbz * px1;
     }
     if (gy) {
         const double h = S.dl[1];
-        const double p0 = by * ay0 + cy * exy / h;
-        const double p1 = by * ay1v + cy * ezy / h;
-        P1[2 * L1 + q1] = p0;
-        P1[3 * L1 + q1] = p1;
-        hz += dbz * p0;
-        hx -= dbx * p1;
+        py0 = by * ay0 + cy * exy / h;
+        py1 = by * ay1v + cy * ezy / h;
+        hz += dbz * py0;
+        hx -= dbx * py1;
     }
     if (gz) {
         const double h = S.dl[2];
-        const double p0 = bz * az0 + cz * eyz / h;
-        const double p1 = bz * az1v + cz * exz / h;
-        P2[2 * L2 + q2] = p0;
-        P2[3 * L2 + q2] = p1;
-        hx += dbx * p0;
-        hy -= dby * p1;
+        pz0 = bz * az0 + cz * eyz / h;
+        pz1 = bz * az1v + cz * exz / h;
+        hx += dbx * pz0;
+        hy -= dby * pz1;
+    }
+    if (gx) {
+        P0[2 * L0 + q0] = px0;
+        P0[3 * L0 + q0] = px1;
+    }
+    if (gy) {
+        P1[2 * L1 + q1] = py0;
+        P1[3 * L1 + q1] = py1;
+    }
+    if (gz) {
+        P2[2 * L2 + q2] = pz0;
+        P2[3 * L2 + q2] = pz1;
     }
     GLOBAL double *__restrict__ Ho = gm(Hout);
     Ho[c] = hx;
@@ -524,7 +559,7 @@ __global__ __launch_bounds__(TX *TY) void k_hfield(const ShotDev *__restrict__ D
     if (MODE & M_INPLACE) {
         so = S.cur;
     } else {
-        if (*S.step + j >= S.it - 1)
+        if (chunk_step(S) + j >= S.it - 1)
             return;
         so = (S.cur + j + 1) % S.ring;
     }
@@ -571,7 +606,7 @@ __global__ void k_receivers(const ShotDev *__restrict__ D, int j, int nover, int
     const ShotDev &S = D[blockIdx.x];
     int n = nover, so = S.cur;
     if (n < 0) {
-        n = *S.step + j;
+        n = chunk_step(S) + j;
         if (n >= S.it - 1)
             return;
     }
@@ -738,7 +773,7 @@ __global__ __launch_bounds__(256, MT_FLUSH_WAVES) void k_flush(const ShotDev *__
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
     const int nf = S.nf;
-    const int n0 = *S.step;
+    const int n0 = chunk_step(S);
     const double *src = HS ? S.hring : S.e;
     double *acc = HS ? S.acc_h : S.acc;
     int Lv = L;
@@ -798,7 +833,7 @@ __global__ __launch_bounds__(256) void k_hstore(const ShotDev *__restrict__ D, i
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    const int n = *S.step + j;
+    const int n = chunk_step(S) + j;
     if (!S.hring || n >= S.it - 1)
         return;
     const size_t N = (size_t)S.n;
@@ -942,7 +977,7 @@ __global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_efield_z(const ShotDev *
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    const int n = *S.step + j;
+    const int n = chunk_step(S) + j;
     if (n >= S.it - 1)
         return;
     ZCol z;
@@ -1020,7 +1055,7 @@ __global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_efield_z(const ShotDev *
             cby = dt / eps2 / (1.0 + sey);
             cbz = dt / eps2 / (1.0 + sez);
         }
-        const double kedz = pz[PR_KED * iz + k];
+        const double kedz = uload(S.prof[2] + PR_KED * iz + k);
         if (upx) {
             const double ca = esx != 0.0 ? (1.0 - sex) / (1.0 + sex) : 1.0;
             const double cu = hzy / kedy - hyz / kedz;
@@ -1098,7 +1133,7 @@ __global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_hfield_z(const ShotDev *
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    if (*S.step + j >= S.it - 1)
+    if (chunk_step(S) + j >= S.it - 1)
         return;
     ZCol z;
     if (!zcol(S, blockIdx.x - pre[s], z))
@@ -1175,7 +1210,7 @@ __global__ __launch_bounds__(ZX *ZY, MT_Z_WAVES) void k_hfield_z(const ShotDev *
                 dby = dt / S.mu0 / (1.0 + smy);
                 dbz = dt / S.mu0 / (1.0 + smz);
             }
-            const double khdz = pz[PR_KHD * iz + k];
+            const double khdz = uload(S.prof[2] + PR_KHD * iz + k);
             double hx = Hx[c], hy = Hy[c], hz = Hz[c];
             if (upx) {
                 const double da = msx != 0.0 ? (1.0 - smx) / (1.0 + smx) : 1.0;
@@ -1271,10 +1306,11 @@ hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks
 
 __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, int &k)
 {
-    const int q = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
-    if (q >= S.fb)
-        return false;
     if (!S.shell) {
+        // every tile of the grid, each XCD owning a contiguous run of planes
+        const int q = (b & 7) * (S.fb_pad >> 3) + (b >> 3);
+        if (q >= S.fb)
+            return false;
         const int tx = q % S.ftx, r = q / S.ftx;
         const int ty = r % S.fty;
         k = r / S.fty;
@@ -1282,7 +1318,13 @@ __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, 
         jj = ty * ZY + threadIdx.x / ZX;
         return i < S.ix && jj < S.iy;
     }
-    const GLOBAL int *t = (const GLOBAL int *)(S.shell + q);   // {x0, y0, k, yend}
+    // the shell tile list in launch order: at 288^3 (16-cell CPML) this runs
+    // each half-step's shell launch 12 us faster than giving each XCD a
+    // contiguous run of the list, while on grids that the per-cell family
+    // covers whole (84^3) the contiguous runs are 1.2x faster
+    if (b >= S.fb)
+        return false;
+    const GLOBAL int *t = (const GLOBAL int *)(S.shell + b);   // {x0, y0, k, yend}
     const int x0 = t[0], y0 = t[1], yend = t[3];
     k = t[2];
     if (yend) {   // 16 x 16 tile of an x-edge segment, rows below yend
@@ -1315,7 +1357,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
     if ((int)blockIdx.x < recn) {
         if ((int)blockIdx.x < ns) {
             const ShotDev &R = D[blockIdx.x];
-            const int n = *R.step + j - 1;
+            const int n = chunk_step(R) + j - 1;
             if (n < R.it - 1)
                 rec_store(R, n, (R.cur + j) % R.ring, hsel(R, j, hf));
         }
@@ -1324,7 +1366,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
     const int b = blockIdx.x - recn;
     const int s = find_shot(pre, ns, b);
     const ShotDev &S = D[s];
-    const int n = *S.step + j;
+    const int n = chunk_step(S) + j;
     if (n >= S.it - 1)
         return;
     int i, jj, k;
@@ -1389,7 +1431,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    if (*S.step + j >= S.it - 1)
+    if (chunk_step(S) + j >= S.it - 1)
         return;
     int i, jj, k;
     if (!fcell(S, blockIdx.x - pre[s], i, jj, k))
@@ -1538,7 +1580,7 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
     __shared__ double sz[3][TZ::N], sy[3][TY::N], sx[3][TX::N];
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    const int n = *S.step + j;
+    const int n = chunk_step(S) + j;
     if (n >= S.it - 1)
         return;
     int x0, y0, k0, k1;
@@ -1594,7 +1636,7 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
             nez = Ein[2 * N + cn];
             ncb = CB[cn];
         }
-        const double kedz = gc(S.prof[2])[PR_KED * S.iz + k];
+        const double kedz = uload(S.prof[2] + PR_KED * S.iz + k);
         const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
         const int zc = (ly + 2) * TZ::W + lx + 2;
         const double z0 = Z[zc], zym1 = Z[zc - TZ::W], zyp1 = Z[zc + TZ::W], zym2 = Z[zc - 2 * TZ::W];
@@ -1657,7 +1699,7 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
     __shared__ double sz[4][TZ::N], sy[4][TY::N], sx[4][TX::N];
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    if (*S.step + j >= S.it - 1)
+    if (chunk_step(S) + j >= S.it - 1)
         return;
     int x0, y0, k0, k1;
     if (!lcol(S, blockIdx.x - pre[s], x0, y0, k0, k1))
@@ -1712,7 +1754,7 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
             nhy = Hy[cn];
             nhz = Hz[cn];
         }
-        const double khdz = gc(S.prof[2])[PR_KHD * S.iz + k];
+        const double khdz = uload(S.prof[2] + PR_KHD * S.iz + k);
         const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
         const int zc = (ly + 1) * TZ::W + lx + 1;
         const double z0 = Z[zc], zyp1 = Z[zc + TZ::W], zyp2 = Z[zc + 2 * TZ::W], zym1 = Z[zc - TZ::W];
@@ -1826,7 +1868,7 @@ __global__ __launch_bounds__(LNT, 4) void k_ehfield_l(const ShotDev *__restrict_
     __shared__ double sh[FHN], se[3][GEN];
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    const int n = *S.step + j;
+    const int n = chunk_step(S) + j;
     if (n >= S.it - 1)
         return;
     int x0, y0, k0, k1;
@@ -2139,7 +2181,7 @@ __global__ __launch_bounds__(256) void k_efield_p(const ShotDev *__restrict__ D,
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    const int n = *S.step + j;
+    const int n = chunk_step(S) + j;
     if (n >= S.it - 1)
         return;
     int i, jj, k;
@@ -2224,7 +2266,7 @@ __global__ __launch_bounds__(256) void k_hfield_p(const ShotDev *__restrict__ D,
 {
     const int s = find_shot(pre, ns, blockIdx.x);
     const ShotDev &S = D[s];
-    if (*S.step + j >= S.it - 1)
+    if (chunk_step(S) + j >= S.it - 1)
         return;
     int i, jj, k;
     if (!fpair(S, blockIdx.x - pre[s], i, jj, k))
