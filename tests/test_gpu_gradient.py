@@ -56,7 +56,8 @@ def test_gradient_matches_central_fd():
         fd = (jp - jm) / (2 * h)
         g = grad[k, j, i]
         # relative error against the gradient's own scale at that cell, floored
-        # at 1e-6 of the largest component (roundoff of J(+h) - J(-h))
+        # at 1e-3 of the largest component (roundoff of J(+h) - J(-h) swamps
+        # cells whose gradient is smaller than that)
         err = abs(fd - g) / max(abs(g), 1e-3 * gmax)
         assert err <= 1e-6, ((k, j, i), g, fd, err)
     P.close()
