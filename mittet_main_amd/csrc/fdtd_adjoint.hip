@@ -150,17 +150,22 @@ __device__ __forceinline__ bool cell_of(const AdjArgs &A, int &i, int &j, int &k
 // Weights of the transposed stencils at a cell (flat index c), component d.
 // H phase: db lH with db = dt/mu0 in the LDS region, where the per-cell pass
 // leaves t3 unwritten; t3 elsewhere.  E phase: cb lE with cb = dt/eps' there.
+// The two bases are read into values before the per-lane choice: choosing
+// between the argument struct's fields would choose their addresses and load
+// the pointer per lane before the weight.
 __device__ __forceinline__ double wgt_h(const AdjArgs &A, int d, int i, int j, int k, int c)
 {
     const bool in = in_region(A, i, j, k);
-    const double v = ((const GLOBAL double *)(in ? A.lH : A.t3))[(size_t)d * A.n + c];
+    const GLOBAL double *fin = (const GLOBAL double *)A.lH, *fout = (const GLOBAL double *)A.t3;
+    const double v = (in ? fin : fout)[(size_t)d * A.n + c];
     return in ? A.dt / A.mu0 * v : v;
 }
 
 __device__ __forceinline__ double wgt_e(const AdjArgs &A, int d, int i, int j, int k, int c)
 {
     const bool in = in_region(A, i, j, k);
-    const double v = ((const GLOBAL double *)(in ? A.lE : A.t3))[(size_t)d * A.n + c];
+    const GLOBAL double *fin = (const GLOBAL double *)A.lE, *fout = (const GLOBAL double *)A.t3;
+    const double v = (in ? fin : fout)[(size_t)d * A.n + c];
     return in ? ((const GLOBAL double *)A.cbi)[c] * v : v;
 }
 
@@ -571,50 +576,70 @@ template <int XL, int XR, int YL, int YR> struct ATile {
     static constexpr int R = (N + ANT - 1) / ANT;   // elements per lane
 };
 
-// weights of component d on plane kk of a tile (halo XL/YL before the tile),
-// elements beyond (xmax, ymax) left 0
+// Weights of component d on plane kk of a tile (halo XL/YL before the tile),
+// in two parts so that a plane's loads overlap the arithmetic of the plane
+// before it: wload issues the raw loads (the field inside the region, t3
+// outside it, and for E weights dt/eps' as well) with no branch and no use of
+// their values; wstore_w applies the region's scaling when the tile goes to
+// LDS.  Coordinates beyond (xmax, ymax) are clamped onto them: those tile
+// cells are read only by the idle lanes of a partial tile.
+template <class T, int XL, int YL>
+__device__ __forceinline__ int tile_cell(int e, int x0, int y0, int xmax, int ymax, int &gx, int &gy)
+{
+    gx = min(x0 - XL + e % T::W, xmax);
+    gy = min(y0 - YL + e / T::W, ymax);
+    return e;
+}
+
 template <class T, int XL, int YL, bool HPHASE>
 __device__ __forceinline__ void wload(const AdjArgs &A, int d, int kk, int x0, int y0, int xmax,
-                                      int ymax, double (&r)[T::R])
+                                      int ymax, double (&r)[T::R], double (&rc)[T::R])
+{
+    // the two bases as values: selecting between struct fields would select
+    // their addresses and load the pointer per lane first
+    const GLOBAL double *fin = (const GLOBAL double *)(HPHASE ? A.lH : A.lE) + (size_t)d * A.n;
+    const GLOBAL double *fout = (const GLOBAL double *)A.t3 + (size_t)d * A.n;
+    const GLOBAL double *cbi = (const GLOBAL double *)A.cbi;
+#pragma unroll
+    for (int q = 0; q < T::R; ++q) {
+        int gx, gy;
+        tile_cell<T, XL, YL>(min((int)threadIdx.x + q * ANT, T::N - 1), x0, y0, xmax, ymax, gx, gy);
+        const int c = (kk * A.iy + gy) * A.ix + gx;
+        const bool in = in_region(A, gx, gy, kk);
+        r[q] = (in ? fin : fout)[c];
+        if (!HPHASE)
+            rc[q] = cbi[c];
+    }
+}
+
+template <class T, int XL, int YL, bool HPHASE>
+__device__ __forceinline__ void wstore_w(const AdjArgs &A, int kk, int x0, int y0, int xmax, int ymax,
+                                         double *lds, const double (&r)[T::R], const double (&rc)[T::R])
 {
     const double dbi = A.dt / A.mu0;
 #pragma unroll
     for (int q = 0; q < T::R; ++q) {
         const int e = threadIdx.x + q * ANT;
-        double v = 0.0;
         if (e < T::N) {
-            const int gy = y0 - YL + e / T::W, gx = x0 - XL + e % T::W;
-            if (gx <= xmax && gy <= ymax) {
-                // one load from the field or from t3, then the region's scaling
-                const int c = (kk * A.iy + gy) * A.ix + gx;
-                const bool in = in_region(A, gx, gy, kk);
-                const GLOBAL double *f = (const GLOBAL double *)(in ? (HPHASE ? A.lH : A.lE) : A.t3);
-                v = f[(size_t)d * A.n + c];
-                if (HPHASE)
-                    v = in ? dbi * v : v;
-                else
-                    v = in ? ((const GLOBAL double *)A.cbi)[c] * v : v;
-            }
+            int gx, gy;
+            tile_cell<T, XL, YL>(e, x0, y0, xmax, ymax, gx, gy);
+            const bool in = in_region(A, gx, gy, kk);
+            const double v = r[q];
+            lds[e] = in ? (HPHASE ? dbi * v : rc[q] * v) : v;
         }
-        r[q] = v;
     }
 }
 
-// values of array f on plane kk of a tile, elements beyond (xmax, ymax) left 0
+// values of array f on plane kk of a tile, coordinates clamped as in wload
 template <class T, int XL, int YL>
 __device__ __forceinline__ void fload(const GLOBAL double *f, int kk, int ix, int pl, int x0,
                                       int y0, int xmax, int ymax, double (&r)[T::R])
 {
 #pragma unroll
     for (int q = 0; q < T::R; ++q) {
-        const int e = threadIdx.x + q * ANT;
-        double v = 0.0;
-        if (e < T::N) {
-            const int gy = y0 - YL + e / T::W, gx = x0 - XL + e % T::W;
-            if (gx <= xmax && gy <= ymax)
-                v = f[(size_t)kk * pl + (size_t)gy * ix + gx];
-        }
-        r[q] = v;
+        int gx, gy;
+        tile_cell<T, XL, YL>(min((int)threadIdx.x + q * ANT, T::N - 1), x0, y0, xmax, ymax, gx, gy);
+        r[q] = f[(size_t)kk * pl + (size_t)gy * ix + gx];
     }
 }
 
@@ -674,29 +699,29 @@ __global__ __launch_bounds__(ANT) void k_adj_hl(AdjArgs A)
     double xq1 = wgt_h(A, 0, ci, cj, k0 - 1, (k0 - 1) * pl + col);
     double rgz[TZ::R], rgy[TY::R], rgx[TX::R];
     int b0 = 0, b1 = 1, b2 = 2;   // LDS buffers of planes k, k+1 and the one being filled
-    wload<TZ, 2, 2, true>(A, 2, k0, x0, y0, xmax, ymax, rgz);
-    wload<TY, 2, 0, true>(A, 1, k0, x0, y0, xmax, ymax, rgy);
-    wload<TX, 0, 2, true>(A, 0, k0, x0, y0, xmax, ymax, rgx);
-    wstore<TZ>(sz[b0], rgz);
-    wstore<TY>(sy[b0], rgy);
-    wstore<TX>(sx[b0], rgx);
-    wload<TZ, 2, 2, true>(A, 2, k0 + 1, x0, y0, xmax, ymax, rgz);
-    wload<TY, 2, 0, true>(A, 1, k0 + 1, x0, y0, xmax, ymax, rgy);
-    wload<TX, 0, 2, true>(A, 0, k0 + 1, x0, y0, xmax, ymax, rgx);
+    wload<TZ, 2, 2, true>(A, 2, k0, x0, y0, xmax, ymax, rgz, rgz);
+    wload<TY, 2, 0, true>(A, 1, k0, x0, y0, xmax, ymax, rgy, rgy);
+    wload<TX, 0, 2, true>(A, 0, k0, x0, y0, xmax, ymax, rgx, rgx);
+    wstore_w<TZ, 2, 2, true>(A, k0, x0, y0, xmax, ymax, sz[b0], rgz, rgz);
+    wstore_w<TY, 2, 0, true>(A, k0, x0, y0, xmax, ymax, sy[b0], rgy, rgy);
+    wstore_w<TX, 0, 2, true>(A, k0, x0, y0, xmax, ymax, sx[b0], rgx, rgx);
+    wload<TZ, 2, 2, true>(A, 2, k0 + 1, x0, y0, xmax, ymax, rgz, rgz);
+    wload<TY, 2, 0, true>(A, 1, k0 + 1, x0, y0, xmax, ymax, rgy, rgy);
+    wload<TX, 0, 2, true>(A, 0, k0 + 1, x0, y0, xmax, ymax, rgx, rgx);
     size_t c = (size_t)k0 * pl + col;
     double l0 = lE[c], l1 = lE[N + c], l2 = lE[2 * N + c];
     for (int k = k0; k < k1; ++k) {
-        wstore<TZ>(sz[b1], rgz);
-        wstore<TY>(sy[b1], rgy);
-        wstore<TX>(sx[b1], rgx);
+        wstore_w<TZ, 2, 2, true>(A, k + 1, x0, y0, xmax, ymax, sz[b1], rgz, rgz);
+        wstore_w<TY, 2, 0, true>(A, k + 1, x0, y0, xmax, ymax, sy[b1], rgy, rgy);
+        wstore_w<TX, 0, 2, true>(A, k + 1, x0, y0, xmax, ymax, sx[b1], rgx, rgx);
         __syncthreads();
         const bool more = k + 1 < k1;
         const size_t cn = c + pl;
         double n0 = 0.0, n1 = 0.0, n2 = 0.0;
         if (more) {
-            wload<TZ, 2, 2, true>(A, 2, k + 2, x0, y0, xmax, ymax, rgz);
-            wload<TY, 2, 0, true>(A, 1, k + 2, x0, y0, xmax, ymax, rgy);
-            wload<TX, 0, 2, true>(A, 0, k + 2, x0, y0, xmax, ymax, rgx);
+            wload<TZ, 2, 2, true>(A, 2, k + 2, x0, y0, xmax, ymax, rgz, rgz);
+            wload<TY, 2, 0, true>(A, 1, k + 2, x0, y0, xmax, ymax, rgy, rgy);
+            wload<TX, 0, 2, true>(A, 0, k + 2, x0, y0, xmax, ymax, rgx, rgx);
             n0 = lE[cn];
             n1 = lE[N + cn];
             n2 = lE[2 * N + cn];
@@ -774,32 +799,33 @@ __global__ __launch_bounds__(ANT) void k_adj_el(AdjArgs A)
     double xq1 = wgt_e(A, 0, ci, cj, k0 - 1, (k0 - 1) * pl + col);
     size_t c = (size_t)k0 * pl + col;
     double rgz[TZ::R], rgy[TY::R], rgx[TX::R];
+    double ccz[TZ::R], ccy[TY::R], ccx[TX::R];   // dt/eps' of the weights' cells
     int b0 = 0, b1 = 1, b2 = 2, b3 = 3;   // planes k, k+1, k+2 (being filled), free
     for (int p = 0; p < 2; ++p) {
         const int bp = p == 0 ? b0 : b1;
-        wload<TZ, 1, 1, false>(A, 2, k0 + p, x0, y0, xmax, ymax, rgz);
-        wload<TY, 1, 0, false>(A, 1, k0 + p, x0, y0, xmax, ymax, rgy);
-        wload<TX, 0, 1, false>(A, 0, k0 + p, x0, y0, xmax, ymax, rgx);
-        wstore<TZ>(sz[bp], rgz);
-        wstore<TY>(sy[bp], rgy);
-        wstore<TX>(sx[bp], rgx);
+        wload<TZ, 1, 1, false>(A, 2, k0 + p, x0, y0, xmax, ymax, rgz, ccz);
+        wload<TY, 1, 0, false>(A, 1, k0 + p, x0, y0, xmax, ymax, rgy, ccy);
+        wload<TX, 0, 1, false>(A, 0, k0 + p, x0, y0, xmax, ymax, rgx, ccx);
+        wstore_w<TZ, 1, 1, false>(A, k0 + p, x0, y0, xmax, ymax, sz[bp], rgz, ccz);
+        wstore_w<TY, 1, 0, false>(A, k0 + p, x0, y0, xmax, ymax, sy[bp], rgy, ccy);
+        wstore_w<TX, 0, 1, false>(A, k0 + p, x0, y0, xmax, ymax, sx[bp], rgx, ccx);
     }
-    wload<TZ, 1, 1, false>(A, 2, k0 + 2, x0, y0, xmax, ymax, rgz);
-    wload<TY, 1, 0, false>(A, 1, k0 + 2, x0, y0, xmax, ymax, rgy);
-    wload<TX, 0, 1, false>(A, 0, k0 + 2, x0, y0, xmax, ymax, rgx);
+    wload<TZ, 1, 1, false>(A, 2, k0 + 2, x0, y0, xmax, ymax, rgz, ccz);
+    wload<TY, 1, 0, false>(A, 1, k0 + 2, x0, y0, xmax, ymax, rgy, ccy);
+    wload<TX, 0, 1, false>(A, 0, k0 + 2, x0, y0, xmax, ymax, rgx, ccx);
     double h0 = lH[c], h1 = lH[N + c], h2 = lH[2 * N + c];
     for (int k = k0; k < k1; ++k) {
-        wstore<TZ>(sz[b2], rgz);
-        wstore<TY>(sy[b2], rgy);
-        wstore<TX>(sx[b2], rgx);
+        wstore_w<TZ, 1, 1, false>(A, k + 2, x0, y0, xmax, ymax, sz[b2], rgz, ccz);
+        wstore_w<TY, 1, 0, false>(A, k + 2, x0, y0, xmax, ymax, sy[b2], rgy, ccy);
+        wstore_w<TX, 0, 1, false>(A, k + 2, x0, y0, xmax, ymax, sx[b2], rgx, ccx);
         __syncthreads();
         const bool more = k + 1 < k1;
         const size_t cn = c + pl;
         double n0 = 0.0, n1 = 0.0, n2 = 0.0;
         if (more) {
-            wload<TZ, 1, 1, false>(A, 2, k + 3, x0, y0, xmax, ymax, rgz);
-            wload<TY, 1, 0, false>(A, 1, k + 3, x0, y0, xmax, ymax, rgy);
-            wload<TX, 0, 1, false>(A, 0, k + 3, x0, y0, xmax, ymax, rgx);
+            wload<TZ, 1, 1, false>(A, 2, k + 3, x0, y0, xmax, ymax, rgz, ccz);
+            wload<TY, 1, 0, false>(A, 1, k + 3, x0, y0, xmax, ymax, rgy, ccy);
+            wload<TX, 0, 1, false>(A, 0, k + 3, x0, y0, xmax, ymax, rgx, ccx);
             n0 = lH[cn];
             n1 = lH[N + cn];
             n2 = lH[2 * N + cn];
