@@ -1896,7 +1896,17 @@ struct fwi3d_grad {
     int ntiles = 0;
     hipGraphExec_t replay[2] = {nullptr, nullptr};   // segment replays: full length, last
     int replay_len[2] = {0, 0};
+    ShotDev rdesc{};                  // the shot's descriptor with E in buf as a ring (replays)
+    ShotDev *d_rdesc = nullptr;
 };
+
+// doubles between stored states: the content rounded up to 256 bytes, so that
+// each state (whose E a replay writes as a ring slot) starts as aligned as the
+// shot's own buffers
+static size_t state_pad(size_t doubles)
+{
+    return (doubles + 31) / 32 * 32;
+}
 
 static void state_layout(fwi3d_grad *g)
 {
@@ -1908,7 +1918,7 @@ static void state_layout(fwi3d_grad *g)
         g->off_psi[a] = o;
         o += mt::PSI_N * (size_t)d.slab[a];
     }
-    g->state = o;
+    g->state = state_pad(o);
 }
 
 // the shot's live state (E slot 0 -- ring 1 -- H, the psi slabs) in the
@@ -1930,6 +1940,19 @@ static mt::StateSegs state_segs(const fwi3d_grad *g)
     return v;
 }
 
+// the same without E: H and the psi slabs, stored from offset off_h of a state
+static mt::StateSegs state_segs_h(const fwi3d_grad *g)
+{
+    const mt::StateSegs all = state_segs(g);
+    mt::StateSegs v{};
+    for (int k = 1; k < all.nseg; ++k) {
+        v.live[k - 1] = all.live[k];
+        v.count[k - 1] = all.count[k];
+    }
+    v.nseg = all.nseg - 1;
+    return v;
+}
+
 // copy the shot's live state to / from a stored state (one launch)
 static int state_copy(fwi3d_grad *g, double *dst_state, bool save)
 {
@@ -1946,7 +1969,13 @@ static int set_step(fwi3d_shot *s, int n)
 }
 
 // L forward steps from the shot's current step, storing the state after step
-// t into buffer slot t + 1; captured once per segment length as a HIP graph
+// t into buffer slot t + 1; captured once per segment length as a HIP graph.
+// Slot 0 holds the state at the start.  The steps run on a copy of the shot's
+// descriptor whose E ring is the buffer itself (slot stride one state, more
+// slots than a leaf has steps): step t reads E of slot t and writes E of slot
+// t + 1 in place of the shot's E, so only H and the psi slabs are copied into
+// each slot.  The last slot's E is copied back to the shot afterwards, which
+// leaves the shot's state at the segment's end as an in-place replay would.
 static int replay_segment(fwi3d_grad *g, int L)
 {
     fwi3d_shot *s = g->s;
@@ -1955,24 +1984,30 @@ static int replay_segment(fwi3d_grad *g, int L)
         (void)hipGraphExecDestroy(g->replay[slot]);
         g->replay[slot] = nullptr;
     }
+    g->rdesc = s->desc;
+    g->rdesc.e = g->buf;
+    g->rdesc.sstride = (long long)g->state;
+    g->rdesc.ring = g->leaf + 1;
+    g->rdesc.cur = 0;
+    HIP_TRY(hipMemcpyAsync(g->d_rdesc, &g->rdesc, sizeof(ShotDev), hipMemcpyHostToDevice, s->st));
     if (!g->replay[slot]) {
         const Fam f = shot_fam(s);
-        const mt::StateSegs segs = state_segs(g);
+        const mt::StateSegs segs = state_segs_h(g);
         hipGraph_t graph = nullptr;
         std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
         HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
         int rc = FWI3D_OK;
         for (int j = 0; j < L && !rc; ++j) {
-            if ((rc = launch_eh(f, s->d_desc, 1, j, false, s->st)) ||
-                (rc = launch_eh(f, s->d_desc, 1, j, true, s->st)))
+            if ((rc = launch_eh(f, g->d_rdesc, 1, j, false, s->st)) ||
+                (rc = launch_eh(f, g->d_rdesc, 1, j, true, s->st)))
                 break;
-            if (s->nrec && mt::launch_receivers(s->d_desc, 1, j, -1, 0, s->st) != hipSuccess)
+            if (s->nrec && mt::launch_receivers(g->d_rdesc, 1, j, -1, 0, s->st) != hipSuccess)
                 rc = fail(FWI3D_EDEVICE, "receiver launch");
-            else if (mt::launch_state_copy(segs, g->buf + g->state * (j + 1), true, s->st) !=
-                     hipSuccess)
+            else if (mt::launch_state_copy(segs, g->buf + g->state * (j + 1) + g->off_h, true,
+                                           s->st) != hipSuccess)
                 rc = fail(FWI3D_EDEVICE, "state copy launch");
         }
-        if (!rc && mt::launch_advance(s->d_desc, 1, L, s->st) != hipSuccess)
+        if (!rc && mt::launch_advance(g->d_rdesc, 1, L, s->st) != hipSuccess)
             rc = fail(FWI3D_EDEVICE, "advance launch");
         hipError_t e = hipStreamEndCapture(s->st, &graph);
         if (rc)
@@ -1986,6 +2021,11 @@ static int replay_segment(fwi3d_grad *g, int L)
         g->replay_len[slot] = L;
     }
     HIP_TRY(hipGraphLaunch(g->replay[slot], s->st));
+    mt::StateSegs e{};
+    e.live[0] = s->d_e;
+    e.count[0] = 3 * (size_t)s->desc.n;
+    e.nseg = 1;
+    HIP_TRY(mt::launch_state_copy(e, g->buf + g->state * L, false, s->st));
     s->step += L;
     return FWI3D_OK;
 }
@@ -2044,7 +2084,8 @@ static std::vector<int4> adj_tiles(const fwi3d_shot *s)
 }
 
 // device bytes a gradient with this plan allocates (states, scratch, gradient,
-// trace sensitivities, profile reciprocals, per-cell tile list)
+// trace sensitivities, profile reciprocals, per-cell tile list; the replay's
+// copy of the shot descriptor, under 1 KB, is not counted)
 static long long grad_plan_bytes(const fwi3d_shot *s, const GradPlan &P)
 {
     const size_t N = (size_t)s->desc.n, st = state_doubles(s);
@@ -2052,7 +2093,7 @@ static long long grad_plan_bytes(const fwi3d_shot *s, const GradPlan &P)
     const size_t other = tmp + N + 1 + std::max<size_t>(1, 3 * (size_t)s->nrec * s->it) +
                          2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz) +
                          2 * adj_tiles(s).size();
-    return 8LL * (long long)(P.states * st + other);
+    return 8LL * (long long)(P.states * state_pad(st) + other);
 }
 
 static int grad_new(fwi3d_shot *s, int levels, int segment, fwi3d_grad **out)
@@ -2093,7 +2134,8 @@ static int grad_new(fwi3d_shot *s, int levels, int segment, fwi3d_grad **out)
         dalloc(&g->adj, g->state) != hipSuccess || dalloc(&g->tmp, tmp) != hipSuccess ||
         dalloc(&g->grad, N) != hipSuccess || dalloc(&g->grad_src, 1) != hipSuccess ||
         dalloc(&g->g, std::max<size_t>(1, 3 * (size_t)s->nrec * s->it)) != hipSuccess ||
-        dalloc(&g->rp, 2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz)) != hipSuccess) {
+        dalloc(&g->rp, 2 * ((size_t)s->desc.ix + s->desc.iy + s->desc.iz)) != hipSuccess ||
+        dalloc(&g->d_rdesc, 1) != hipSuccess) {
         fwi3d_grad_destroy(g);
         return fail(FWI3D_ENOMEM, "gradient storage (%zu states of %zu doubles)",
                     g->plan.states, g->state);
@@ -2170,6 +2212,7 @@ int fwi3d_grad_destroy(fwi3d_grad *g)
     dfree(g->g);
     dfree(g->rp);
     dfree(g->tiles);
+    dfree(g->d_rdesc);
     for (auto &r : g->replay)
         if (r)
             (void)hipGraphExecDestroy(r);

@@ -199,6 +199,7 @@ struct StateSegs {
     double *live[5];
     size_t count[5];
     int nseg;
+    unsigned first[6];   // first block of each segment (set by launch_state_copy)
 };
 hipError_t launch_state_copy(const StateSegs &segs, double *stored, bool save, hipStream_t st);
 // plain 16-byte copy kernel of fwi3d_diag_copy (even count, 16-byte-aligned buffers)

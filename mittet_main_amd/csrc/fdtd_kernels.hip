@@ -2453,59 +2453,83 @@ hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t
     return hipGetLastError();
 }
 
-// Copies the shot's live state to a stored state (save) or back.  Segment
-// blockIdx.y, stored right after the segments before it; 16-byte accesses
-// when both sides of the segment start 16-byte aligned.
-__global__ void k_state_copy(StateSegs segs, double *__restrict__ stored, int save)
+// Copies the shot's live state to a stored state (save) or back.  Segment k
+// is stored right after the segments before it and owns the blocks
+// [first[k], first[k + 1]); each block copies one chunk of 2 * kStateChunk
+// doubles of its segment, every lane issuing its kStateUnroll loads before any
+// store, as 16-byte units when both sides of the segment start 16-byte
+// aligned, else as doubles.  Blocks are sized per segment, so a short segment
+// (a psi slab) launches no idle blocks.
+constexpr int kStateLanes = 256, kStateUnroll = 4;
+constexpr size_t kStateChunk = (size_t)kStateLanes * kStateUnroll;
+
+__global__ __launch_bounds__(kStateLanes) void k_state_copy(StateSegs segs, double *__restrict__ stored,
+                                                            int save)
 {
-    const int k = blockIdx.y;
+    const unsigned b = blockIdx.x;
+    int k = 0;
+    while (k + 1 < segs.nseg && b >= segs.first[k + 1])
+        ++k;
     size_t base = 0;
-    for (int j = 0; j < k; ++j)
-        base += segs.count[j];
-    double *__restrict__ live = segs.live[k];
-    double *__restrict__ st = stored + base;
+    for (int q = 0; q < k; ++q)
+        base += segs.count[q];
+    double *live = segs.live[k], *st = stored + base;
+    const double *__restrict__ src = save ? live : st;
+    double *__restrict__ dst = save ? st : live;
     const size_t n = segs.count[k];
-    const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t blk = b - segs.first[k];
     if ((((uintptr_t)live | (uintptr_t)st) & 15) == 0) {
-        double2 *__restrict__ a = reinterpret_cast<double2 *>(live);
-        double2 *__restrict__ b = reinterpret_cast<double2 *>(st);
-        for (size_t q = t0; q < n / 2; q += stride) {
-            if (save)
-                b[q] = a[q];
-            else
-                a[q] = b[q];
+        const double2 *__restrict__ s2 = reinterpret_cast<const double2 *>(src);
+        double2 *__restrict__ d2 = reinterpret_cast<double2 *>(dst);
+        const size_t n2 = n / 2, u0 = blk * kStateChunk + threadIdx.x;
+        double2 v[kStateUnroll];
+#pragma unroll
+        for (int q = 0; q < kStateUnroll; ++q) {
+            const size_t u = u0 + (size_t)q * kStateLanes;
+            if (u < n2)
+                v[q] = s2[u];
         }
-        if (t0 == 0 && (n & 1)) {
-            if (save)
-                st[n - 1] = live[n - 1];
-            else
-                live[n - 1] = st[n - 1];
+#pragma unroll
+        for (int q = 0; q < kStateUnroll; ++q) {
+            const size_t u = u0 + (size_t)q * kStateLanes;
+            if (u < n2)
+                d2[u] = v[q];
         }
+        if ((n & 1) && blk == 0 && threadIdx.x == 0)
+            dst[n - 1] = src[n - 1];
     } else {
-        for (size_t q = t0; q < n; q += stride) {
-            if (save)
-                st[q] = live[q];
-            else
-                live[q] = st[q];
+        const size_t e0 = blk * 2 * kStateChunk + threadIdx.x;
+        double v[2 * kStateUnroll];
+#pragma unroll
+        for (int q = 0; q < 2 * kStateUnroll; ++q) {
+            const size_t e = e0 + (size_t)q * kStateLanes;
+            if (e < n)
+                v[q] = src[e];
+        }
+#pragma unroll
+        for (int q = 0; q < 2 * kStateUnroll; ++q) {
+            const size_t e = e0 + (size_t)q * kStateLanes;
+            if (e < n)
+                dst[e] = v[q];
         }
     }
 }
 
-hipError_t launch_state_copy(const StateSegs &segs, double *stored, bool save, hipStream_t st)
+hipError_t launch_state_copy(const StateSegs &segs_in, double *stored, bool save, hipStream_t st)
 {
-    size_t most = 0;
-    for (int k = 0; k < segs.nseg; ++k)
-        most = segs.count[k] > most ? segs.count[k] : most;
-    if (most == 0)
+    StateSegs segs = segs_in;
+    size_t nb = 0;
+    for (int k = 0; k < segs.nseg; ++k) {
+        segs.first[k] = (unsigned)nb;
+        nb += (segs.count[k] + 2 * kStateChunk - 1) / (2 * kStateChunk);
+    }
+    segs.first[segs.nseg] = (unsigned)nb;
+    if (nb == 0)
         return hipSuccess;
-    size_t blocks = (most / 2 + 255) / 256;
-    if (blocks > 65536)
-        blocks = 65536;
-    if (blocks == 0)
-        blocks = 1;
-    hipLaunchKernelGGL(k_state_copy, dim3((unsigned)blocks, (unsigned)segs.nseg), dim3(256), 0, st,
-                       segs, stored, save ? 1 : 0);
+    if (nb > 0xffffffffu)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_state_copy, dim3((unsigned)nb), dim3(kStateLanes), 0, st, segs, stored,
+                       save ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -214,8 +214,8 @@ def _adjoint_tile_doubles(grid: Grid) -> int:
 
 def plan_bytes(grid: Grid, nt: int, nrec: int, levels: int) -> int:
     """Device bytes of a gradient's checkpoint storage (host-side mirror of
-    fwi3d_grad_plan_bytes: states of 6 fields + shell psi, scratch, gradient,
-    the adjoint's tile list)."""
+    fwi3d_grad_plan_bytes: states of 6 fields + shell psi, each padded to
+    256 bytes, scratch, gradient, the adjoint's tile list)."""
     import ctypes as C
     sl = (C.c_int * levels)()
     ns, st = C.c_int(0), C.c_longlong(0)
@@ -225,7 +225,7 @@ def plan_bytes(grid: Grid, nt: int, nrec: int, levels: int) -> int:
     state = 6 * n + psi
     other = ((state - 3 * n) + n + 1 + max(1, 3 * nrec * (nt + 1)) +
              2 * (grid.ix + grid.iy + grid.iz) + _adjoint_tile_doubles(grid))
-    return 8 * (st.value * state + other)
+    return 8 * (st.value * ((state + 31) // 32 * 32) + other)
 
 
 def run_gradient(req: Request, budget: float) -> Result:
