@@ -166,7 +166,8 @@ __device__ __forceinline__ double wgt_e(const AdjArgs &A, int d, int i, int j, i
     const bool in = in_region(A, i, j, k);
     const GLOBAL double *fin = (const GLOBAL double *)A.lE, *fout = (const GLOBAL double *)A.t3;
     const double v = (in ? fin : fout)[(size_t)d * A.n + c];
-    return in ? ((const GLOBAL double *)A.cbi)[c] * v : v;
+    const double cb = ((const GLOBAL double *)A.cbi)[c];
+    return in ? cb * v : v;
 }
 
 // Cell-local CPML-free interior [nc, n - nc): ca = da = 1, cb = dt/eps',
@@ -222,13 +223,20 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
     const double dt = A.dt, c1 = A.c1, c2 = A.c2;
     const double dbi = dt / A.mu0;
     if (__all(inner(A, i, j, k))) {
+        double l[3];
         for (int d = 0; d < 3; ++d)
-            A.t3[(size_t)d * N + c] = dbi * A.lH[(size_t)d * N + c];
+            l[d] = A.lH[(size_t)d * N + c];
+        for (int d = 0; d < 3; ++d)
+            A.t3[(size_t)d * N + c] = dbi * l[d];
         return;
     }
     const int idx[3] = {i, j, k};
     const long long st[3] = {1, sy, sz};
+    // Every store comes after the last load: the vector-memory counter counts
+    // loads and stores in issue order, so a wait for a load issued after a
+    // store would also wait for that store.
     const double e = A.eps[c];
+    const double g0 = A.grad[c];
     // one reciprocal per axis: r = 1/(e + Km), Km = msig dt/2; then
     // db = dt/mu0 e r, da = (e - Km) r, dda/de = 2 Km r^2, ddb/de = dt/mu0 Km r^2
     double lh[3], ms[3], km[3], rr[3], db[3];
@@ -239,14 +247,15 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
         rr[d] = 1.0 / (e + km[d]);
         db[d] = dbi * e * rr[d];
     }
-    double g = 0.0;
+    double g = 0.0, w[3];
+    bool up[3];
     const GLOBAL double *E1 = (const GLOBAL double *)A.uE1;
     const GLOBAL double *H0 = (const GLOBAL double *)A.uH0;
     for (int d = 0; d < 3; ++d) {
         const int a = (d + 1) % 3, b = (d + 2) % 3;
-        const bool up = in_h(A, d, i, j, k);
-        A.t3[(size_t)d * N + c] = up ? db[d] * lh[d] : 0.0;
-        if (up && ms[d] != 0.0) {
+        up[d] = in_h(A, d, i, j, k);
+        w[d] = up[d] ? db[d] * lh[d] : 0.0;
+        if (up[d] && ms[d] != 0.0) {
             // gradient: lH_d (dda/de H_d^n - ddb/de curl+_d(E^{n+1}))
             const GLOBAL double *Eb = E1 + (size_t)b * N, *Ea = E1 + (size_t)a * N;
             const double r2 = rr[d] * rr[d];
@@ -258,57 +267,76 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h1(AdjArgs A)
             g += lh[d] * (da_de * H0[(size_t)d * N + c] - db_de * cu);
         }
     }
+    bool fc[3];
+    double m0[3], m1[3], bh[3], ch[3];
     for (int a = 0; a < 3; ++a) {
-        if (!face_h(A, a, i, j, k))
+        fc[a] = face_h(A, a, i, j, k);
+        if (!fc[a])
             continue;
         const int p = (a + 1) % 3, q = (a + 2) % 3;
         const size_t L = (size_t)A.slab[a], s = slabq(a, A, i, j, k);
-        double *P = A.lpsi[a];
-        double *T = A.tpsi[a];
-        const double bh = prof(A, a, PR_BH, idx[a]), ch = prof(A, a, PR_CH, idx[a]);
+        const GLOBAL double *P = (const GLOBAL double *)A.lpsi[a];
+        bh[a] = prof(A, a, PR_BH, idx[a]);
+        ch[a] = prof(A, a, PR_CH, idx[a]);
         // psiH0 feeds H_p (+db_p), psiH1 feeds H_q (-db_q)
-        const double m0 = P[2 * L + s] + db[p] * lh[p];
-        const double m1 = P[3 * L + s] - db[q] * lh[q];
-        P[2 * L + s] = bh * m0;
-        P[3 * L + s] = bh * m1;
-        T[2 * L + s] = ch * m0 * A.rdl[a];
-        T[3 * L + s] = ch * m1 * A.rdl[a];
+        m0[a] = P[2 * L + s] + db[p] * lh[p];
+        m1[a] = P[3 * L + s] - db[q] * lh[q];
         // gradient through db in H_p += db_p psi0', H_q -= db_q psi1'
         const GLOBAL double *Q1 = (const GLOBAL double *)A.upsi1[a];
         const double dbp = dbi * km[p] * rr[p] * rr[p];
         const double dbq = dbi * km[q] * rr[q] * rr[q];
         g += lh[p] * dbp * Q1[2 * L + s] - lh[q] * dbq * Q1[3 * L + s];
     }
+    for (int d = 0; d < 3; ++d)
+        A.t3[(size_t)d * N + c] = w[d];
+    for (int a = 0; a < 3; ++a) {
+        if (!fc[a])
+            continue;
+        const size_t L = (size_t)A.slab[a], s = slabq(a, A, i, j, k);
+        double *P = A.lpsi[a];
+        double *T = A.tpsi[a];
+        P[2 * L + s] = bh[a] * m0[a];
+        P[3 * L + s] = bh[a] * m1[a];
+        T[2 * L + s] = ch[a] * m0[a] * A.rdl[a];
+        T[3 * L + s] = ch[a] * m1[a] * A.rdl[a];
+    }
     for (int d = 0; d < 3; ++d) {
-        if (in_h(A, d, i, j, k) && ms[d] != 0.0)
+        if (up[d] && ms[d] != 0.0)
             A.lH[(size_t)d * N + c] = lh[d] * (e - km[d]) * rr[d];
     }
     if (g != 0.0)
-        A.grad[c] += g;
+        A.grad[c] = g0 + g;
 }
 
 // weighted value at a neighbour for a transposed stencil (0 outside the grid);
-// h: H-phase weights and 1/khd, else E-phase weights and 1/ked
+// h: H-phase weights and 1/khd, else E-phase weights and 1/ked.  A neighbour
+// outside the grid reads the nearest cell and its value is dropped, so that
+// no load waits behind a branch.
 __device__ __forceinline__ double wnb(const AdjArgs &A, int comp, int ni, int nj, int nk, int axis,
                                       bool h)
 {
-    if (ni < 0 || nj < 0 || nk < 0 || ni >= A.ix || nj >= A.iy || nk >= A.iz)
-        return 0.0;
+    const bool out = ni < 0 || nj < 0 || nk < 0 || ni >= A.ix || nj >= A.iy || nk >= A.iz;
+    ni = min(max(ni, 0), A.ix - 1);
+    nj = min(max(nj, 0), A.iy - 1);
+    nk = min(max(nk, 0), A.iz - 1);
     const int c = (nk * A.iy + nj) * A.ix + ni;
     const int id[3] = {ni, nj, nk};
     const double w = h ? wgt_h(A, comp, ni, nj, nk, c) : wgt_e(A, comp, ni, nj, nk, c);
-    return w * rprof(A, axis, h, id[axis]);
+    const double v = w * rprof(A, axis, h, id[axis]);
+    return out ? 0.0 : v;
 }
 
 // Both psi weight rows (r0, r1) of axis a at the four neighbours of a cell
-// along a (offsets off[0..3]); 0 where the neighbour is not on a CPML face
-// (HF: the CPML-H faces [1,nc-1)U[n-nc,n-2), else the CPML-E faces) or lies
-// on the last plane of another axis.  The tests of the other two axes and the
-// slab-index base are done once per cell.
+// along a (offsets off[0..3]), raw, and on[q]: whether the neighbour is on a
+// CPML face (HF: the CPML-H faces [1,nc-1)U[n-nc,n-2), else the CPML-E faces)
+// and not on the last plane of another axis; the weight is 0 where it is not
+// (the caller drops the raw value, read from a slab row clamped into the
+// slab, after every axis' loads are in flight).  The tests of the other two
+// axes and the slab-index base are done once per cell.
 template <bool HF>
 __device__ __forceinline__ void psi_rows(const AdjArgs &A, int a, int i, int j, int k, int r0,
                                          int r1, const int (&off)[4], double (&w0)[4],
-                                         double (&w1)[4])
+                                         double (&w1)[4], bool (&on)[4])
 {
     const int nc = A.nc;
     int r, n;
@@ -343,15 +371,11 @@ __device__ __forceinline__ void psi_rows(const AdjArgs &A, int a, int i, int j, 
             face = (rr >= 1 && rr < nc - 1) || (rr >= n - nc && rr < n - 2);
         else
             face = (rr >= 2 && rr < (a == 0 ? nc - 1 : nc)) || (rr >= n - nc + 1 && rr < n - 2);
-        double v0 = 0.0, v1 = 0.0;
-        if (others && face) {
-            const int sp = rr < nc ? rr : rr - (n - 2 * nc);
-            const size_t qq = base + (size_t)sp * stride;
-            v0 = T[(size_t)r0 * L + qq];
-            v1 = T[(size_t)r1 * L + qq];
-        }
-        w0[q] = v0;
-        w1[q] = v1;
+        const int sp = min(max(rr < nc ? rr : rr - (n - 2 * nc), 0), 2 * nc - 1);
+        const size_t qq = base + (size_t)sp * stride;
+        w0[q] = T[(size_t)r0 * L + qq];
+        w1[q] = T[(size_t)r1 * L + qq];
+        on[q] = others && face;
     }
 }
 
@@ -364,6 +388,12 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
         return;
     const double c1 = A.c1, c2 = A.c2;
     const int N = A.n;
+    // the cell's own lE first: loaded after the gathers' stores of earlier
+    // components, each load would wait for those stores
+    double l0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        l0[d] = A.lE[(size_t)d * N + c];
     double acc[3] = {0.0, 0.0, 0.0};
     if (A.kappa_one && __all(core(A, i, j, k))) {
         const int st[3] = {1, A.ix, A.ix * A.iy};
@@ -376,6 +406,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
             };
             return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
         };
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
             acc[b] += dm(d, a) * A.rdl[a];
@@ -390,6 +421,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
             };
             return c1 * at(0) - c1 * at(-1) + c2 * at(1) - c2 * at(-2);
         };
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
             // H_d -= db (D+_a E_b/khd_a - D+_b E_a/khd_b):  lE_b += D-_a(k/khd_a), lE_a -= D-_b(k/khd_b)
@@ -398,21 +430,35 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_h2(AdjArgs A)
         }
     }
     const int idx[3] = {i, j, k};
+    // the psi rows of every axis near a face are loaded before any is used
+    bool act[3], on[3][4];
+    double w2[3][4], w3[3][4];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (!__any(near_face(A, a, idx[a])))
-            continue;
-        const int p = (a + 1) % 3, q = (a + 2) % 3;
         // psiH0 <- D+_a E_q, psiH1 <- D+_a E_p; transpose of D+ is -D-
         // (offsets 0, -1, +1, -2 of D-)
         const int off[4] = {0, -1, 1, -2};
-        double w2[4], w3[4];
-        psi_rows<true>(A, a, i, j, k, 2, 3, off, w2, w3);
-        acc[q] -= c1 * w2[0] - c1 * w2[1] + c2 * w2[2] - c2 * w2[3];
-        acc[p] -= c1 * w3[0] - c1 * w3[1] + c2 * w3[2] - c2 * w3[3];
+        act[a] = __any(near_face(A, a, idx[a]));
+        if (act[a])
+            psi_rows<true>(A, a, i, j, k, 2, 3, off, w2[a], w3[a], on[a]);
     }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!act[a])
+            continue;
+        const int p = (a + 1) % 3, q = (a + 2) % 3;
+        double u2[4], u3[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            u2[r] = on[a][r] ? w2[a][r] : 0.0;
+            u3[r] = on[a][r] ? w3[a][r] : 0.0;
+        }
+        acc[q] -= c1 * u2[0] - c1 * u2[1] + c2 * u2[2] - c2 * u2[3];
+        acc[p] -= c1 * u3[0] - c1 * u3[1] + c2 * u3[2] - c2 * u3[3];
+    }
+#pragma unroll
     for (int d = 0; d < 3; ++d)
-        A.lE[(size_t)d * N + c] += acc[d];
+        A.lE[(size_t)d * N + c] = l0[d] + acc[d];
     if (c == A.src_cell) {
         // E[src] -= amp (amp ~ 1/sigma): d/dsigma = +amp/sigma  (kept in sigma units)
         A.grad_src[0] += A.lE[(size_t)A.src_comp * N + c] * A.amp_n / A.sig_src;
@@ -431,19 +477,23 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
     const GLOBAL double *H0 = (const GLOBAL double *)A.uH0;
     if (__all(inner(A, i, j, k))) {
         const double r = 1.0 / e, cb = dt * r, cbd = -dt * r * r;
-        double g = 0.0;
+        const double g0 = A.grad[c];
+        double g = 0.0, le[3];
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
-            const double le = A.lE[(size_t)d * N + c];
-            A.t3[(size_t)d * N + c] = cb * le;
+            le[d] = A.lE[(size_t)d * N + c];
             const double cu = dminus_at(H0 + (size_t)b * N, c, (int)st[a], c1, c2) * A.rdl[a] -
                               dminus_at(H0 + (size_t)a * N, c, (int)st[b], c1, c2) * A.rdl[b];
-            g += le * cbd * cu;
+            g += le[d] * cbd * cu;
         }
-        A.grad[c] += g;
+        for (int d = 0; d < 3; ++d)
+            A.t3[(size_t)d * N + c] = cb * le[d];
+        A.grad[c] = g0 + g;
         return;
     }
     const int idx[3] = {i, j, k};
+    // every store after the last load (as in k_adj_h1)
+    const double g0 = A.grad[c];
     // one reciprocal per axis: r = 1/(e + K), K = esig dt/2; then cb = dt r,
     // ca = (e - K) r, dca/de = 2 K r^2, dcb/de = -dt r^2
     double le[3], ke[3], rr[3], cb[3], cbd[3];
@@ -454,13 +504,14 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
         cb[d] = dt * rr[d];
         cbd[d] = -dt * rr[d] * rr[d];
     }
-    double g = 0.0;
+    double g = 0.0, w[3];
+    bool up[3];
     const GLOBAL double *E0 = (const GLOBAL double *)A.uE0;
     for (int d = 0; d < 3; ++d) {
         const int a = (d + 1) % 3, b = (d + 2) % 3;
-        const bool up = in_e(A, d, i, j, k);
-        A.t3[(size_t)d * N + c] = up ? cb[d] * le[d] : 0.0;
-        if (up) {
+        up[d] = in_e(A, d, i, j, k);
+        w[d] = up[d] ? cb[d] * le[d] : 0.0;
+        if (up[d]) {
             const GLOBAL double *Hb = H0 + (size_t)b * N, *Ha = H0 + (size_t)a * N;
             const double dma = c1 * Hb[c] - c1 * Hb[c - st[a]] + c2 * Hb[c + st[a]] - c2 * Hb[c - 2 * st[a]];
             const double dmb = c1 * Ha[c] - c1 * Ha[c - st[b]] + c2 * Ha[c + st[b]] - c2 * Ha[c - 2 * st[b]];
@@ -471,28 +522,40 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e1(AdjArgs A)
             g += le[d] * gd;
         }
     }
+    bool fc[3];
+    double m0[3], m1[3], be[3], ce[3];
     for (int a = 0; a < 3; ++a) {
-        if (!face_e(A, a, i, j, k))
+        fc[a] = face_e(A, a, i, j, k);
+        if (!fc[a])
             continue;
         const int p = (a + 1) % 3, q = (a + 2) % 3;
         const size_t L = (size_t)A.slab[a], s = slabq(a, A, i, j, k);
-        double *P = A.lpsi[a];
-        double *T = A.tpsi[a];
-        const double be = prof(A, a, PR_BE, idx[a]), ce = prof(A, a, PR_CE, idx[a]);
+        const GLOBAL double *P = (const GLOBAL double *)A.lpsi[a];
+        be[a] = prof(A, a, PR_BE, idx[a]);
+        ce[a] = prof(A, a, PR_CE, idx[a]);
         // psiE0 feeds E_q (+cb_q), psiE1 feeds E_p (-cb_p)
-        const double m0 = P[s] + cb[q] * le[q];
-        const double m1 = P[L + s] - cb[p] * le[p];
-        P[s] = be * m0;
-        P[L + s] = be * m1;
-        T[s] = ce * m0 * A.rdl[a];
-        T[L + s] = ce * m1 * A.rdl[a];
+        m0[a] = P[s] + cb[q] * le[q];
+        m1[a] = P[L + s] - cb[p] * le[p];
         const GLOBAL double *Q1 = (const GLOBAL double *)A.upsi1[a];
         g += le[q] * cbd[q] * Q1[s] - le[p] * cbd[p] * Q1[L + s];
     }
     for (int d = 0; d < 3; ++d)
-        if (in_e(A, d, i, j, k) && ke[d] != 0.0)
+        A.t3[(size_t)d * N + c] = w[d];
+    for (int a = 0; a < 3; ++a) {
+        if (!fc[a])
+            continue;
+        const size_t L = (size_t)A.slab[a], s = slabq(a, A, i, j, k);
+        double *P = A.lpsi[a];
+        double *T = A.tpsi[a];
+        P[s] = be[a] * m0[a];
+        P[L + s] = be[a] * m1[a];
+        T[s] = ce[a] * m0[a] * A.rdl[a];
+        T[L + s] = ce[a] * m1[a] * A.rdl[a];
+    }
+    for (int d = 0; d < 3; ++d)
+        if (up[d] && ke[d] != 0.0)
             A.lE[(size_t)d * N + c] = le[d] * (e - ke[d]) * rr[d];
-    A.grad[c] += g;
+    A.grad[c] = g0 + g;
 }
 
 // E-phase part 2 (gathers): lH += transposed curl- of the weights and the
@@ -504,6 +567,10 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
         return;
     const double c1 = A.c1, c2 = A.c2;
     const int N = A.n;
+    double l0[3];   // the cell's own lH first (as in k_adj_h2)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        l0[d] = A.lH[(size_t)d * N + c];
     double acc[3] = {0.0, 0.0, 0.0};
     if (A.kappa_one && __all(core(A, i, j, k))) {
         const int st[3] = {1, A.ix, A.ix * A.iy};
@@ -516,6 +583,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
             };
             return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
         };
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
             acc[b] -= dp(d, a) * A.rdl[a];
@@ -530,6 +598,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
             };
             return c1 * at(1) - c1 * at(0) + c2 * at(2) - c2 * at(-1);
         };
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int a = (d + 1) % 3, b = (d + 2) % 3;
             // E_d += cb (D-_a H_b/ked_a - D-_b H_a/ked_b): lH_b -= D+_a(nu/ked_a), lH_a += D+_b(nu/ked_b)
@@ -538,21 +607,35 @@ __global__ __launch_bounds__(ZX *ZY) void k_adj_e2(AdjArgs A)
         }
     }
     const int idx[3] = {i, j, k};
+    // the psi rows of every axis near a face are loaded before any is used
+    bool act[3], on[3][4];
+    double w0[3][4], w1[3][4];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (!__any(near_face(A, a, idx[a])))
-            continue;
-        const int p = (a + 1) % 3, q = (a + 2) % 3;
         // psiE0 <- D-_a H_p, psiE1 <- D-_a H_q; transpose of D- is -D+
         // (offsets +1, 0, +2, -1 of D+)
         const int off[4] = {1, 0, 2, -1};
-        double w0[4], w1[4];
-        psi_rows<false>(A, a, i, j, k, 0, 1, off, w0, w1);
-        acc[p] -= c1 * w0[0] - c1 * w0[1] + c2 * w0[2] - c2 * w0[3];
-        acc[q] -= c1 * w1[0] - c1 * w1[1] + c2 * w1[2] - c2 * w1[3];
+        act[a] = __any(near_face(A, a, idx[a]));
+        if (act[a])
+            psi_rows<false>(A, a, i, j, k, 0, 1, off, w0[a], w1[a], on[a]);
     }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!act[a])
+            continue;
+        const int p = (a + 1) % 3, q = (a + 2) % 3;
+        double u0[4], u1[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            u0[r] = on[a][r] ? w0[a][r] : 0.0;
+            u1[r] = on[a][r] ? w1[a][r] : 0.0;
+        }
+        acc[p] -= c1 * u0[0] - c1 * u0[1] + c2 * u0[2] - c2 * u0[3];
+        acc[q] -= c1 * u1[0] - c1 * u1[1] + c2 * u1[2] - c2 * u1[3];
+    }
+#pragma unroll
     for (int d = 0; d < 3; ++d)
-        A.lH[(size_t)d * N + c] += acc[d];
+        A.lH[(size_t)d * N + c] = l0[d] + acc[d];
 }
 
 // ------------------------------------------------------------------------

@@ -1974,8 +1974,10 @@ static int set_step(fwi3d_shot *s, int n)
 // descriptor whose E ring is the buffer itself (slot stride one state, more
 // slots than a leaf has steps): step t reads E of slot t and writes E of slot
 // t + 1 in place of the shot's E, so only H and the psi slabs are copied into
-// each slot.  The last slot's E is copied back to the shot afterwards, which
-// leaves the shot's state at the segment's end as an in-place replay would.
+// each slot, and each step's receiver gather rides on the next step's E
+// launch as in a ring shot's chunk.  The last slot's E is copied back to the
+// shot afterwards, which leaves the shot's state at the segment's end as an
+// in-place replay would.
 static int replay_segment(fwi3d_grad *g, int L)
 {
     fwi3d_shot *s = g->s;
@@ -1991,17 +1993,20 @@ static int replay_segment(fwi3d_grad *g, int L)
     g->rdesc.cur = 0;
     HIP_TRY(hipMemcpyAsync(g->d_rdesc, &g->rdesc, sizeof(ShotDev), hipMemcpyHostToDevice, s->st));
     if (!g->replay[slot]) {
-        const Fam f = shot_fam(s);
+        Fam f = shot_fam(s);
+        f.rings = true;   // the stored states: an E ring of leaf + 1 slots
+        const bool fold = s->nrec && rec_folds(f);
         const mt::StateSegs segs = state_segs_h(g);
         hipGraph_t graph = nullptr;
         std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
         HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
         int rc = FWI3D_OK;
         for (int j = 0; j < L && !rc; ++j) {
-            if ((rc = launch_eh(f, g->d_rdesc, 1, j, false, s->st)) ||
+            if ((rc = launch_eh(f, g->d_rdesc, 1, j, false, s->st, fold && j > 0)) ||
                 (rc = launch_eh(f, g->d_rdesc, 1, j, true, s->st)))
                 break;
-            if (s->nrec && mt::launch_receivers(g->d_rdesc, 1, j, -1, 0, s->st) != hipSuccess)
+            if (s->nrec && !(fold && j + 1 < L) &&
+                mt::launch_receivers(g->d_rdesc, 1, j, -1, 0, s->st) != hipSuccess)
                 rc = fail(FWI3D_EDEVICE, "receiver launch");
             else if (mt::launch_state_copy(segs, g->buf + g->state * (j + 1) + g->off_h, true,
                                            s->st) != hipSuccess)
