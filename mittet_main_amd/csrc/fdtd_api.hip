@@ -1940,16 +1940,17 @@ static mt::StateSegs state_segs(const fwi3d_grad *g)
     return v;
 }
 
-// the same without E: H and the psi slabs, stored from offset off_h of a state
-static mt::StateSegs state_segs_h(const fwi3d_grad *g)
+// the segments from `first` on (1: H and the psi slabs, stored from offset
+// off_h of a state; 2: the psi slabs, from off_psi[0])
+static mt::StateSegs state_segs_from(const fwi3d_grad *g, int first)
 {
     const mt::StateSegs all = state_segs(g);
     mt::StateSegs v{};
-    for (int k = 1; k < all.nseg; ++k) {
-        v.live[k - 1] = all.live[k];
-        v.count[k - 1] = all.count[k];
+    for (int k = first; k < all.nseg; ++k) {
+        v.live[k - first] = all.live[k];
+        v.count[k - first] = all.count[k];
     }
-    v.nseg = all.nseg - 1;
+    v.nseg = all.nseg - first;
     return v;
 }
 
@@ -1973,11 +1974,13 @@ static int set_step(fwi3d_shot *s, int n)
 // Slot 0 holds the state at the start.  The steps run on a copy of the shot's
 // descriptor whose E ring is the buffer itself (slot stride one state, more
 // slots than a leaf has steps): step t reads E of slot t and writes E of slot
-// t + 1 in place of the shot's E, so only H and the psi slabs are copied into
-// each slot, and each step's receiver gather rides on the next step's E
-// launch as in a ring shot's chunk.  The last slot's E is copied back to the
-// shot afterwards, which leaves the shot's state at the segment's end as an
-// in-place replay would.
+// t + 1 in place of the shot's E, and each step's receiver gather rides on the
+// next step's E launch as in a ring shot's chunk.  For the per-cell and LDS
+// kernel families H goes the same way (a ring of the states' H parts, see
+// hsel), so only the psi slabs are copied into each slot; the other families
+// update H in place and copy it too.  The last slot's E and H are copied back
+// to the shot afterwards, which leaves the shot's state at the segment's end
+// as an in-place replay would.
 static int replay_segment(fwi3d_grad *g, int L)
 {
     fwi3d_shot *s = g->s;
@@ -1986,17 +1989,24 @@ static int replay_segment(fwi3d_grad *g, int L)
         (void)hipGraphExecDestroy(g->replay[slot]);
         g->replay[slot] = nullptr;
     }
+    const Fam f0 = shot_fam(s);
+    const bool hring = (f0.kind == 0 || f0.kind == 3) && !f0.fused;
     g->rdesc = s->desc;
     g->rdesc.e = g->buf;
     g->rdesc.sstride = (long long)g->state;
     g->rdesc.ring = g->leaf + 1;
     g->rdesc.cur = 0;
+    if (hring) {
+        g->rdesc.h = g->buf + g->off_h;
+        g->rdesc.hstride = (long long)g->state;
+    }
     HIP_TRY(hipMemcpyAsync(g->d_rdesc, &g->rdesc, sizeof(ShotDev), hipMemcpyHostToDevice, s->st));
     if (!g->replay[slot]) {
-        Fam f = shot_fam(s);
+        Fam f = f0;
         f.rings = true;   // the stored states: an E ring of leaf + 1 slots
         const bool fold = s->nrec && rec_folds(f);
-        const mt::StateSegs segs = state_segs_h(g);
+        const mt::StateSegs segs = state_segs_from(g, hring ? 2 : 1);
+        const size_t soff = hring ? g->off_psi[0] : g->off_h;
         hipGraph_t graph = nullptr;
         std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
         HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
@@ -2008,7 +2018,7 @@ static int replay_segment(fwi3d_grad *g, int L)
             if (s->nrec && !(fold && j + 1 < L) &&
                 mt::launch_receivers(g->d_rdesc, 1, j, -1, 0, s->st) != hipSuccess)
                 rc = fail(FWI3D_EDEVICE, "receiver launch");
-            else if (mt::launch_state_copy(segs, g->buf + g->state * (j + 1) + g->off_h, true,
+            else if (mt::launch_state_copy(segs, g->buf + g->state * (j + 1) + soff, true,
                                            s->st) != hipSuccess)
                 rc = fail(FWI3D_EDEVICE, "state copy launch");
         }
@@ -2026,11 +2036,14 @@ static int replay_segment(fwi3d_grad *g, int L)
         g->replay_len[slot] = L;
     }
     HIP_TRY(hipGraphLaunch(g->replay[slot], s->st));
-    mt::StateSegs e{};
-    e.live[0] = s->d_e;
-    e.count[0] = 3 * (size_t)s->desc.n;
-    e.nseg = 1;
-    HIP_TRY(mt::launch_state_copy(e, g->buf + g->state * L, false, s->st));
+    const mt::StateSegs eh = state_segs(g);   // E then H, as stored
+    mt::StateSegs back{};
+    for (int k = 0; k < (hring ? 2 : 1); ++k) {
+        back.live[k] = eh.live[k];
+        back.count[k] = eh.count[k];
+    }
+    back.nseg = hring ? 2 : 1;
+    HIP_TRY(mt::launch_state_copy(back, g->buf + g->state * L, false, s->st));
     s->step += L;
     return FWI3D_OK;
 }

@@ -69,6 +69,8 @@ struct ShotDev {
     double *h2;               // second H buffer of shots that run the fused E+H interior
                               // kernel (null: none)
     int hpar;                 // 1: the current H lives in h2
+    long long hstride;        // nonzero: H in ring slots like E (slot q at h + q*hstride;
+                              // a gradient replay's stored states), see hsel
     double *psi[3];           // [PSI_N][slab[a]]
     long long slab[3];
     int src_comp;
@@ -98,9 +100,14 @@ enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
 // (hf = 1) read H from one buffer and write the other, because the halo of a
 // fused tile reads old H that neighbouring tiles overwrite: step j then reads
 // hsel(S, j, 1) and writes hsel(S, j + 1, 1).  Cells no step updates hold the
-// same value in both buffers.
+// same value in both buffers.  With hstride set, H before step j is ring slot
+// (cur + j) % ring whatever hf is, and step j writes slot (cur + j + 1) % ring;
+// only the per-cell and LDS kernel families read H through hsel(S, j, .) and
+// write it through hsel(S, j + 1, .) (the others update in place).
 __host__ __device__ inline double *hsel(const ShotDev &S, int j, int hf)
 {
+    if (S.hstride)
+        return S.h + (size_t)((S.cur + j) % S.ring) * (size_t)S.hstride;
     return ((S.hpar + (hf ? j : 0)) & 1) ? S.h2 : S.h;
 }
 

@@ -1597,7 +1597,7 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
     const int si = (S.cur + j) % S.ring, so = (S.cur + j + 1) % S.ring;
     const GLOBAL double *Ein = gc(S.e + (size_t)si * S.sstride);
     GLOBAL double *Eout = gm(S.e + (size_t)so * S.sstride);
-    const double *Hc = hsel(S, 0, 0);
+    const double *Hc = hsel(S, j, 0);
     const GLOBAL double *Hx = gc(Hc), *Hy = gc(Hc + N), *Hz = gc(Hc + 2 * N);
     const GLOBAL double *CB = gc(S.cbi);
     const double kedx = gc(S.prof[0])[PR_KED * ix + ci];
@@ -1714,8 +1714,11 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
     const int so = (S.cur + j + 1) % S.ring;
     const GLOBAL double *Ex = gc(S.e + (size_t)so * S.sstride);
     const GLOBAL double *Ey = Ex + N, *Ez = Ex + 2 * N;
-    double *Hc = hsel(S, 0, 0);
-    GLOBAL double *Hx = gm(Hc), *Hy = gm(Hc + N), *Hz = gm(Hc + 2 * N);
+    // H in place, or from ring slot j to slot j + 1 (see hsel)
+    const double *Hc = hsel(S, j, 0);
+    double *Hn = hsel(S, j + 1, 0);
+    const GLOBAL double *Hxi = gc(Hc), *Hyi = gc(Hc + N), *Hzi = gc(Hc + 2 * N);
+    GLOBAL double *Hx = gm(Hn), *Hy = gm(Hn + N), *Hz = gm(Hn + 2 * N);
     const double khdx = gc(S.prof[0])[PR_KHD * ix + ci];
     const double khdy = gc(S.prof[1])[PR_KHD * iy + cj];
     const double c1 = S.c1, c2 = S.c2;
@@ -1737,7 +1740,7 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
     tile_load<TY, 1, 0>(Ey, (size_t)(k0 + 2) * pl, ix, x0, y0, xmax, ymax, ry);
     tile_load<TX, 0, 1>(Ex, (size_t)(k0 + 2) * pl, ix, x0, y0, xmax, ymax, rx);
     size_t c = (size_t)k0 * pl + col;
-    double hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
+    double hx0 = Hxi[c], hy0 = Hyi[c], hz0 = Hzi[c];
     for (int k = k0; k < k1; ++k) {
         tile_store<TZ>(sz[b2], rz);
         tile_store<TY>(sy[b2], ry);
@@ -1750,9 +1753,9 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
             tile_load<TZ, 1, 1>(Ez, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, rz);
             tile_load<TY, 1, 0>(Ey, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, ry);
             tile_load<TX, 0, 1>(Ex, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, rx);
-            nhx = Hx[cn];
-            nhy = Hy[cn];
-            nhz = Hz[cn];
+            nhx = Hxi[cn];
+            nhy = Hyi[cn];
+            nhz = Hzi[cn];
         }
         const double khdz = uload(S.prof[2] + PR_KHD * S.iz + k);
         const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
