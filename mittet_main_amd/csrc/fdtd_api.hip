@@ -1976,11 +1976,12 @@ static int set_step(fwi3d_shot *s, int n)
 // slots than a leaf has steps): step t reads E of slot t and writes E of slot
 // t + 1 in place of the shot's E, and each step's receiver gather rides on the
 // next step's E launch as in a ring shot's chunk.  For the per-cell and LDS
-// kernel families H goes the same way (a ring of the states' H parts, see
-// hsel), so only the psi slabs are copied into each slot; the other families
-// update H in place and copy it too.  The last slot's E and H are copied back
-// to the shot afterwards, which leaves the shot's state at the segment's end
-// as an in-place replay would.
+// kernel families H and the psi slabs go the same way (rings over the states'
+// H and psi parts, see hsel and ShotDev::pstride), so the steps write every
+// stored state themselves and nothing is copied; the other families update H
+// and psi in place and copy them into each slot.  The last slot is copied back
+// to the shot afterwards (E, or the whole state with the rings), which leaves
+// the shot's state at the segment's end as an in-place replay would.
 static int replay_segment(fwi3d_grad *g, int L)
 {
     fwi3d_shot *s = g->s;
@@ -1999,14 +2000,16 @@ static int replay_segment(fwi3d_grad *g, int L)
     if (hring) {
         g->rdesc.h = g->buf + g->off_h;
         g->rdesc.hstride = (long long)g->state;
+        for (int a = 0; a < 3; ++a)
+            g->rdesc.psi[a] = g->buf + g->off_psi[a];
+        g->rdesc.pstride = (long long)g->state;
     }
     HIP_TRY(hipMemcpyAsync(g->d_rdesc, &g->rdesc, sizeof(ShotDev), hipMemcpyHostToDevice, s->st));
     if (!g->replay[slot]) {
         Fam f = f0;
         f.rings = true;   // the stored states: an E ring of leaf + 1 slots
         const bool fold = s->nrec && rec_folds(f);
-        const mt::StateSegs segs = state_segs_from(g, hring ? 2 : 1);
-        const size_t soff = hring ? g->off_psi[0] : g->off_h;
+        const mt::StateSegs segs = state_segs_from(g, 1);   // H and psi, without the rings
         hipGraph_t graph = nullptr;
         std::lock_guard<std::recursive_mutex> lk(g_rt_mu);
         HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
@@ -2018,8 +2021,8 @@ static int replay_segment(fwi3d_grad *g, int L)
             if (s->nrec && !(fold && j + 1 < L) &&
                 mt::launch_receivers(g->d_rdesc, 1, j, -1, 0, s->st) != hipSuccess)
                 rc = fail(FWI3D_EDEVICE, "receiver launch");
-            else if (mt::launch_state_copy(segs, g->buf + g->state * (j + 1) + soff, true,
-                                           s->st) != hipSuccess)
+            else if (!hring && mt::launch_state_copy(segs, g->buf + g->state * (j + 1) + g->off_h,
+                                                     true, s->st) != hipSuccess)
                 rc = fail(FWI3D_EDEVICE, "state copy launch");
         }
         if (!rc && mt::launch_advance(g->d_rdesc, 1, L, s->st) != hipSuccess)
@@ -2036,13 +2039,9 @@ static int replay_segment(fwi3d_grad *g, int L)
         g->replay_len[slot] = L;
     }
     HIP_TRY(hipGraphLaunch(g->replay[slot], s->st));
-    const mt::StateSegs eh = state_segs(g);   // E then H, as stored
-    mt::StateSegs back{};
-    for (int k = 0; k < (hring ? 2 : 1); ++k) {
-        back.live[k] = eh.live[k];
-        back.count[k] = eh.count[k];
-    }
-    back.nseg = hring ? 2 : 1;
+    mt::StateSegs back = state_segs(g);   // the whole state with the rings, else E
+    if (!hring)
+        back.nseg = 1;
     HIP_TRY(mt::launch_state_copy(back, g->buf + g->state * L, false, s->st));
     s->step += L;
     return FWI3D_OK;

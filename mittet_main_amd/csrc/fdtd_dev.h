@@ -72,6 +72,9 @@ struct ShotDev {
     long long hstride;        // nonzero: H in ring slots like E (slot q at h + q*hstride;
                               // a gradient replay's stored states), see hsel
     double *psi[3];           // [PSI_N][slab[a]]
+    long long pstride;        // nonzero: psi in ring slots like E (slot q at psi[a] + q*pstride;
+                              // a gradient replay's stored states), read and written by the
+                              // per-cell code only
     long long slab[3];
     int src_comp;
     int src_cell;             // -1: no source

@@ -185,9 +185,12 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
         ky = py[PR_KED * iy + jj];
         kz = pz[PR_KED * iz + k];
     }
-    GLOBAL double *__restrict__ P0 = gm(S.psi[0]);
-    GLOBAL double *__restrict__ P1 = gm(S.psi[1]);
-    GLOBAL double *__restrict__ P2 = gm(S.psi[2]);
+    // psi in place, or (pstride set) read from ring slot si and written to
+    // slot so like E
+    const size_t pr = S.pstride ? (size_t)si * S.pstride : 0;
+    const size_t pw = S.pstride ? (size_t)so * S.pstride : 0;
+    const GLOBAL double *Q0 = gc(S.psi[0] + pr), *Q1 = gc(S.psi[1] + pr), *Q2 = gc(S.psi[2] + pr);
+    GLOBAL double *P0 = gm(S.psi[0] + pw), *P1 = gm(S.psi[1] + pw), *P2 = gm(S.psi[2] + pw);
     const size_t L0 = (size_t)S.slab[0], L1 = (size_t)S.slab[1], L2 = (size_t)S.slab[2];
     size_t q0 = 0, q1 = 0, q2 = 0;
     double bx = 0.0, cx = 0.0, ax0 = 0.0, ax1 = 0.0;
@@ -197,22 +200,22 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
         q0 = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
         bx = px[PR_BE * ix + i];
         cx = px[PR_CE * ix + i];
-        ax0 = P0[q0];
-        ax1 = P0[L0 + q0];
+        ax0 = Q0[q0];
+        ax1 = Q0[L0 + q0];
     }
     if (fy) {
         q1 = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
         by = py[PR_BE * iy + jj];
         cy = py[PR_CE * iy + jj];
-        ay0 = P1[q1];
-        ay1 = P1[L1 + q1];
+        ay0 = Q1[q1];
+        ay1 = Q1[L1 + q1];
     }
     if (fz) {
         q2 = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
         bz = pz[PR_BE * iz + k];
         cz = pz[PR_CE * iz + k];
-        az0 = P2[q2];
-        az1 = P2[L2 + q2];
+        az0 = Q2[q2];
+        az1 = Q2[L2 + q2];
     }
     const bool src = (MODE & M_SRC) && c == S.src_cell;
     double amp = 0.0;
@@ -437,9 +440,12 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin,
         ky = py[PR_KHD * iy + jj];
         kz = pz[PR_KHD * iz + k];
     }
-    GLOBAL double *__restrict__ P0 = gm(S.psi[0]);
-    GLOBAL double *__restrict__ P1 = gm(S.psi[1]);
-    GLOBAL double *__restrict__ P2 = gm(S.psi[2]);
+    // psi in place, or (pstride set) read from the ring slot before so and
+    // written to slot so like H
+    const size_t pr = S.pstride ? (size_t)((so + S.ring - 1) % S.ring) * S.pstride : 0;
+    const size_t pw = S.pstride ? (size_t)so * S.pstride : 0;
+    const GLOBAL double *Q0 = gc(S.psi[0] + pr), *Q1 = gc(S.psi[1] + pr), *Q2 = gc(S.psi[2] + pr);
+    GLOBAL double *P0 = gm(S.psi[0] + pw), *P1 = gm(S.psi[1] + pw), *P2 = gm(S.psi[2] + pw);
     const size_t L0 = (size_t)S.slab[0], L1 = (size_t)S.slab[1], L2 = (size_t)S.slab[2];
     size_t q0 = 0, q1 = 0, q2 = 0;
     double bx = 0.0, cx = 0.0, ax0 = 0.0, ax1v = 0.0;
@@ -449,22 +455,22 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin,
         q0 = ((size_t)k * iy + jj) * (2 * nc) + slab_pos(i, ix, nc);
         bx = px[PR_BH * ix + i];
         cx = px[PR_CH * ix + i];
-        ax0 = P0[2 * L0 + q0];
-        ax1v = P0[3 * L0 + q0];
+        ax0 = Q0[2 * L0 + q0];
+        ax1v = Q0[3 * L0 + q0];
     }
     if (gy) {
         q1 = ((size_t)k * (2 * nc) + slab_pos(jj, iy, nc)) * ix + i;
         by = py[PR_BH * iy + jj];
         cy = py[PR_CH * iy + jj];
-        ay0 = P1[2 * L1 + q1];
-        ay1v = P1[3 * L1 + q1];
+        ay0 = Q1[2 * L1 + q1];
+        ay1v = Q1[3 * L1 + q1];
     }
     if (gz) {
         q2 = ((size_t)slab_pos(k, iz, nc) * iy + jj) * ix + i;
         bz = pz[PR_BH * iz + k];
         cz = pz[PR_CH * iz + k];
-        az0 = P2[2 * L2 + q2];
-        az1v = P2[3 * L2 + q2];
+        az0 = Q2[2 * L2 + q2];
+        az1v = Q2[3 * L2 + q2];
     }
 
     // ---- arithmetic
