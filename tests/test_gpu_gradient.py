@@ -127,6 +127,24 @@ def test_odd_cell_count_gradient():
     P1.close()
 
 
+def test_gradient_with_the_x_pair_forward_family(monkeypatch):
+    """The replays of the LDS and per-cell forward families write E, H and
+    psi straight into the stored states; other families update H and psi in
+    place and copy them into each state.  With the x-pair family
+    (MITTET_FDTD_PAIR=1) the gradient is bitwise that of the default family:
+    the forward is bitwise the same and the adjoint kernels are shared."""
+    print("gradient: x-pair forward family vs default, 26 x 24 x 22", flush=True)
+    P1, sig0, _ = problem(seed=9, segment=9)
+    J1, g1 = P1.value_and_grad(np.log(sig0))
+    P1.close()
+    monkeypatch.setenv("MITTET_FDTD_PAIR", "1")
+    P2, _, _ = problem(seed=9, segment=9)
+    J2, g2 = P2.value_and_grad(np.log(sig0))
+    P2.close()
+    assert J1 == J2
+    np.testing.assert_array_equal(g1, g2)
+
+
 @pytest.mark.parametrize("levels", [2, 3, 4])
 def test_recursive_checkpoints_give_the_same_gradient(levels):
     """Recursive recomputation replays the same forward states, so misfit and
