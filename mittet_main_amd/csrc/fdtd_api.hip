@@ -322,9 +322,9 @@ bool takes_lds(const fwi3d_grid_params &p, int zchunk)
 // of more than one slot run the fused E+H interior kernel (k_ehfield_l),
 // which reads E and H of the chunk's planes k0 - 3 .. k1 + 2 (so ncpml >= 3)
 // and indexes H planes with 32-bit offsets.  It is opt-in: at 288^3 it moves
-// 104 B per interior cell instead of 152 but takes 0.60 ms against 0.49 ms
-// for k_efield_l + k_hfield_l (register-bound at two blocks per CU, two
-// barriers per plane and a 1.42x E halo recomputation).
+// 104 B per interior cell instead of 152 but takes 580 us against 531 us
+// for k_efield_l + k_hfield_l at 288^3 (register-bound at two blocks per CU,
+// two barriers per plane and a 1.42x E halo recomputation).
 bool takes_fused(const fwi3d_grid_params &p, int zchunk, int ring)
 {
     const long long N = (long long)p.ix * p.iy * p.iz;
