@@ -306,6 +306,10 @@ int lds_chunk(int ltx, int lty, int nz)
 // 216^3 18.1 / 16.5.  MITTET_FDTD_LDS=1 takes it on any grid.
 constexpr long long kLdsMinCells = 2500000;
 
+// Launches whose steps touch more bytes than the 256 MB Infinity Cache load
+// and store their once-touched field values non-temporally (stream_nt).
+constexpr long long kNtMinBytes = 256LL << 20;
+
 // Whether a C-variant shot takes the LDS family (family 3): size, no z-chunk
 // or x-pair override, and a CPML-free interior to march.
 bool takes_lds(const fwi3d_grid_params &p, int zchunk)
@@ -471,6 +475,7 @@ struct Fam {
     int nblocks_l;
     bool rings;   // every shot keeps E in a ring of more than one slot
     bool fused;   // family 3, every shot with a second H buffer: the fused E+H step
+    bool nt;      // E/H launches with non-temporal once-touched values (stream_nt)
 };
 
 // Families whose E half-step has a per-cell launch that can carry the
@@ -507,15 +512,15 @@ int launch_eh(const Fam &f, const ShotDev *d, int ns, int j, bool h, hipStream_t
         e = h ? mt::launch_hfield_p(d, ns, f.pre, f.nblocks, j, 1, st)
               : mt::launch_efield_p(d, ns, f.pre, f.nblocks, j, 1, st);
     else
-        e = h ? mt::launch_hfield_f(d, ns, f.pre, f.nblocks, j, 1, st, f.fused)
-              : mt::launch_efield_f(d, ns, f.pre, f.nblocks, j, 1, st, rec_prev, f.fused);
+        e = h ? mt::launch_hfield_f(d, ns, f.pre, f.nblocks, j, 1, st, f.fused, f.nt)
+              : mt::launch_efield_f(d, ns, f.pre, f.nblocks, j, 1, st, rec_prev, f.fused, f.nt);
     HIP_TRY(e);
     if (f.kind == 3 && f.fused) {
         if (!h)
             HIP_TRY(mt::launch_ehfield_l(d, ns, f.pre_l, f.nblocks_l, j, st));
     } else if (f.kind == 3) {
-        HIP_TRY(h ? mt::launch_hfield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st)
-                  : mt::launch_efield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st));
+        HIP_TRY(h ? mt::launch_hfield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st, f.nt)
+                  : mt::launch_efield_l(d, ns, f.pre_l, f.nblocks_l, j, 1, st, f.nt));
     }
     return FWI3D_OK;
 }
@@ -1169,10 +1174,35 @@ int fwi3d_hpml_abcs4(fwi3d_shot *s) { return single_op(s, 3, 1, 0); }
 int fwi3d_read_source_3d(fwi3d_shot *s, int step) { return single_op(s, 4, 1, step); }
 int fwi3d_output(fwi3d_shot *s, int step) { return single_op(s, 5, 1, step); }
 
+// Bytes per cell one step touches: E read, E written (to the next ring slot,
+// or in place with one slot), H, and the two coefficient arrays.
+long long step_touch_bytes(const fwi3d_shot *s)
+{
+    return (long long)s->desc.n * (s->ring > 1 ? 88 : 64);
+}
+
+// Non-temporal field traffic pays once a launch's fields no longer stay in the
+// Infinity Cache from one half-step to the next: then L2 and that cache keep
+// the stencil neighbours that other tiles still read.  Single shots with a
+// 12-cell CPML and a 4-frequency volume DFT (88 B per cell), Gcell-updates/s
+// without / with: 96^3 17.7 / 15.2, 120^3 19.0 / 15.6, 136^3 (211 MiB) 16.3 /
+// 15.5, 152^3 (295 MiB) 18.4 / 18.8, 184^3 17.9 / 19.0, 216^3 19.3 / 20.6,
+// 280^3 20.2 / 21.4-23.1; 84^3 with a 10-cell CPML 14.8 / 12.9.  Without DFT
+// (one slot, 64 B per cell): 184^3 20.3 / 21.4, 216^3 22.2 / 23.5, 280^3
+// 23.1 / 25.2, while eight 152^3 shots (214 MiB) run their forward in 4.79 s
+// without, 5.23 s with.  MITTET_FDTD_NT=0 or 1 overrides the choice.
+bool stream_nt(long long touch_bytes)
+{
+    const long long e = env_pad("MITTET_FDTD_NT", -1);
+    if (e >= 0)
+        return e != 0;
+    return touch_bytes > kNtMinBytes;
+}
+
 static Fam shot_fam(const fwi3d_shot *s)
 {
     return {s->kind, s->d_pre, s->nblocks, s->d_pre, s->nblocks_l, s->ring > 1,
-            s->kind == 3 && s->fused};
+            s->kind == 3 && s->fused, stream_nt(step_touch_bytes(s))};
 }
 
 // After `steps` steps of a fused schedule the current H is in the other buffer
@@ -2885,11 +2915,13 @@ static int batch_upload(fwi3d_batch *b)
 static Fam batch_fam(const fwi3d_batch *b)
 {
     bool rings = true, fused = b->kind == 3;
+    long long touch = 0;
     for (const auto *s : b->shots) {
         rings = rings && s->ring > 1;
         fused = fused && s->kind == 3 && s->fused;
+        touch += step_touch_bytes(s);
     }
-    return {b->kind, b->d_pre, b->nblocks, b->d_pre_l, b->nblocks_l, rings, fused};
+    return {b->kind, b->d_pre, b->nblocks, b->d_pre_l, b->nblocks_l, rings, fused, stream_nt(touch)};
 }
 
 int fwi3d_batch_step(fwi3d_batch *b, int nsteps)

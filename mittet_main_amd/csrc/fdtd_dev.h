@@ -96,7 +96,8 @@ struct ShotDev {
     const double *mu;         // variant 1: per-cell permeability
 };
 
-enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8 };
+// M_NT: once-touched field values are loaded and stored non-temporally
+enum { M_UPD = 1, M_PSI = 2, M_SRC = 4, M_INPLACE = 8, M_NT = 16 };
 
 // H before local step j of a chunk.  In a launch sequence that updates H in
 // place (hf = 0) that is always the current buffer.  The fused E+H launches
@@ -161,19 +162,21 @@ hipError_t launch_hfield_z(const ShotDev *d, int ns, const int *pre, int nblocks
                            int sign, hipStream_t st);
 // rec_prev (j > 0): ns more blocks gather the receivers of local step j - 1.
 // hf: part of a fused E+H step (H read from hsel(S, j, 1), written to
-// hsel(S, j + 1, 1)); 0: H updated in place.
+// hsel(S, j + 1, 1)); 0: H updated in place.  nt (also for the _l launches):
+// the once-touched field values go non-temporal (M_NT).
 hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
-                           int sign, hipStream_t st, bool rec_prev = false, int hf = 0);
+                           int sign, hipStream_t st, bool rec_prev = false, int hf = 0,
+                           bool nt = false);
 hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
-                           int sign, hipStream_t st, int hf = 0);
+                           int sign, hipStream_t st, int hf = 0, bool nt = false);
 hipError_t launch_efield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_hfield_p(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
                            int sign, hipStream_t st);
 hipError_t launch_efield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
-                           int sign, hipStream_t st);
+                           int sign, hipStream_t st, bool nt = false);
 hipError_t launch_hfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j,
-                           int sign, hipStream_t st);
+                           int sign, hipStream_t st, bool nt = false);
 // Fused E+H step over the LDS region (same blocks as launch_efield_l): after
 // launch_efield_f(hf = 1), before launch_hfield_f(hf = 1); shots with h2 and a
 // ring of more than one slot

@@ -44,6 +44,26 @@ __device__ __forceinline__ const GLOBAL int *gc(const int *p) { return (const GL
 __device__ __forceinline__ const GLOBAL double *gc(const GLOBAL double *p) { return p; }
 __device__ __forceinline__ GLOBAL double *gm(GLOBAL double *p) { return p; }
 
+// Field values each half-step reads or writes once (own E or H, coefficient,
+// the updated field, psi).  With NT they are non-temporal, so they do not
+// evict the stencil neighbours that other tiles still read from L2 and the
+// Infinity Cache; the host sets NT only for launches whose fields do not fit
+// that cache anyway.
+template <bool NT> __device__ __forceinline__ double ld_s(const GLOBAL double *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+template <bool NT> __device__ __forceinline__ void st_s(GLOBAL double *p, double v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 __device__ __forceinline__ double dminus(const GLOBAL double *__restrict__ F, int c, int s,
                                          double c1, double c2)
 {
@@ -103,6 +123,7 @@ template <int MODE>
 __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, int i, int jj, int k,
                                             int n, int si, int so, int sign)
 {
+    constexpr bool NT = (MODE & M_NT) != 0;
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc, N = S.n;
     const int sy = ix, sz = ix * iy;
     const int c = (k * iy + jj) * ix + i;
@@ -133,7 +154,7 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
                     i < ix - 1 && jj < iy - 1;
 
     // ---- loads
-    double ex = Ein[c], ey = Ein[N + c], ez = Ein[2 * (size_t)N + c];
+    double ex = ld_s<NT>(Ein + c), ey = ld_s<NT>(Ein + N + c), ez = ld_s<NT>(Ein + 2 * (size_t)N + c);
     // every backward stencil in bounds: all six derivatives are loaded
     const bool all = rx2 && ry2 && rz2;
     const bool nzy = all || upx || fy, nyz = all || upx || fz, nxz = all || upy || fz;
@@ -297,20 +318,20 @@ __device__ __forceinline__ void efield_cell(const ShotDev &S, const double *H, i
             ez -= amp;
     }
     if (fx) {
-        P0[q0] = px0;
-        P0[L0 + q0] = px1;
+        st_s<NT>(P0 + q0, px0);
+        st_s<NT>(P0 + L0 + q0, px1);
     }
     if (fy) {
-        P1[q1] = py0;
-        P1[L1 + q1] = py1;
+        st_s<NT>(P1 + q1, py0);
+        st_s<NT>(P1 + L1 + q1, py1);
     }
     if (fz) {
-        P2[q2] = pz0;
-        P2[L2 + q2] = pz1;
+        st_s<NT>(P2 + q2, pz0);
+        st_s<NT>(P2 + L2 + q2, pz1);
     }
-    Eout[c] = ex;
-    Eout[N + c] = ey;
-    Eout[2 * (size_t)N + c] = ez;
+    st_s<NT>(Eout + c, ex);
+    st_s<NT>(Eout + N + c, ey);
+    st_s<NT>(Eout + 2 * (size_t)N + c, ez);
 }
 
 
@@ -356,6 +377,7 @@ template <int MODE>
 __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin, double *Hout, int i,
                                             int jj, int k, int so, int sign)
 {
+    constexpr bool NT = (MODE & M_NT) != 0;
     const int ix = S.ix, iy = S.iy, iz = S.iz, nc = S.nc, N = S.n;
     const int sy = ix, sz = ix * iy;
     const int c = (k * iy + jj) * ix + i;
@@ -433,7 +455,8 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin,
     // eps' is only used where a magnetic CPML conductivity is nonzero; loading
     // it beside the conductivities rather than after them saves a round trip
     const double eps2 = gc(S.eps)[c];
-    double hx = Hin[c], hy = Hin[N + c], hz = Hin[2 * (size_t)N + c];
+    const GLOBAL double *__restrict__ Hi = gc(Hin);
+    double hx = ld_s<NT>(Hi + c), hy = ld_s<NT>(Hi + N + c), hz = ld_s<NT>(Hi + 2 * (size_t)N + c);
     double kx = 1.0, ky = 1.0, kz = 1.0;
     if (MODE & M_UPD) {
         kx = px[PR_KHD * ix + i];
@@ -537,21 +560,21 @@ __device__ __forceinline__ void hfield_cell(const ShotDev &S, const double *Hin,
         hy -= dby * pz1;
     }
     if (gx) {
-        P0[2 * L0 + q0] = px0;
-        P0[3 * L0 + q0] = px1;
+        st_s<NT>(P0 + 2 * L0 + q0, px0);
+        st_s<NT>(P0 + 3 * L0 + q0, px1);
     }
     if (gy) {
-        P1[2 * L1 + q1] = py0;
-        P1[3 * L1 + q1] = py1;
+        st_s<NT>(P1 + 2 * L1 + q1, py0);
+        st_s<NT>(P1 + 3 * L1 + q1, py1);
     }
     if (gz) {
-        P2[2 * L2 + q2] = pz0;
-        P2[3 * L2 + q2] = pz1;
+        st_s<NT>(P2 + 2 * L2 + q2, pz0);
+        st_s<NT>(P2 + 3 * L2 + q2, pz1);
     }
     GLOBAL double *__restrict__ Ho = gm(Hout);
-    Ho[c] = hx;
-    Ho[N + c] = hy;
-    Ho[2 * (size_t)N + c] = hz;
+    st_s<NT>(Ho + c, hx);
+    st_s<NT>(Ho + N + c, hy);
+    st_s<NT>(Ho + 2 * (size_t)N + c, hz);
 }
 
 
@@ -1356,6 +1379,7 @@ __device__ __forceinline__ bool fcell(const ShotDev &S, int b, int &i, int &jj, 
 // launch after step j - 1 takes, with one graph node less per step.  Leading,
 // they overlap the tile blocks instead of trailing the launch.  hf: the step
 // is a fused E+H launch sequence (H before step j is hsel(S, j, 1)).
+template <bool NT>
 __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__ D, int ns,
                                                      const int *__restrict__ pre, int j, int sign,
                                                      int recn, int hf)
@@ -1383,7 +1407,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
     const bool inner = i >= nc && i < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
     const double *Hc = hsel(S, j, hf);
     if (!__all(inner)) {
-        efield_cell<M_UPD | M_PSI | M_SRC>(S, Hc, i, jj, k, n, si, so, sign);
+        efield_cell<M_UPD | M_PSI | M_SRC | (NT ? M_NT : 0)>(S, Hc, i, jj, k, n, si, so, sign);
         return;
     }
     const size_t N = (size_t)S.n;
@@ -1394,7 +1418,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
     const GLOBAL double *__restrict__ Hx = gc(Hc);
     const GLOBAL double *__restrict__ Hy = gc(Hc + N);
     const GLOBAL double *__restrict__ Hz = gc(Hc + 2 * N);
-    const double ex0 = Ein[c], ey0 = Ein[N + c], ez0 = Ein[2 * N + c], cb = gc(S.cbi)[c];
+    const double ex0 = ld_s<NT>(Ein + c), ey0 = ld_s<NT>(Ein + N + c), ez0 = ld_s<NT>(Ein + 2 * N + c);
+    const double cb = ld_s<NT>(gc(S.cbi) + c);
     const double z0 = Hz[c], zym1 = Hz[c - sy], zyp1 = Hz[c + sy], zym2 = Hz[c - 2 * sy];
     const double zxm1 = Hz[c - 1], zxp1 = Hz[c + 1], zxm2 = Hz[c - 2];
     const double y0 = Hy[c], yzm1 = Hy[c - sz], yzp1 = Hy[c + sz], yzm2 = Hy[c - 2 * sz];
@@ -1426,11 +1451,12 @@ __global__ __launch_bounds__(ZX *ZY) void k_efield_f(const ShotDev *__restrict__
         else
             ez -= a;
     }
-    Eout[c] = ex;
-    Eout[N + c] = ey;
-    Eout[2 * N + c] = ez;
+    st_s<NT>(Eout + c, ex);
+    st_s<NT>(Eout + N + c, ey);
+    st_s<NT>(Eout + 2 * N + c, ez);
 }
 
+template <bool NT>
 __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__ D, int ns,
                                                      const int *__restrict__ pre, int j, int sign,
                                                      int hf)
@@ -1447,7 +1473,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
     const bool inner = i >= nc && i < ix - nc && jj >= nc && jj < iy - nc && k >= nc && k < iz - nc;
     double *Hi = hsel(S, j, hf), *Ho = hsel(S, j + 1, hf);
     if (!__all(inner)) {
-        hfield_cell<M_UPD | M_PSI>(S, Hi, Ho, i, jj, k, so, sign);
+        hfield_cell<M_UPD | M_PSI | (NT ? M_NT : 0)>(S, Hi, Ho, i, jj, k, so, sign);
         return;
     }
     const size_t N = (size_t)S.n;
@@ -1462,7 +1488,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
     GLOBAL double *__restrict__ Hx = gm(Ho);
     GLOBAL double *__restrict__ Hy = gm(Ho + N);
     GLOBAL double *__restrict__ Hz = gm(Ho + 2 * N);
-    const double hx0 = Hxi[c], hy0 = Hyi[c], hz0 = Hzi[c];
+    const double hx0 = ld_s<NT>(Hxi + c), hy0 = ld_s<NT>(Hyi + c), hz0 = ld_s<NT>(Hzi + c);
     const double z0 = Ez[c], zyp1 = Ez[c + sy], zyp2 = Ez[c + 2 * sy], zym1 = Ez[c - sy];
     const double zxp1 = Ez[c + 1], zxp2 = Ez[c + 2], zxm1 = Ez[c - 1];
     const double y0 = Ey[c], yzp1 = Ey[c + sz], yzp2 = Ey[c + 2 * sz], yzm1 = Ey[c - sz];
@@ -1483,28 +1509,35 @@ __global__ __launch_bounds__(ZX *ZY) void k_hfield_f(const ShotDev *__restrict__
     const double cux = ezy / khdy - eyz / khdz;
     const double cuy = exz / khdz - ezx / khdx;
     const double cuz = eyx / khdx - exy / khdy;
-    Hx[c] = sign > 0 ? hx0 - db * cux : hx0 + db * cux;
-    Hy[c] = sign > 0 ? hy0 - db * cuy : hy0 + db * cuy;
-    Hz[c] = sign > 0 ? hz0 - db * cuz : hz0 + db * cuz;
+    st_s<NT>(Hx + c, sign > 0 ? hx0 - db * cux : hx0 + db * cux);
+    st_s<NT>(Hy + c, sign > 0 ? hy0 - db * cuy : hy0 + db * cuy);
+    st_s<NT>(Hz + c, sign > 0 ? hz0 - db * cuz : hz0 + db * cuz);
 }
 
 hipError_t launch_efield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
-                           hipStream_t st, bool rec_prev, int hf)
+                           hipStream_t st, bool rec_prev, int hf, bool nt)
 {
     if (nblocks <= 0)
         return rec_prev ? hipErrorInvalidValue : hipSuccess;
     const int recn = rec_prev && j > 0 ? (ns + 7) & ~7 : 0;
-    hipLaunchKernelGGL(k_efield_f, dim3(recn + nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign,
-                       recn, hf);
+    if (nt)
+        hipLaunchKernelGGL(k_efield_f<true>, dim3(recn + nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j,
+                           sign, recn, hf);
+    else
+        hipLaunchKernelGGL(k_efield_f<false>, dim3(recn + nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j,
+                           sign, recn, hf);
     return hipGetLastError();
 }
 
 hipError_t launch_hfield_f(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
-                           hipStream_t st, int hf)
+                           hipStream_t st, int hf, bool nt)
 {
     if (nblocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_hfield_f, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign, hf);
+    if (nt)
+        hipLaunchKernelGGL(k_hfield_f<true>, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign, hf);
+    else
+        hipLaunchKernelGGL(k_hfield_f<false>, dim3(nblocks), dim3(ZX * ZY), 0, st, d, ns, pre, j, sign, hf);
     return hipGetLastError();
 }
 
@@ -1576,6 +1609,7 @@ __device__ __forceinline__ bool lcol(const ShotDev &S, int b, int &x0, int &y0, 
 
 }  // namespace
 
+template <bool NT>
 __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D, int ns,
                                                   const int *__restrict__ pre, int j, int sign)
 {
@@ -1624,7 +1658,8 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
     tile_load<TY, 2, 0>(Hy, (size_t)(k0 + 1) * pl, ix, x0, y0, xmax, ymax, ry);
     tile_load<TX, 0, 2>(Hx, (size_t)(k0 + 1) * pl, ix, x0, y0, xmax, ymax, rx);
     size_t c = (size_t)k0 * pl + col;
-    double ex0 = Ein[c], ey0 = Ein[N + c], ez0 = Ein[2 * N + c], cb = CB[c];
+    double ex0 = ld_s<NT>(Ein + c), ey0 = ld_s<NT>(Ein + N + c), ez0 = ld_s<NT>(Ein + 2 * N + c);
+    double cb = ld_s<NT>(CB + c);
     for (int k = k0; k < k1; ++k) {
         tile_store<TZ>(sz[b1], rz);
         tile_store<TY>(sy[b1], ry);
@@ -1637,10 +1672,10 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
             tile_load<TZ, 2, 2>(Hz, (size_t)(k + 2) * pl, ix, x0, y0, xmax, ymax, rz);
             tile_load<TY, 2, 0>(Hy, (size_t)(k + 2) * pl, ix, x0, y0, xmax, ymax, ry);
             tile_load<TX, 0, 2>(Hx, (size_t)(k + 2) * pl, ix, x0, y0, xmax, ymax, rx);
-            nex = Ein[cn];
-            ney = Ein[N + cn];
-            nez = Ein[2 * N + cn];
-            ncb = CB[cn];
+            nex = ld_s<NT>(Ein + cn);
+            ney = ld_s<NT>(Ein + N + cn);
+            nez = ld_s<NT>(Ein + 2 * N + cn);
+            ncb = ld_s<NT>(CB + cn);
         }
         const double kedz = uload(S.prof[2] + PR_KED * S.iz + k);
         const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
@@ -1675,9 +1710,9 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
                 else
                     ez -= a;
             }
-            Eout[c] = ex;
-            Eout[N + c] = ey;
-            Eout[2 * N + c] = ez;
+            st_s<NT>(Eout + c, ex);
+            st_s<NT>(Eout + N + c, ey);
+            st_s<NT>(Eout + 2 * N + c, ez);
         }
         yq2 = yq1;
         yq1 = y0v;
@@ -1695,6 +1730,7 @@ __global__ __launch_bounds__(LNT) void k_efield_l(const ShotDev *__restrict__ D,
     }
 }
 
+template <bool NT>
 __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D, int ns,
                                                   const int *__restrict__ pre, int j, int sign)
 {
@@ -1746,7 +1782,7 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
     tile_load<TY, 1, 0>(Ey, (size_t)(k0 + 2) * pl, ix, x0, y0, xmax, ymax, ry);
     tile_load<TX, 0, 1>(Ex, (size_t)(k0 + 2) * pl, ix, x0, y0, xmax, ymax, rx);
     size_t c = (size_t)k0 * pl + col;
-    double hx0 = Hxi[c], hy0 = Hyi[c], hz0 = Hzi[c];
+    double hx0 = ld_s<NT>(Hxi + c), hy0 = ld_s<NT>(Hyi + c), hz0 = ld_s<NT>(Hzi + c);
     for (int k = k0; k < k1; ++k) {
         tile_store<TZ>(sz[b2], rz);
         tile_store<TY>(sy[b2], ry);
@@ -1759,9 +1795,9 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
             tile_load<TZ, 1, 1>(Ez, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, rz);
             tile_load<TY, 1, 0>(Ey, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, ry);
             tile_load<TX, 0, 1>(Ex, (size_t)(k + 3) * pl, ix, x0, y0, xmax, ymax, rx);
-            nhx = Hxi[cn];
-            nhy = Hyi[cn];
-            nhz = Hzi[cn];
+            nhx = ld_s<NT>(Hxi + cn);
+            nhy = ld_s<NT>(Hyi + cn);
+            nhz = ld_s<NT>(Hzi + cn);
         }
         const double khdz = uload(S.prof[2] + PR_KHD * S.iz + k);
         const double *Z = sz[b0], *Y = sy[b0], *X = sx[b0];
@@ -1784,9 +1820,9 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
         const double cuy = exz / khdz - ezx / khdx;
         const double cuz = eyx / khdx - exy / khdy;
         if (valid) {
-            Hx[c] = sign > 0 ? hx0 - db * cux : hx0 + db * cux;
-            Hy[c] = sign > 0 ? hy0 - db * cuy : hy0 + db * cuy;
-            Hz[c] = sign > 0 ? hz0 - db * cuz : hz0 + db * cuz;
+            st_s<NT>(Hx + c, sign > 0 ? hx0 - db * cux : hx0 + db * cux);
+            st_s<NT>(Hy + c, sign > 0 ? hy0 - db * cuy : hy0 + db * cuy);
+            st_s<NT>(Hz + c, sign > 0 ? hz0 - db * cuz : hz0 + db * cuz);
         }
         yq1 = y0v;
         xq1 = x0v;
@@ -1803,20 +1839,26 @@ __global__ __launch_bounds__(LNT) void k_hfield_l(const ShotDev *__restrict__ D,
 }
 
 hipError_t launch_efield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
-                           hipStream_t st)
+                           hipStream_t st, bool nt)
 {
     if (nblocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_efield_l, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
+    if (nt)
+        hipLaunchKernelGGL(k_efield_l<true>, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
+    else
+        hipLaunchKernelGGL(k_efield_l<false>, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
     return hipGetLastError();
 }
 
 hipError_t launch_hfield_l(const ShotDev *d, int ns, const int *pre, int nblocks, int j, int sign,
-                           hipStream_t st)
+                           hipStream_t st, bool nt)
 {
     if (nblocks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_hfield_l, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
+    if (nt)
+        hipLaunchKernelGGL(k_hfield_l<true>, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
+    else
+        hipLaunchKernelGGL(k_hfield_l<false>, dim3(nblocks), dim3(LNT), 0, st, d, ns, pre, j, sign);
     return hipGetLastError();
 }
 

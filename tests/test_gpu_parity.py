@@ -92,13 +92,15 @@ def test_backprop_sign_variants_bitwise():
         np.testing.assert_array_equal(shot.field(q), S.arr(nm), err_msg=nm)
 
 
-@pytest.mark.parametrize("family", ["lds", "auto"])
+@pytest.mark.parametrize("family", ["lds", "auto", "lds-nt", "auto-nt"])
 @pytest.mark.parametrize("ring,pieces", [(0, None), (8, (5, 3, 19, 8, 100)), (1, (7, 50, 100))])
 def test_fused_run_bitwise(ring, pieces, family, monkeypatch):
     """Forced LDS family, and the size-based choice (per-cell kernels alone on
-    this small grid)."""
-    if family == "auto":
+    this small grid); "-nt": the kernels with non-temporal field loads and
+    stores that grids beyond the Infinity Cache take (MITTET_FDTD_NT=1)."""
+    if family.startswith("auto"):
         monkeypatch.delenv("MITTET_FDTD_LDS", raising=False)
+    monkeypatch.setenv("MITTET_FDTD_NT", "1" if family.endswith("-nt") else "0")
     it = 90
     m, shot, S, sig, signal, src, rec, om, dt = setup_pair(seed=5, it=it, ring=ring)
     if pieces is None:
@@ -156,12 +158,15 @@ def test_laplace_to_freq_vs_oracle():
             np.testing.assert_allclose(got["J"], want["J"], rtol=1e-10)
 
 
-@pytest.mark.parametrize("families", ["lds", "mixed"])
+@pytest.mark.parametrize("families", ["lds", "mixed", "lds-nt"])
 def test_batch_mixed_sizes_equal_alone(families, monkeypatch):
     """Shots of different grid sizes in one persistent launch give the same
     bits as each run alone; one joins mid-way (continuous batching).  "mixed":
     the second shot runs the per-cell family alone (as grids below 2.5 M cells
-    do), the others the LDS family, in the same launches."""
+    do), the others the LDS family, in the same launches.  "lds-nt": the
+    batch launches load and store non-temporally (MITTET_FDTD_NT=1), the
+    lone runs do not."""
+    monkeypatch.setenv("MITTET_FDTD_NT", "0")
     specs = [dict(ix=30, iy=26, iz=22, seed=10, it=70), dict(ix=24, iy=28, iz=20, seed=11, it=50),
              dict(ix=34, iy=22, iz=26, seed=12, it=90)]
 
@@ -178,6 +183,8 @@ def test_batch_mixed_sizes_equal_alone(families, monkeypatch):
         shot.run()
         alone.append((shot.traces(), shot.volume_dft(), [shot.field(q) for q in range(6)]))
     keep = []
+    if families == "lds-nt":
+        monkeypatch.setenv("MITTET_FDTD_NT", "1")
     b = Batch(8, chunk=8)
     for q, sp in enumerate(specs):
         m, shot, *_ = make(q, sp)
