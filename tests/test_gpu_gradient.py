@@ -145,6 +145,24 @@ def test_gradient_with_the_x_pair_forward_family(monkeypatch):
     np.testing.assert_array_equal(g1, g2)
 
 
+def test_gradient_with_non_temporal_forward_kernels(monkeypatch):
+    """Grids whose steps outgrow the Infinity Cache run the forward and its
+    replays with non-temporal field loads and stores (MITTET_FDTD_NT=1 forces
+    them here).  A cache hint only: misfit and gradient are bitwise those of
+    the cached kernels, with the replays writing into the stored states."""
+    print("gradient: non-temporal forward kernels vs cached, 26 x 24 x 22", flush=True)
+    monkeypatch.setenv("MITTET_FDTD_NT", "0")
+    P1, sig0, _ = problem(seed=13, segment=9)
+    J1, g1 = P1.value_and_grad(np.log(sig0))
+    P1.close()
+    monkeypatch.setenv("MITTET_FDTD_NT", "1")
+    P2, _, _ = problem(seed=13, segment=9)
+    J2, g2 = P2.value_and_grad(np.log(sig0))
+    P2.close()
+    assert J1 == J2
+    np.testing.assert_array_equal(g1, g2)
+
+
 @pytest.mark.parametrize("levels", [2, 3, 4])
 def test_recursive_checkpoints_give_the_same_gradient(levels):
     """Recursive recomputation replays the same forward states, so misfit and
