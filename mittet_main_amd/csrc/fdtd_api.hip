@@ -2038,6 +2038,10 @@ static int replay_segment(fwi3d_grad *g, int L)
     if (!g->replay[slot]) {
         Fam f = f0;
         f.rings = true;   // the stored states: an E ring of leaf + 1 slots
+        // a replay step reads one stored state and writes the next: E and H
+        // (and psi) both ways plus the coefficients, 112 B per cell (88 where H
+        // is updated in place and copied)
+        f.nt = stream_nt((long long)s->desc.n * (hring ? 112 : 88));
         const bool fold = s->nrec && rec_folds(f);
         const mt::StateSegs segs = state_segs_from(g, 1);   // H and psi, without the rings
         hipGraph_t graph = nullptr;
